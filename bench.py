@@ -1,0 +1,150 @@
+"""Throughput of the simulation step on one MI355X: cell-updates/s and the HBM roofline of the hydro kernel.
+
+python bench.py [--steps K] [--warmup W] [--config 2] [--no-cpu]
+
+Workload (default): the 512^3 ionized cliff wall (n=100 wall with a tanh edge, lognormal sigma 0.4, 30 dense
+globules, one star 0.4 above the face), FP32, HLL + minmod MUSCL, RK2, CFL 0.3, ionization every 2 steps with
+1024x2048 rays at half-voxel steps. Inputs are generated on the host before timing and resident in HBM.
+
+One step = Sim.step(1): the ray casts and the fused ionize pass on ionization steps, the CFL readback, and the
+two fused RK stages. value = cells * K / wall time of K steps (ionization amortised over its cadence).
+
+roofline: the dominant kernel (the fused RK stage). Algorithmic bytes per cell per RK2 step: 88 (stage 1 reads
+U + xi and writes U1: 36 B; stage 2 reads U1 + xi + U^n and writes U^n+1: 52 B), i.e. 44 B per cell per launch
+on average; achieved = 44 B x cells / (mean launch time from HIP events on the launch stream).
+cpu_baseline: the C++ oracle (oracle/, a restatement of the reference kernels; the reference's Warp CPU device
+cannot run here) on the same 512^3 grid: one ionization and one hydro step on all host cores.
+"""
+import argparse
+import json
+import math
+import os
+import re
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_CELL_STEP = 88.0      # 5 S_U + 2 S_xi with S_U = 16, S_xi = 4
+BYTES_PER_CELL_LAUNCH = BYTES_PER_CELL_STEP / 2
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def measured_traffic(kernel="hydro_stage_kernel"):
+    """HBM bytes per launch of the kernel from the committed rocprofv3 --pmc summary (null if absent)."""
+    path = os.path.join(ROOT, "profiles", "hydro_stage_wall512_traffic.md")
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"hbm_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read())
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(setup, ion_every):
+    import numpy as np
+    from nebulaviewer_amd import scenarios as S
+    from oracle.oracle import OracleSim
+    threads = min(16, os.cpu_count() or 1)
+    C = S.constants(setup)
+    t0 = time.perf_counter()
+    ref = OracleSim(setup, C, S.ray_grid(setup.shape[2]), S.fuv_grid(setup.shape[2]), np.float32, threads=threads)
+    t_ion = time.perf_counter() - t0          # construction = the first ionize() (casts + per-cell passes)
+    ref.it = 1                                # next step is not an ionization step: time dt + 2 stages alone
+    t0 = time.perf_counter()
+    ref.step(1)
+    t_hydro = time.perf_counter() - t0
+    n = math.prod(setup.shape)
+    per_step = t_hydro + t_ion / ion_every
+    return dict(value=n / per_step, unit="cell-updates/s", cores=threads, kind="port",
+                sample=f"C++ oracle (float, -O3, OpenMP {threads} threads) on the same {setup.shape[0]}^3 grid: one "
+                       f"ionization ({t_ion:.2f} s) and one CFL + two-stage hydro step ({t_hydro:.2f} s), "
+                       f"per step = hydro + ionization/{ion_every}")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--config", type=int, default=2)
+    ap.add_argument("--n", type=int, default=None, help="override the grid size (testing only)")
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not supported (the step has no "
+              "multi-GPU path)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from nebulaviewer_amd import scenarios as S
+    from nebulaviewer_amd.sim import Sim
+    from nebulaviewer_amd.timing import KernelTimer
+
+    scenario, n, seed = S.CONFIGS[a.config]
+    n = a.n or n
+    t0 = time.perf_counter()
+    setup = S.make_setup(scenario, n, seed)
+    log(f"[bench] {scenario} {n}^3 initial conditions in {time.perf_counter() - t0:.1f} s")
+    sim = Sim(setup=setup, verbose=False)
+    cells = math.prod(sim.shape)
+    log(f"[bench] warmup {a.warmup} steps")
+    sim.step(a.warmup)
+    torch.cuda.synchronize()
+
+    timer = KernelTimer()
+    sim.timer = timer
+    log(f"[bench] timing {a.steps} steps")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sim.step(a.steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    sim.timer = None
+    ks = timer.summary()
+    hyd = ks["hydro_stage"]
+    hyd_launch_s = hyd["avg_ms"] / 1e3
+    achieved = BYTES_PER_CELL_LAUNCH * cells / hyd_launch_s / 1e9
+    traffic = measured_traffic()
+    value = cells * a.steps / wall
+    hydro_rate = cells / (2 * hyd_launch_s)
+    log(f"[bench] {value / 1e9:.2f} G cell-updates/s full step, hydro {hydro_rate / 1e9:.2f} G/s, "
+        f"hydro launch {hyd['avg_ms']:.3f} ms = {achieved:.0f} GB/s; kernels {json.dumps(ks)}")
+
+    cpu = None
+    if not a.no_cpu:
+        log("[bench] CPU baseline (C++ oracle)")
+        cpu = cpu_baseline(setup, sim.ion_every)
+        log(f"[bench] CPU baseline {cpu['value'] / 1e6:.2f} M cell-updates/s")
+
+    out = {
+        "metric": "cell-updates/s at 512^3 (FP32 step) and % of HBM peak",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": wall / a.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded initial conditions generated on the host)",
+        "config": {"workload": f"{scenario} {n}^3 FP32, HLL+minmod RK2, CFL 0.3, ionization every "
+                               f"{sim.ion_every} steps with {sim.rfronts.shape[1]}x{sim.rfronts.shape[2]} rays "
+                               f"(dr = dx/2), seed {seed}", "cells": cells, "stars": len(sim.stars)},
+        "roofline": {"bound": "hbm", "kernel": "hydro_stage_kernel", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic, "algorithmic_bytes_per_launch": BYTES_PER_CELL_LAUNCH * cells},
+        "hydro_only_cell_updates_per_s": hydro_rate,
+        "kernels_ms": {k: round(v["avg_ms"], 4) for k, v in ks.items()},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

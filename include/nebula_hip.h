@@ -1,0 +1,110 @@
+/* nebula_hip.h - C ABI of the MI355X (gfx950) simulation-step kernels of nebulaviewer_amd.
+ *
+ * Every entry point:
+ *   - takes device pointers owned by the caller (plain float / uint16_t buffers, C-order [nx][ny][nz] with
+ *     z fastest, the layout of sim.py:721-733), grid sizes as ints and a hipStream_t passed as void*;
+ *   - launches asynchronously on that stream and returns NV_OK (0) or a nonzero code; nv_last_error()
+ *     then holds a message for the calling thread. Nothing throws across this boundary;
+ *   - is stateless (no allocations, no globals written) and reentrant per stream.
+ * Scalars follow the reference launch semantics: the caller computes them in double (Python floats) and
+ * passes them rounded to float, exactly as a Warp launch rounds its float arguments.
+ *
+ * Star tables: stars = float4[nstars] of (x, y, z, w) with w = Q_eff / q_main, 0 for a star not yet on
+ * (sim.py:914-925); jets = float4[nstars] of (axis x, y, z, strength).
+ */
+#ifndef NEBULA_HIP_H
+#define NEBULA_HIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NV_OK 0
+#define NV_ERR_ARG 1001    /* invalid argument (null pointer, bad size, bad enum) */
+#define NV_ERR_LAUNCH 1002 /* the kernel launch failed (message in nv_last_error) */
+
+#define NV_SOLVER_RUSANOV 0 /* nebula3d.face_flux (nebula3d.py:54-72) */
+#define NV_SOLVER_HLL 1     /* sim.flux_hll (sim.py:397-420) */
+#define NV_LIM_MINMOD 0     /* sim.py:1022 limiter codes */
+#define NV_LIM_VANLEER 1
+#define NV_LIM_MC 2
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* nv_last_error(void);
+/* ABI version of this library (bumped on any signature change). */
+int nv_abi_version(void);
+
+/* One RK stage of the fused finite-volume update: MUSCL reconstruction, face fluxes on x/y/z, wind and jet
+ * sources, out = a*U0 + b*(U + dt*dU), density floor, |v| <= vmax.
+ * Replaces the launches of hydro_stage_hll (sim.py:423-498, solver=HLL) and nebula3d.hydro_stage
+ * (nebula3d.py:76-151, solver=Rusanov) made by Sim._hydro (sim.py:1014-1028).
+ * In-place use as in sim.py:1026-1028 is allowed: out may alias U0 (each cell reads its own U0 before
+ * writing it); out must not alias U. U0 may alias U (stage 1).
+ * smax_out: if non-null, the kernel also max-reduces |u|+|v|+|w|+c of the cells it writes into smax_out[0]
+ * (float bits, atomic; the caller zeroes it), which equals nv_max_speed on the output fields and xi. */
+int nv_hydro_stage(const float* rho, const float* mx, const float* my, const float* mz, const float* xi,
+                   const float* r0, const float* mx0, const float* my0, const float* mz0, float* o_r,
+                   float* o_mx, float* o_my, float* o_mz, int nx, int ny, int nz, const float* stars,
+                   int nstars, float wind, const float* jets, float jet_cos, float jet_r, float pulse, float dt,
+                   float dx, float a, float b, float floor_, float cn2, float ci2, float vmax, int solver,
+                   int lim, float* smax_out, void* stream);
+
+/* CFL signal speed: out[0] = max(out[0], max over cells of |u|+|v|+|w| + sqrt(cn2 + xi (ci2 - cn2))), raw rho.
+ * Replaces nebula3d.max_speed (nebula3d.py:154-162) as launched by Sim._compute_dt (sim.py:1009-1012). */
+int nv_max_speed(const float* rho, const float* mx, const float* my, const float* mz, const float* xi, int nx,
+                 int ny, int nz, float cn2, float ci2, float* out, void* stream);
+
+/* Stromgren front radius per direction: rfront[nu][nphi]. Replaces nebula3d.cast_rays (nebula3d.py:247-270),
+ * launched per star by Sim.ionize (sim.py:936-937). */
+int nv_cast_rays(const float* rho, int nx, int ny, int nz, float sx, float sy, float sz, float Q, float dr,
+                 int nsteps, float dx, float rcap, float bx, float by, float bz, float* rfront, int nu, int nphi,
+                 void* stream);
+
+/* Per-cell front lookup: xi = max(xi, ramp), dfront = min(dfront, r - rf). Replaces nebula3d.apply_front
+ * (nebula3d.py:281-307), launched per star by Sim.ionize (sim.py:938). */
+int nv_apply_front(const float* rfront, int nu, int nphi, float sx, float sy, float sz, float dx, float width,
+                   float* xi, float* dfront, int nx, int ny, int nz, void* stream);
+
+/* dst = src/2 + (mean of the 6 clamped neighbours)/2. Replaces nebula3d.smooth_xi (nebula3d.py:310-315). */
+int nv_smooth_xi(const float* src, float* dst, int nx, int ny, int nz, void* stream);
+
+/* float32 -> float16 (round to nearest even). Replaces sim.to_half (sim.py:511-514). */
+int nv_to_half(const float* src, uint16_t* dst, long long n, void* stream);
+
+/* Fill n floats with v (xi.zero_(), df.fill_(1e3) of sim.py:930-931, smax.zero_() of sim.py:1010). */
+int nv_fill_f32(float* dst, float v, long long n, void* stream);
+
+/* The whole per-cell part of Sim.ionize (sim.py:930-943) in one pass: xi = 0 and df = 1e3, then apply_front
+ * for each of the nstars stars in order (rfronts = float[nstars][nu][nphi] from nv_cast_rays), then smooth_xi
+ * and the copy back into xi, then the fp16 store of dfront. Bit-identical to that launch sequence.
+ * star_xyz = float[3*nstars]. dfront_h (fp16) or dfront_f (fp32, P['fp32_fields']): exactly one non-null.
+ * If smax_out is non-null, the pass also reduces the CFL signal speed of (rho, m) with the new xi into
+ * smax_out[0] (float bits, atomic max; the caller zeroes it); rho..mz must then be non-null. */
+int nv_ionize_apply(const float* rfronts, int nu, int nphi, const float* star_xyz, int nstars, float dx,
+                    float width, float* xi, uint16_t* dfront_h, float* dfront_f, int nx, int ny, int nz,
+                    const float* rho, const float* mx, const float* my, const float* mz, float cn2, float ci2,
+                    float* smax_out, void* stream);
+
+/* FUV dust attenuation att[nr][nu][nphi] along rays from one star. Replaces nebula3d.cast_fuv
+ * (nebula3d.py:320-352), launched per star by Sim.fuv_field (sim.py:964-965). */
+int nv_cast_fuv(const float* rho, const float* xi, int nx, int ny, int nz, float sx, float sy, float sz,
+                float kappa, float dr, int nsteps, float dx, float binr, float bx, float by, float bz, float* att,
+                int nr, int nu, int nphi, void* stream);
+
+/* g0 += lum * trilinear(att) / max(r^2, 4e-4). Replaces nebula3d.apply_fuv (nebula3d.py:355-388). */
+int nv_apply_fuv(const float* att, int nr, int nu, int nphi, float sx, float sy, float sz, float lum, float dx,
+                 float binr, float* g0, int nx, int ny, int nz, void* stream);
+
+/* The per-cell part of Sim.fuv_field (sim.py:955-978) in one pass: acc = 0, then for each star in order
+ * acc += lum_s * trilinear(att_s) / max(r^2, 4e-4), then the store into g0 (fp16 g0_h or fp32 g0_f, exactly
+ * one non-null). atts = float[nstars][nr][nu][nphi], star_xyz = float[3*nstars], lums = float[nstars].
+ * Bit-identical to that launch sequence. */
+int nv_fuv_apply(const float* atts, int nr, int nu, int nphi, const float* star_xyz, const float* lums,
+                 int nstars, float dx, float binr, uint16_t* g0_h, float* g0_f, int nx, int ny, int nz,
+                 void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NEBULA_HIP_H */

@@ -1,0 +1,63 @@
+// nv_common.h - shared device helpers and the host-side error plumbing of the C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string>
+#include "../../include/nebula_hip.h"
+
+namespace nv {
+
+// ------------------------------------------------------------------------------------ host error plumbing
+void set_error(const char* fmt, ...);
+void clear_error();
+
+#define NV_REQUIRE(cond, ...)        \
+  do {                               \
+    if (!(cond)) {                   \
+      ::nv::set_error(__VA_ARGS__);  \
+      return NV_ERR_ARG;             \
+    }                                \
+  } while (0)
+
+// Checks the launch that was just issued; returns NV_OK or NV_ERR_LAUNCH with a message.
+int check_launch(const char* what);
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+// ------------------------------------------------------------------------------------ device helpers
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// C-order [nx][ny][nz] offset; 64-bit so 1024^3 grids (2^30 cells) index safely
+__device__ __forceinline__ long long cell(int i, int j, int k, int ny, int nz) {
+  return ((long long)i * ny + j) * (long long)nz + k;
+}
+
+// wave64 max of non-negative floats
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+// Block-wide max of non-negative floats into dst[0] (float bits compared as unsigned ints, which orders
+// non-negative floats correctly). `red` is LDS scratch of at least blockDim/64 floats.
+__device__ __forceinline__ void block_max_atomic(float v, float* red, float* dst) {
+  const int nthreads = blockDim.x * blockDim.y * blockDim.z;
+  const int tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+  v = wave_max(v);
+  __syncthreads();
+  if ((tid & 63) == 0) red[tid >> 6] = v;
+  __syncthreads();
+  if (tid == 0) {
+    float m = red[0];
+    for (int w = 1; w < (nthreads + 63) / 64; ++w) m = fmaxf(m, red[w]);
+    atomicMax(reinterpret_cast<unsigned int*>(dst), __float_as_uint(m));
+  }
+}
+
+__device__ __forceinline__ uint16_t f32_to_f16_bits(float x) {
+  return __builtin_bit_cast(uint16_t, (_Float16)x);
+}
+
+}  // namespace nv

@@ -1,0 +1,413 @@
+"""GPU-resident simulation step with the interface of the reference's sim.Sim (sim.py:543-1222).
+
+    sim = Sim("torus", res=64); sim.step(4); sim.update_render_fields()
+
+Fields are torch tensors on the ROCm device, C-order [nx, ny, nz] (x, y, z; z fastest), the reference layout:
+rho, mx, my, mz, xi (float32), dfront, g0 (float16, or float32 with P['fp32_fields']) and stage (4, nx, ny, nz)
+float32 scratch that is free between steps (sim.py:721-733). Every kernel is a hand-written HIP kernel in
+libnebula_hip.so, called through its C ABI (include/nebula_hip.h); there is no CPU fallback.
+
+Step order (Sim._main_step, sim.py:1099-1114): ionize every ion_every steps, star weights, CFL dt, jet pulse,
+two RK stages, clocks. Two fusions change no bits:
+  - the per-cell part of ionize (zero/fill, apply_front per star, smooth_xi, copy, to_half) is one pass
+    (nv_ionize_apply);
+  - the CFL max of the next step is reduced in the epilogue of RK stage 2 (or of the ionize pass when it
+    changes xi), so no separate max_speed pass reads the grid; the host still reads smax once per step,
+    as the reference does (sim.py:1012), to compute dt in double.
+"""
+import math
+import time
+
+import numpy as np
+import torch
+
+from . import _lib
+from . import scenarios as S
+
+
+def _f(x):
+    import ctypes
+    return ctypes.c_float(float(x))
+
+
+class Sim:
+    profile = False
+    dt_cap = None
+
+    def __init__(self, scenario="torus", res=None, params=None, rays=None, fuv_grid=None, ion_every=None,
+                 device="cuda:0", verbose=True, fast=False, setup=None, fused=True, seed=None):
+        self.device = torch.device(device)
+        self.fast = fast
+        self.verbose = verbose
+        self.rays_override = rays
+        self.fuv_override = fuv_grid
+        self.ion_every_override = ion_every
+        self.fused = fused
+        self.prof = {}
+        self.timer = None         # optional timing.KernelTimer: events around each kernel group
+        self._seed = seed
+        _lib.load()
+        if self.device.type != "cuda":
+            raise _lib.NebulaHipError("Sim runs on a ROCm device (device='cuda:N')")
+        self.reset(scenario, res, params, setup=setup)
+
+    # ------------------------------------------------------------------ helpers
+    def log(self, *a):
+        if self.verbose:
+            print("[sim]", *a, flush=True)
+
+    def _tick(self, key, t0):
+        self.prof[key] = self.prof.get(key, 0.0) + (time.perf_counter() - t0)
+
+    @property
+    def _stream(self):
+        return torch.cuda.current_stream(self.device).cuda_stream
+
+    def _t0(self, name):
+        return self.timer.start(name) if self.timer is not None else None
+
+    def _t1(self, tok):
+        if tok is not None:
+            self.timer.stop(tok)
+
+    @staticmethod
+    def _p(t):
+        return t.data_ptr() if t is not None else None
+
+    # ------------------------------------------------------------------ setup
+    def reset(self, scenario=None, res=None, params=None, setup=None):
+        t_setup = time.perf_counter()
+        for k in ("rho", "mx", "my", "mz", "xi", "dfront", "g0", "stage", "rfronts", "atts"):
+            self.__dict__.pop(k, None)
+        if setup is None:
+            setup = S.make_setup(scenario or "torus", res, self._seed, params)
+        elif params:
+            setup.P.update(params)
+        self.setup = setup
+        self.scenario = setup.name
+        self.res = setup.res
+        self.P = setup.P
+        P = self.P
+        nx, ny, nz = self.shape = tuple(setup.shape)
+        self.dx = setup.dx
+        self.L = setup.L
+        C = S.constants(setup, fast=self.fast, ion_every=self.ion_every_override)
+        self.C = C
+        self.cn2, self.ci2, self.diag, self.dr, self.nsteps = C["cn2"], C["ci2"], C["diag"], C["dr"], C["nsteps"]
+        self.bmax = C["bmax"]
+        self.ion_every, self.fuv_every = C["ion_every"], C["fuv_every"]
+        self.stars = [tuple(float(v) for v in s) for s in setup.stars]
+        self.kinds = list(setup.kinds)
+        self.jets = [tuple(float(v) for v in j[:3]) for j in setup.jets]
+        self.jkinds = list(setup.jkinds)
+        self.cap = max(len(self.stars), P["max_stars"])
+
+        dev = self.device
+        f32 = dict(dtype=torch.float32, device=dev)
+        self.rho = torch.from_numpy(np.ascontiguousarray(setup.rho, dtype=np.float32)).to(dev)
+        z = lambda a: (torch.zeros(self.shape, **f32) if a is None
+                       else torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).to(dev))
+        self.mx, self.my, self.mz = z(setup.mx), z(setup.my), z(setup.mz)
+        self.xi = torch.zeros(self.shape, **f32)
+        self.stage = torch.zeros((4,) + self.shape, **f32)
+        hdt = torch.float32 if P.get("fp32_fields") else torch.float16
+        self.dfront = torch.full(self.shape, 1e3, dtype=hdt, device=dev)
+        self.g0 = torch.zeros(self.shape, dtype=hdt, device=dev)
+        self.smax = torch.zeros(1, **f32)
+        self.star_arr = torch.zeros((self.cap, 4), **f32)
+        self.jet_arr = torch.zeros((self.cap, 4), **f32)
+        self.star_xyz = torch.zeros((self.cap, 3), **f32)
+        self.lums = torch.zeros(self.cap, **f32)
+        self._wkey = None
+        self._smax_ok = False
+        self.generation = getattr(self, "generation", 0) + 1
+        nu, nphi = self.rays_override or S.ray_grid(nz, self.fast)
+        self.rfronts = torch.zeros((self.cap, nu, nphi), **f32)
+        self.rfront = self.rfronts[0]
+        fg = tuple(self.fuv_override or S.fuv_grid(nz))
+        self.fuv_shape = fg
+        self.atts = None          # allocated on the first FUV update (render cadence only)
+        self.t = 0.0
+        self.it = 0
+        self.dt = 0.0
+        self.version = 0
+        self.rf_version = -1
+        self.fuv_version = -10 ** 9
+        self.events = []
+        self.phase = "main"       # the pre-turbulence phase is not part of this step (config pre_t = 0)
+        self._start_main()
+        torch.cuda.synchronize(dev)
+        self.setup_s = time.perf_counter() - t_setup
+        self.log(f"{self.scenario} grid {nx}x{ny}x{nz} ({nx * ny * nz / 1e6:.1f}M cells) rays {nu}x{nphi} "
+                 f"fuv {fg} solver {P['solver']}/{P['limiter']} setup {self.setup_s:.2f}s, {len(self.stars)} stars")
+
+    def _start_main(self):
+        self._set_star_weights()
+        self.ionize()
+
+    # ------------------------------------------------------------------ stars (sim.py:879-925)
+    def q_of(self, i):
+        s = self.stars[i]
+        return s[3] * self.P["q_main"] if self.kinds[i] == "main" else s[3]
+
+    def ramp(self, i):
+        age = self.t - self.stars[i][4]
+        if self.phase != "main" or age < 0.0:
+            return 0.0
+        tr = float(self.P.get("q_ramp", 0.0))
+        if tr <= 0.0 or age >= tr:
+            return 1.0
+        x = age / tr
+        s = x * x * (3.0 - 2.0 * x)
+        return max(s * s * s, 1e-4)
+
+    def q_eff(self, i):
+        return self.q_of(i) * self.ramp(i)
+
+    def jet_of(self, i):
+        k = self.jkinds[i]
+        return self.P["jet"] if k == "jet" else (self.P["jet_proto"] if k == "proto" else 0.0)
+
+    def active_stars(self):
+        out = []
+        for i, s in enumerate(self.stars):
+            if self.t >= s[4]:
+                out.append((s[0], s[1], s[2], self.q_eff(i) / self.P["q_main"], self.t - s[4]))
+        return out
+
+    def _set_star_weights(self):
+        qm = self.P["q_main"]
+        w = [(s[0], s[1], s[2], (self.q_eff(i) / qm) if self.t >= s[4] else 0.0) for i, s in enumerate(self.stars)]
+        w += [(0.0, 0.0, 0.0, 0.0)] * (self.cap - len(w))
+        key = tuple(round(v[3], 9) for v in w) + tuple(self.jet_of(i) for i in range(len(self.stars)))
+        if key == self._wkey:
+            return
+        self._wkey = key
+        self.star_arr.copy_(torch.tensor(np.array(w, dtype=np.float32)))
+        jw = [(a[0], a[1], a[2], self.jet_of(i)) for i, a in enumerate(self.jets)]
+        jw += [(0.0, 0.0, 1.0, 0.0)] * (self.cap - len(jw))
+        self.jet_arr.copy_(torch.tensor(np.array(jw, dtype=np.float32)))
+
+    def replace_stars(self, stars, kinds, jets, jkinds):
+        self.stars = [tuple(float(v) for v in s) for s in stars]
+        self.kinds = list(kinds)
+        self.jets = [tuple(float(v) for v in j[:3]) for j in jets]
+        self.jkinds = list(jkinds)
+        if len(self.stars) > self.cap:
+            self.cap = len(self.stars)
+            f32 = dict(dtype=torch.float32, device=self.device)
+            self.star_arr = torch.zeros((self.cap, 4), **f32)
+            self.jet_arr = torch.zeros((self.cap, 4), **f32)
+            self.star_xyz = torch.zeros((self.cap, 3), **f32)
+            self.lums = torch.zeros(self.cap, **f32)
+            self.rfronts = torch.zeros((self.cap,) + tuple(self.rfronts.shape[1:]), **f32)
+            self.atts = None
+        self._wkey = None
+        self._set_star_weights()
+        self.ionize()
+
+    def stars_absolute(self):
+        st = [(s[0], s[1], s[2], self.q_of(i), s[4]) for i, s in enumerate(self.stars)]
+        jt = [(j[0], j[1], j[2], self.jet_of(i)) for i, j in enumerate(self.jets)]
+        return st, jt
+
+    # ------------------------------------------------------------------ physics pieces
+    def ionize(self):
+        """Stromgren fronts of every switched-on star -> xi, dfront (Sim.ionize, sim.py:927-947)."""
+        t0 = time.perf_counter()
+        nx, ny, nz = self.shape
+        nu, nphi = self.rfronts.shape[1:]
+        strm = self._stream
+        on = [i for i, s in enumerate(self.stars) if self.t >= s[4]]
+        Lx, Ly, Lz = self.bmax
+        tk = self._t0("cast_rays")
+        for slot, i in enumerate(on):
+            sx, sy, sz = self.stars[i][:3]
+            _lib.call("nv_cast_rays", self._p(self.rho), nx, ny, nz, _f(sx), _f(sy), _f(sz), _f(self.q_eff(i)),
+                      _f(self.dr), self.nsteps, _f(self.dx), _f(self.diag), _f(Lx), _f(Ly), _f(Lz),
+                      self._p(self.rfronts[slot]), nu, nphi, strm)
+        self._t1(tk)
+        width = 1.2 * self.dx
+        half = self.dfront.dtype == torch.float16
+        if self.fused:
+            xyz = np.array([self.stars[i][:3] for i in on], dtype=np.float32).reshape(-1, 3)
+            if len(on):
+                self.star_xyz[:len(on)].copy_(torch.from_numpy(xyz))
+            self.smax.zero_()
+            tk = self._t0("ionize_apply")
+            _lib.call("nv_ionize_apply", self._p(self.rfronts), nu, nphi, self._p(self.star_xyz), len(on),
+                      _f(self.dx), _f(width), self._p(self.xi), self._p(self.dfront) if half else None,
+                      None if half else self._p(self.dfront), nx, ny, nz, self._p(self.rho), self._p(self.mx),
+                      self._p(self.my), self._p(self.mz), _f(self.cn2), _f(self.ci2), self._p(self.smax), strm)
+            self._t1(tk)
+            self._smax_ok = True
+        else:
+            # the reference launch sequence, kernel by kernel (sim.py:930-943)
+            n = nx * ny * nz
+            df = self.stage[1]
+            _lib.call("nv_fill_f32", self._p(self.xi), _f(0.0), n, strm)
+            _lib.call("nv_fill_f32", self._p(df), _f(1e3), n, strm)
+            for slot, i in enumerate(on):
+                sx, sy, sz = self.stars[i][:3]
+                _lib.call("nv_apply_front", self._p(self.rfronts[slot]), nu, nphi, _f(sx), _f(sy), _f(sz),
+                          _f(self.dx), _f(width), self._p(self.xi), self._p(df), nx, ny, nz, strm)
+            _lib.call("nv_smooth_xi", self._p(self.xi), self._p(self.stage[0]), nx, ny, nz, strm)
+            self.xi.copy_(self.stage[0])
+            if half:
+                _lib.call("nv_to_half", self._p(df), self._p(self.dfront), n, strm)
+            else:
+                self.dfront.copy_(df)
+            self._smax_ok = False
+        if self.profile:
+            torch.cuda.synchronize(self.device)
+            self._tick("ionize", t0)
+
+    def fuv_field(self):
+        """G0 = sum over stars of L exp(-tau_FUV) / r^2 (Sim.fuv_field, sim.py:949-978), stored into g0."""
+        nx, ny, nz = self.shape
+        fnr, fnu, fnp = self.fuv_shape
+        if self.atts is None:
+            self.atts = torch.zeros((self.cap, fnr, fnu, fnp), dtype=torch.float32, device=self.device)
+        self.att = self.atts[0]
+        binr = self.diag / fnr
+        strm = self._stream
+        on = [i for i, s in enumerate(self.stars) if self.t >= s[4]]
+        Lx, Ly, Lz = self.bmax
+        for slot, i in enumerate(on):
+            sx, sy, sz = self.stars[i][:3]
+            _lib.call("nv_cast_fuv", self._p(self.rho), self._p(self.xi), nx, ny, nz, _f(sx), _f(sy), _f(sz),
+                      _f(self.P["kappa_fuv"]), _f(self.dr), self.nsteps, _f(self.dx), _f(binr), _f(Lx), _f(Ly),
+                      _f(Lz), self._p(self.atts[slot]), fnr, fnu, fnp, strm)
+        xyz = np.array([self.stars[i][:3] for i in on], dtype=np.float32).reshape(-1, 3)
+        lum = np.array([self.q_eff(i) / self.P["q_main"] for i in on], dtype=np.float32)
+        if len(on):
+            self.star_xyz[:len(on)].copy_(torch.from_numpy(xyz))
+            self.lums[:len(on)].copy_(torch.from_numpy(lum))
+        half = self.g0.dtype == torch.float16
+        _lib.call("nv_fuv_apply", self._p(self.atts), fnr, fnu, fnp, self._p(self.star_xyz), self._p(self.lums),
+                  len(on), _f(self.dx), _f(binr), self._p(self.g0) if half else None,
+                  None if half else self._p(self.g0), nx, ny, nz, strm)
+
+    def update_render_fields(self, force=False):
+        """recompute G0 at the render cadence (Sim.update_render_fields, sim.py:989-1007)"""
+        if self.rf_version == self.version and not force:
+            return False
+        n_on = sum(1 for s in self.stars if self.t >= s[4])
+        tr = float(self.P.get("q_ramp", 0.0))
+        ramping = tr > 0 and any(0.0 <= self.t - s[4] < tr for s in self.stars)
+        if ramping and self.version - self.fuv_version >= max(2, self.fuv_every // 4):
+            force = True
+        if (force or self.version - self.fuv_version >= self.fuv_every
+                or n_on != getattr(self, "fuv_nstars", -1) or self.phase != getattr(self, "fuv_phase", None)):
+            self.fuv_field()
+            self.fuv_version, self.fuv_nstars, self.fuv_phase = self.version, n_on, self.phase
+        self.rf_version = self.version
+        return True
+
+    def fields_modified(self):
+        """Call after writing rho/m/xi from outside (timeline restore, tests): the cached CFL max is stale."""
+        self._smax_ok = False
+        self.version += 1
+
+    def _compute_dt(self):
+        """dt = cfl dx / max(|u|+|v|+|w|+c) (Sim._compute_dt, sim.py:1009-1012), the max read in double."""
+        if not self._smax_ok:
+            nx, ny, nz = self.shape
+            self.smax.zero_()
+            _lib.call("nv_max_speed", self._p(self.rho), self._p(self.mx), self._p(self.my), self._p(self.mz),
+                      self._p(self.xi), nx, ny, nz, _f(self.cn2), _f(self.ci2), self._p(self.smax), self._stream)
+        smax = float(self.smax.item())
+        self._smax_ok = False
+        return self.P["cfl"] * self.dx / smax
+
+    def _hydro(self, dt, nstars, wind, pulse):
+        """Two RK stages (Sim._hydro, sim.py:1014-1028): U1 = U + dt L(U); U = U/2 + (U1 + dt L(U1))/2 in place.
+        Stage 2 also reduces the next step's CFL signal speed into self.smax."""
+        P = self.P
+        nx, ny, nz = self.shape
+        solver = S.SOLVERS[P.get("solver", "rusanov")]
+        lim = S.LIMITERS[P.get("limiter", "minmod")] if solver == 1 else 0
+        strm = self._stream
+        U = [self._p(t) for t in (self.rho, self.mx, self.my, self.mz)]
+        U1 = [self._p(self.stage[c]) for c in range(4)]
+        common = [nx, ny, nz, self._p(self.star_arr), nstars, _f(wind), self._p(self.jet_arr), _f(self.C["jet_cos"]),
+                  _f(P["jet_r"]), _f(pulse), _f(dt), _f(self.dx)]
+        tail = [_f(P["floor"]), _f(self.cn2), _f(self.ci2), _f(6.0), solver, lim]
+        tk = self._t0("hydro_stage")
+        _lib.call("nv_hydro_stage", *U, self._p(self.xi), *U, *U1, *common, _f(0.0), _f(1.0), *tail, None, strm)
+        self._t1(tk)
+        self.smax.zero_()
+        tk = self._t0("hydro_stage")
+        _lib.call("nv_hydro_stage", *U1, self._p(self.xi), *U, *U, *common, _f(0.5), _f(0.5), *tail,
+                  self._p(self.smax), strm)
+        self._t1(tk)
+        self._smax_ok = True
+
+    def _main_step(self):
+        P = self.P
+        if self.it % self.ion_every == 0:
+            self.ionize()
+        self._set_star_weights()
+        dt = self._compute_dt()
+        if self.dt_cap is not None:
+            dt = min(dt, self.dt_cap)
+        pulse = max(0.0, 1.0 + P["jet_pulse"] * math.sin(2 * math.pi * self.t / P["jet_period"]))
+        self._hydro(dt, len(self.stars), P["wind"], pulse)
+        self.t += dt
+        self.it += 1
+        self.dt = dt
+
+    # ------------------------------------------------------------------ public API
+    def step(self, k=1):
+        """advance k steps. Returns wall seconds."""
+        t0 = time.perf_counter()
+        for _ in range(k):
+            self._main_step()
+            self.version += 1
+        self._tick("step", t0)
+        self.prof["n_steps"] = self.prof.get("n_steps", 0) + k
+        return time.perf_counter() - t0
+
+    def set_param(self, name, value):
+        old = self.P.get(name)
+        if isinstance(old, int) and not isinstance(old, bool):
+            value = int(round(value))
+        self.P[name] = value
+        if name in S.LIVE:
+            self.events.append((float(self.t), name, value))
+            if name in ("q_main", "jet", "jet_proto"):
+                self._wkey = None
+        return value
+
+    def tau(self):
+        return self.t
+
+    def time_label(self):
+        return f"t={self.t:.4f}"
+
+    def get_state_meta(self):
+        return dict(t=self.t, it=self.it, dt=self.dt, phase=self.phase, stars=list(self.stars),
+                    kinds=list(self.kinds), jets=list(self.jets), jkinds=list(self.jkinds), P=dict(self.P),
+                    n_events=len(self.events))
+
+    def set_state_meta(self, m):
+        for k in ("t", "it", "dt", "phase"):
+            setattr(self, k, m[k])
+        self.stars, self.kinds = list(m["stars"]), list(m["kinds"])
+        self.jets, self.jkinds = list(m["jets"]), list(m["jkinds"])
+        self.P = dict(m["P"])
+        self.events = self.events[:m["n_events"]]
+        self._wkey = None
+        self._set_star_weights()
+        self.fields_modified()
+        self.rf_version = self.version
+        self.fuv_version = self.version
+
+    def render_scratch(self):
+        """two (nx, ny, nz, 4) float16 views of the stage block, free between steps (sim.py:821-825)"""
+        flat = self.stage.view(-1).view(torch.float16)
+        n = self.shape[0] * self.shape[1] * self.shape[2]
+        return flat[:4 * n].view(self.shape + (4,)), flat[4 * n:8 * n].view(self.shape + (4,))
+
+    def stats(self):
+        return dict(ion=float(self.xi.mean()), rho_max=float(self.rho.max()), rho_min=float(self.rho.min()))

@@ -1,0 +1,565 @@
+// nebula_oracle.cpp - CPU restatement of NebulaViewer's simulation-step kernels. TEST INFRASTRUCTURE ONLY.
+//
+// This file is the correctness oracle for the HIP path: only tests/, __graft_entry__.smoke() and bench.py's
+// cpu_baseline leg may load it, and only as the checker or the timed CPU baseline. The product never links it.
+//
+// It restates the reference Warp kernels operation by operation (same expression order, no FMA contraction:
+// the Makefile builds with -ffp-contract=off) and is instantiated for float (the drop-in parity oracle) and
+// double (the FP64 error-budget oracle). Each function cites the reference lines it follows.
+//
+// Pinning (what checks this restatement independently; the reference has no tests or fixtures):
+//   limiters, fluxes   : closed-form scalar tables and flux identities (tests/test_oracle_closed_forms.py)
+//   hydro_stage        : an independent NumPy FP64 restatement (tests/oracle_numpy.py), uniform-state and
+//                        mirror invariants, FP64 mass telescoping, the exact 1D isothermal Riemann solution
+//   max_speed          : brute-force NumPy max
+//   sample_n, box_range: exact on affine fields, zero outside, closed-form slab test
+//   cast_rays          : uniform-density Stromgren radius (3Q/n^2)^(1/3), brute-force NumPy ray sums
+//   apply_front        : constant rfront -> closed-form spherical ramp
+//   smooth_xi          : brute force
+//   cast_fuv/apply_fuv : uniform n, x=0 -> exp(-kappa n r) and L att / r^2
+//   to_half            : numpy float16 round-to-nearest-even on all exponents
+// Exact Warp codegen bits (its FMA contraction and math library) cannot be pinned: Warp is not installed.
+#include <stdint.h>
+#include <string.h>
+#include <math.h>
+#include <algorithm>
+#include "../nebulaviewer_amd/csrc/nv_math.h"
+
+namespace {
+
+// transcendentals: deterministic float versions shared with the GPU path; libm for the FP64 oracle
+inline float t_sin(float x) { return nvm::sinf_det(x); }
+inline float t_cos(float x) { return nvm::cosf_det(x); }
+inline float t_atan2(float y, float x) { return nvm::atan2f_det(y, x); }
+inline float t_exp(float x) { return nvm::expf_det(x); }
+inline double t_sin(double x) { return sin(x); }
+inline double t_cos(double x) { return cos(x); }
+inline double t_atan2(double y, double x) { return atan2(y, x); }
+inline double t_exp(double x) { return exp(x); }
+
+template <typename R> inline R rmax(R a, R b) { return a > b ? a : b; }
+template <typename R> inline R rmin(R a, R b) { return a < b ? a : b; }
+inline int iclamp(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+struct Grid {
+  int nx, ny, nz;
+  size_t at(int i, int j, int k) const { return ((size_t)i * ny + j) * nz + k; }
+  // ldc: zero-gradient ghosts by index clamping (nebula3d.py:23-25)
+  size_t atc(int i, int j, int k) const {
+    return at(iclamp(i, 0, nx - 1), iclamp(j, 0, ny - 1), iclamp(k, 0, nz - 1));
+  }
+};
+
+template <typename R> struct V4 { R c[4]; };
+
+// prim (nebula3d.py:28-32): q = (r, mx/r, my/r, mz/r), r = max(rho, floor)
+template <typename R>
+inline V4<R> prim(const R* rho, const R* mx, const R* my, const R* mz, const Grid& g, int i, int j, int k,
+                  R floor) {
+  size_t o = g.atc(i, j, k);
+  R r = rmax(rho[o], floor);
+  return V4<R>{{r, mx[o] / r, my[o] / r, mz[o] / r}};
+}
+
+// csq (nebula3d.py:35-37)
+template <typename R>
+inline R csq(const R* xi, const Grid& g, int i, int j, int k, R cn2, R ci2) {
+  return cn2 + xi[g.atc(i, j, k)] * (ci2 - cn2);
+}
+
+// minmod (nebula3d.py:40-46)
+template <typename R> inline R minmod(R a, R b) {
+  if (a * b <= R(0)) return R(0);
+  if (fabs(a) < fabs(b)) return a;
+  return b;
+}
+// mc_lim (sim.py:366-372)
+template <typename R> inline R mc_lim(R a, R b) {
+  if (a * b <= R(0)) return R(0);
+  R s = a > R(0) ? R(1) : R(-1);
+  return s * rmin(rmin(R(2) * R(fabs(a)), R(2) * R(fabs(b))), R(0.5) * R(fabs(a + b)));
+}
+// vl_lim (sim.py:375-380)
+template <typename R> inline R vl_lim(R a, R b) {
+  if (a * b <= R(0)) return R(0);
+  return R(2) * a * b / (a + b);
+}
+// slope (sim.py:383-389)
+template <typename R> inline R slope(R a, R b, int lim) {
+  if (lim == 2) return mc_lim(a, b);
+  if (lim == 1) return vl_lim(a, b);
+  return minmod(a, b);
+}
+
+// face_flux (Rusanov, nebula3d.py:54-72) and flux_hll (sim.py:397-420). Rusanov always uses minmod4
+// (nebula3d.py:60-61); HLL uses lim4 with the chosen limiter. e = (0, one-hot axis).
+template <typename R>
+inline V4<R> face_flux(const V4<R>& qa, const V4<R>& qb, const V4<R>& qc, const V4<R>& qd, R c2b, R c2c,
+                       const R e[4], R floor, int solver, int lim) {
+  V4<R> L, Rr;
+  for (int c = 0; c < 4; ++c) {
+    R sl = solver ? slope(qb.c[c] - qa.c[c], qc.c[c] - qb.c[c], lim) : minmod(qb.c[c] - qa.c[c], qc.c[c] - qb.c[c]);
+    R sr = solver ? slope(qc.c[c] - qb.c[c], qd.c[c] - qc.c[c], lim) : minmod(qc.c[c] - qb.c[c], qd.c[c] - qc.c[c]);
+    L.c[c] = qb.c[c] + R(0.5) * sl;
+    Rr.c[c] = qc.c[c] - R(0.5) * sr;
+  }
+  R rL = rmax(L.c[0], floor);
+  R rR = rmax(Rr.c[0], floor);
+  // un = dot((0, u, v, w), e)
+  R vL[4] = {R(0), L.c[1], L.c[2], L.c[3]};
+  R vR[4] = {R(0), Rr.c[1], Rr.c[2], Rr.c[3]};
+  R unL = R(0), unR = R(0);
+  for (int c = 0; c < 4; ++c) { unL += vL[c] * e[c]; unR += vR[c] * e[c]; }
+  R UL[4] = {rL, rL * L.c[1], rL * L.c[2], rL * L.c[3]};
+  R UR[4] = {rR, rR * Rr.c[1], rR * Rr.c[2], rR * Rr.c[3]};
+  R pL = rL * c2b, pR = rR * c2c;
+  R FL[4], FR[4];
+  for (int c = 0; c < 4; ++c) {
+    FL[c] = UL[c] * unL + e[c] * pL;
+    FR[c] = UR[c] * unR + e[c] * pR;
+  }
+  V4<R> F;
+  if (!solver) {
+    R amax = rmax(R(fabs(unL)) + R(sqrt(c2b)), R(fabs(unR)) + R(sqrt(c2c)));
+    R ha = R(0.5) * amax;
+    for (int c = 0; c < 4; ++c) F.c[c] = R(0.5) * (FL[c] + FR[c]) - ha * (UR[c] - UL[c]);
+    return F;
+  }
+  R cb = sqrt(c2b), cc = sqrt(c2c);
+  R SL = rmin(unL - cb, unR - cc);
+  R SR = rmax(unL + cb, unR + cc);
+  if (SL >= R(0)) { for (int c = 0; c < 4; ++c) F.c[c] = FL[c]; return F; }
+  if (SR <= R(0)) { for (int c = 0; c < 4; ++c) F.c[c] = FR[c]; return F; }
+  R ss = SL * SR, den = SR - SL;
+  for (int c = 0; c < 4; ++c) F.c[c] = ((SR * FL[c] - SL * FR[c]) + ss * (UR[c] - UL[c])) / den;
+  return F;
+}
+
+// hydro_stage (nebula3d.py:76-151) / hydro_stage_hll (sim.py:423-498): one RK stage for one cell.
+template <typename R>
+inline void hydro_cell(const R* rho, const R* mx, const R* my, const R* mz, const R* xi, const R* r0,
+                       const R* mx0, const R* my0, const R* mz0, R* o_r, R* o_mx, R* o_my, R* o_mz,
+                       const Grid& g, int i, int j, int k, const R* stars, int nstars, R wind, const R* jets,
+                       R jet_cos, R jet_r, R pulse, R dt, R dx, R a, R b, R floor, R cn2, R ci2, R vmax,
+                       int solver, int lim) {
+  R div[4] = {R(0), R(0), R(0), R(0)};
+  for (int axis = 0; axis < 3; ++axis) {
+    int oi = axis == 0, oj = axis == 1, ok = axis == 2;
+    R e[4] = {R(0), R(oi), R(oj), R(ok)};
+    V4<R> qm2 = prim(rho, mx, my, mz, g, i - 2 * oi, j - 2 * oj, k - 2 * ok, floor);
+    V4<R> qm1 = prim(rho, mx, my, mz, g, i - oi, j - oj, k - ok, floor);
+    V4<R> q0 = prim(rho, mx, my, mz, g, i, j, k, floor);
+    V4<R> qp1 = prim(rho, mx, my, mz, g, i + oi, j + oj, k + ok, floor);
+    V4<R> qp2 = prim(rho, mx, my, mz, g, i + 2 * oi, j + 2 * oj, k + 2 * ok, floor);
+    R cm1 = csq(xi, g, i - oi, j - oj, k - ok, cn2, ci2);
+    R c0 = csq(xi, g, i, j, k, cn2, ci2);
+    R cp1 = csq(xi, g, i + oi, j + oj, k + ok, cn2, ci2);
+    V4<R> fR = face_flux(qm1, q0, qp1, qp2, c0, cp1, e, floor, solver, lim);
+    V4<R> fL = face_flux(qm2, qm1, q0, qp1, cm1, c0, e, floor, solver, lim);
+    for (int c = 0; c < 4; ++c) div[c] = div[c] + (fR.c[c] - fL.c[c]);
+  }
+  const size_t o = g.at(i, j, k);
+  R U[4] = {rho[o], mx[o], my[o], mz[o]};
+  R dU[4];
+  for (int c = 0; c < 4; ++c) dU[c] = -div[c] / dx;
+  const R p[3] = {(R(i) + R(0.5)) * dx, (R(j) + R(0.5)) * dx, (R(k) + R(0.5)) * dx};
+  const R x_ion = xi[o];
+  if (wind > R(0) && x_ion > R(0.01)) {
+    R acc[3] = {R(0), R(0), R(0)};
+    for (int s = 0; s < nstars; ++s) {
+      const R* st = stars + 4 * s;
+      if (st[3] > R(0)) {
+        R rel[3] = {p[0] - st[0], p[1] - st[1], p[2] - st[2]};
+        R r2 = (R(0) + rel[0] * rel[0] + rel[1] * rel[1] + rel[2] * rel[2]) + R(0.0004);
+        R f = st[3] / (r2 * R(sqrt(r2)));
+        for (int c = 0; c < 3; ++c) acc[c] = acc[c] + rel[c] * f;
+      }
+    }
+    R wx = wind * x_ion;
+    for (int c = 0; c < 3; ++c) acc[c] = acc[c] * wx;
+    dU[0] = dU[0] + R(0);
+    for (int c = 0; c < 3; ++c) dU[c + 1] = dU[c + 1] + U[0] * acc[c];
+  }
+  R accj[3] = {R(0), R(0), R(0)};
+  for (int s = 0; s < nstars; ++s) {
+    const R* jt = jets + 4 * s;
+    const R* st = stars + 4 * s;
+    if (jt[3] > R(0) && st[3] > R(0)) {
+      R rel[3] = {p[0] - st[0], p[1] - st[1], p[2] - st[2]};
+      R r = sqrt(R(0) + rel[0] * rel[0] + rel[1] * rel[1] + rel[2] * rel[2]);
+      if (r < jet_r && r > R(1e-6)) {
+        R cth = (R(0) + rel[0] * jt[0] + rel[1] * jt[1] + rel[2] * jt[2]) / r;
+        if (R(fabs(cth)) > jet_cos) {
+          R f = (cth > R(0) ? R(1) : R(-1)) * jt[3] * pulse / (r * r + R(0.0004));
+          for (int c = 0; c < 3; ++c) accj[c] = accj[c] + jt[c] * f;
+        }
+      }
+    }
+  }
+  dU[0] = dU[0] + R(0);
+  for (int c = 0; c < 3; ++c) dU[c + 1] = dU[c + 1] + U[0] * accj[c];
+  R Un[4], out[4];
+  for (int c = 0; c < 4; ++c) Un[c] = U[c] + dt * dU[c];
+  const R U0[4] = {r0[o], mx0[o], my0[o], mz0[o]};
+  for (int c = 0; c < 4; ++c) out[c] = a * U0[c] + b * Un[c];
+  R r = rmax(out[0], floor);
+  R vel[3] = {out[1] / r, out[2] / r, out[3] / r};
+  R sp = sqrt(R(0) + vel[0] * vel[0] + vel[1] * vel[1] + vel[2] * vel[2]);
+  if (sp > vmax) {
+    R f = vmax / sp;
+    for (int c = 0; c < 3; ++c) vel[c] = vel[c] * f;
+  }
+  o_r[o] = r;
+  o_mx[o] = r * vel[0];
+  o_my[o] = r * vel[1];
+  o_mz[o] = r * vel[2];
+}
+
+template <typename R>
+void hydro_stage(const R* rho, const R* mx, const R* my, const R* mz, const R* xi, const R* r0, const R* mx0,
+                 const R* my0, const R* mz0, R* o_r, R* o_mx, R* o_my, R* o_mz, int nx, int ny, int nz,
+                 const R* stars, int nstars, R wind, const R* jets, R jet_cos, R jet_r, R pulse, R dt, R dx, R a,
+                 R b, R floor, R cn2, R ci2, R vmax, int solver, int lim, int nthreads) {
+  Grid g{nx, ny, nz};
+  // stage 2 writes over U^n (o == r0): every cell reads its own U0 before writing it and nothing else reads
+  // U0, so the in-place update is race-free (sim.py:1026-1028)
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+  for (int i = 0; i < nx; ++i)
+    for (int j = 0; j < ny; ++j)
+      for (int k = 0; k < nz; ++k)
+        hydro_cell(rho, mx, my, mz, xi, r0, mx0, my0, mz0, o_r, o_mx, o_my, o_mz, g, i, j, k, stars, nstars, wind,
+                   jets, jet_cos, jet_r, pulse, dt, dx, a, b, floor, cn2, ci2, vmax, solver, lim);
+}
+
+// max_speed (nebula3d.py:154-162): max over cells of |u|+|v|+|w| + c, raw rho (no floor)
+template <typename R>
+R max_speed(const R* rho, const R* mx, const R* my, const R* mz, const R* xi, size_t n, R cn2, R ci2,
+            int nthreads) {
+  R best = R(0);
+#pragma omp parallel for num_threads(nthreads) reduction(max : best) schedule(static)
+  for (long long q = 0; q < (long long)n; ++q) {
+    R r = rho[q];
+    R v0 = mx[q] / r, v1 = my[q] / r, v2 = mz[q] / r;
+    R c = sqrt(cn2 + xi[q] * (ci2 - cn2));
+    R s = ((R(fabs(v0)) + R(fabs(v1))) + R(fabs(v2))) + c;
+    if (s > best) best = s;
+  }
+  return best;
+}
+
+// sample_n (nebula3d.py:195-226): trilinear at world p, 0 outside [-1/2, n-1/2] in cell units
+template <typename R>
+inline R sample_n(const R* f, const Grid& g, R px, R py, R pz, R dx) {
+  R gx = px / dx - R(0.5), gy = py / dx - R(0.5), gz = pz / dx - R(0.5);
+  if (gx < R(-0.5) || gy < R(-0.5) || gz < R(-0.5)) return R(0);
+  if (gx > R(g.nx) - R(0.5) || gy > R(g.ny) - R(0.5) || gz > R(g.nz) - R(0.5)) return R(0);
+  int i0 = (int)floor(gx), j0 = (int)floor(gy), k0 = (int)floor(gz);
+  R fx = gx - R(i0), fy = gy - R(j0), fz = gz - R(k0);
+  R c000 = f[g.atc(i0, j0, k0)], c100 = f[g.atc(i0 + 1, j0, k0)];
+  R c010 = f[g.atc(i0, j0 + 1, k0)], c110 = f[g.atc(i0 + 1, j0 + 1, k0)];
+  R c001 = f[g.atc(i0, j0, k0 + 1)], c101 = f[g.atc(i0 + 1, j0, k0 + 1)];
+  R c011 = f[g.atc(i0, j0 + 1, k0 + 1)], c111 = f[g.atc(i0 + 1, j0 + 1, k0 + 1)];
+  R c00 = c000 + (c100 - c000) * fx;
+  R c10 = c010 + (c110 - c010) * fx;
+  R c01 = c001 + (c101 - c001) * fx;
+  R c11 = c011 + (c111 - c011) * fx;
+  R c0 = c00 + (c10 - c00) * fy;
+  R c1 = c01 + (c11 - c01) * fy;
+  return c0 + (c1 - c0) * fz;
+}
+
+// box_range (nebula3d.py:229-244): slab test against [0, bmax]
+template <typename R>
+inline void box_range(const R o[3], const R d[3], const R bmax[3], R* t0o, R* t1o) {
+  R t0 = R(0), t1 = R(1e9);
+  for (int a = 0; a < 3; ++a) {
+    R da = d[a];
+    if (R(fabs(da)) < R(1e-9)) {
+      if (o[a] < R(0) || o[a] > bmax[a]) t1 = R(-1);
+    } else {
+      R ta = (R(0) - o[a]) / da, tb = (bmax[a] - o[a]) / da;
+      t0 = rmax(t0, rmin(ta, tb));
+      t1 = rmin(t1, rmax(ta, tb));
+    }
+  }
+  *t0o = t0;
+  *t1o = t1;
+}
+
+// ray direction of angular cell (iu, ip) (nebula3d.py:252-257, 325-330)
+template <typename R>
+inline void ray_dir(int iu, int ip, int nu, int nphi, R d[3]) {
+  const R pi = R(3.14159265358979323846);
+  R u = R(-1) + (R(iu) + R(0.5)) * R(2) / R(nu);
+  R s = sqrt(rmax(R(1) - u * u, R(0)));
+  R phi = (R(ip) + R(0.5)) * R(2) * pi / R(nphi);
+  d[0] = s * t_cos(phi);
+  d[1] = s * t_sin(phi);
+  d[2] = u;
+}
+
+// cast_rays (nebula3d.py:247-270): front radius where the recombination column int n^2 r^2 dr reaches Q
+template <typename R>
+void cast_rays(const R* rho, int nx, int ny, int nz, const R star[3], R Q, R dr, int nsteps, R dx, R rcap,
+               const R bmax[3], R* rfront, int nu, int nphi, int nthreads) {
+  Grid g{nx, ny, nz};
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 4)
+  for (int iu = 0; iu < nu; ++iu)
+    for (int ip = 0; ip < nphi; ++ip) {
+      R d[3];
+      ray_dir(iu, ip, nu, nphi, d);
+      R col = R(0), rf = rcap, t0, t1;
+      box_range(star, d, bmax, &t0, &t1);
+      int n0 = (int)(t0 / dr);
+      int n1 = std::min(nsteps, (int)(t1 / dr) + 2);
+      for (int n = n0; n < n1; ++n) {
+        R r = (R(n) + R(0.5)) * dr;
+        R nv = sample_n(rho, g, star[0] + d[0] * r, star[1] + d[1] * r, star[2] + d[2] * r, dx);
+        R inc = nv * nv * r * r * dr;
+        if (col + inc > Q) {
+          rf = r - R(0.5) * dr + dr * (Q - col) / rmax(inc, R(1e-20));
+          break;
+        }
+        col = col + inc;
+      }
+      rfront[(size_t)iu * nphi + ip] = rf;
+    }
+}
+
+// rf_at (nebula3d.py:273-278): u clamped, phi wrapped
+template <typename R> inline R rf_at(const R* rfront, int nu, int nphi, int iu, int ip) {
+  int ipw = ((ip % nphi) + nphi) % nphi;
+  return rfront[(size_t)iclamp(iu, 0, nu - 1) * nphi + ipw];
+}
+
+// direction-grid coordinates of a cell centre seen from a star (nebula3d.py:284-293, 358-371)
+template <typename R>
+inline void cell_sph(int i, int j, int k, const R star[3], R dx, R* r_o, R* u_o, R* phi_o) {
+  const R pi = R(3.14159265358979323846);
+  R rel[3] = {(R(i) + R(0.5)) * dx - star[0], (R(j) + R(0.5)) * dx - star[1], (R(k) + R(0.5)) * dx - star[2]};
+  R r = rmax(R(sqrt(R(0) + rel[0] * rel[0] + rel[1] * rel[1] + rel[2] * rel[2])), R(1e-6));
+  R u = rel[2] / r;
+  R phi = t_atan2(rel[1], rel[0]);
+  if (phi < R(0)) phi = phi + R(2) * pi;
+  *r_o = r;
+  *u_o = u;
+  *phi_o = phi;
+}
+
+// apply_front (nebula3d.py:281-307)
+template <typename R>
+void apply_front(const R* rfront, int nu, int nphi, const R star[3], R dx, R width, R* xi, R* dfront, int nx,
+                 int ny, int nz, int nthreads) {
+  Grid g{nx, ny, nz};
+  const R pi = R(3.14159265358979323846);
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+  for (int i = 0; i < nx; ++i)
+    for (int j = 0; j < ny; ++j)
+      for (int k = 0; k < nz; ++k) {
+        R r, u, phi;
+        cell_sph(i, j, k, star, dx, &r, &u, &phi);
+        R fu = (u + R(1)) * R(0.5) * R(nu) - R(0.5);
+        R fp = phi / (R(2) * pi) * R(nphi) - R(0.5);
+        int iu = (int)floor(fu), ip = (int)floor(fp);
+        R tu = fu - R(iu), tp = fp - R(ip);
+        R av = rf_at(rfront, nu, nphi, iu, ip) * (R(1) - tp) + rf_at(rfront, nu, nphi, iu, ip + 1) * tp;
+        R bv = rf_at(rfront, nu, nphi, iu + 1, ip) * (R(1) - tp) + rf_at(rfront, nu, nphi, iu + 1, ip + 1) * tp;
+        R rf = av * (R(1) - tu) + bv * tu;
+        R val = rmin(rmax((rf - r) / width + R(0.5), R(0)), R(1));
+        size_t o = g.at(i, j, k);
+        if (val > xi[o]) xi[o] = val;
+        dfront[o] = rmin(dfront[o], r - rf);
+      }
+}
+
+// smooth_xi (nebula3d.py:310-315)
+template <typename R>
+void smooth_xi(const R* src, R* dst, int nx, int ny, int nz, int nthreads) {
+  Grid g{nx, ny, nz};
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+  for (int i = 0; i < nx; ++i)
+    for (int j = 0; j < ny; ++j)
+      for (int k = 0; k < nz; ++k) {
+        R m = (src[g.atc(i - 1, j, k)] + src[g.atc(i + 1, j, k)] + src[g.atc(i, j - 1, k)] +
+               src[g.atc(i, j + 1, k)] + src[g.atc(i, j, k - 1)] + src[g.atc(i, j, k + 1)]) / R(6);
+        size_t o = g.at(i, j, k);
+        dst[o] = R(0.5) * src[o] + R(0.5) * m;
+      }
+}
+
+// cast_fuv (nebula3d.py:320-352): dust attenuation exp(-tau) on radial bins along each ray
+template <typename R>
+void cast_fuv(const R* rho, const R* xi, int nx, int ny, int nz, const R star[3], R kappa, R dr, int nsteps,
+              R dx, R binr, const R bmax[3], R* att, int nr, int nu, int nphi, int nthreads) {
+  Grid g{nx, ny, nz};
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 4)
+  for (int iu = 0; iu < nu; ++iu)
+    for (int ip = 0; ip < nphi; ++ip) {
+      R d[3];
+      ray_dir(iu, ip, nu, nphi, d);
+      R tau = R(0), t0, t1;
+      int ib = 0;
+      box_range(star, d, bmax, &t0, &t1);
+      int n0 = (int)(t0 / dr);
+      int n1 = std::min(nsteps, (int)(t1 / dr) + 2);
+      for (int n = n0; n < n1; ++n) {
+        R r = (R(n) + R(0.5)) * dr;
+        R qx = star[0] + d[0] * r, qy = star[1] + d[1] * r, qz = star[2] + d[2] * r;
+        R n_tot = sample_n(rho, g, qx, qy, qz, dx);
+        R x_i = sample_n(xi, g, qx, qy, qz, dx);
+        tau = tau + kappa * n_tot * (R(1) - R(0.92) * x_i) * dr;
+        int nb = (int)(r / binr);
+        if (nb > ib) {
+          R e = t_exp(-tau);
+          for (int b = ib; b < std::min(nb, nr); ++b) att[((size_t)b * nu + iu) * nphi + ip] = e;
+          ib = nb;
+        }
+        if (ib >= nr) break;
+      }
+      R e = t_exp(-tau);
+      for (int b = ib; b < nr; ++b) att[((size_t)b * nu + iu) * nphi + ip] = e;
+    }
+}
+
+// apply_fuv (nebula3d.py:355-388): g0 += lum * trilinear(att) / max(r^2, 4e-4)
+template <typename R>
+void apply_fuv(const R* att, int nr, int nu, int nphi, const R star[3], R lum, R dx, R binr, R* g0, int nx,
+               int ny, int nz, int nthreads) {
+  Grid g{nx, ny, nz};
+  const R pi = R(3.14159265358979323846);
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+  for (int i = 0; i < nx; ++i)
+    for (int j = 0; j < ny; ++j)
+      for (int k = 0; k < nz; ++k) {
+        R r, u, phi;
+        cell_sph(i, j, k, star, dx, &r, &u, &phi);
+        R fr = r / binr - R(0.5);
+        R fu = (u + R(1)) * R(0.5) * R(nu) - R(0.5);
+        R fp = phi / (R(2) * pi) * R(nphi) - R(0.5);
+        int ir = (int)floor(fr), iu = (int)floor(fu), ip = (int)floor(fp);
+        R tr = fr - R(ir), tu = fu - R(iu), tp = fp - R(ip);
+        R acc = R(0);
+        for (int a = 0; a < 2; ++a)
+          for (int b = 0; b < 2; ++b)
+            for (int c = 0; c < 2; ++c) {
+              R w = (a == 0 ? R(1) - tr : tr) * (b == 0 ? R(1) - tu : tu) * (c == 0 ? R(1) - tp : tp);
+              int rr = iclamp(ir + a, 0, nr - 1), uu = iclamp(iu + b, 0, nu - 1);
+              int pp = (((ip + c) % nphi) + nphi) % nphi;
+              acc = acc + w * att[((size_t)rr * nu + uu) * nphi + pp];
+            }
+        R r2 = rmax(r * r, R(0.0004));
+        size_t o = g.at(i, j, k);
+        g0[o] = g0[o] + lum * acc / r2;
+      }
+}
+
+// IEEE binary32 -> binary16, round to nearest even (Warp float16 store, sim.py:511-514)
+inline uint16_t f32_to_f16(float f) {
+  uint32_t x;
+  memcpy(&x, &f, 4);
+  uint32_t sign = (x >> 16) & 0x8000u;
+  uint32_t ax = x & 0x7fffffffu;
+  if (ax >= 0x7f800000u) return (uint16_t)(sign | 0x7c00u | (ax > 0x7f800000u ? 0x200u : 0u));  // inf / nan
+  if (ax >= 0x477ff000u) return (uint16_t)(sign | 0x7c00u);  // rounds to >= 65520 -> inf
+  if (ax < 0x38800000u) {                                    // result is subnormal or zero in half
+    if (ax < 0x33000000u) return (uint16_t)sign;             // < 2^-25 rounds to zero
+    uint32_t e = ax >> 23;
+    uint32_t m = (ax & 0x7fffffu) | 0x800000u;
+    uint32_t shift = 126u - e;                               // 14 + (113 - e) + ... : m >> (126-e) -> units of 2^-24
+    uint32_t q = m >> shift;
+    uint32_t rem = m & ((1u << shift) - 1u);
+    uint32_t half = 1u << (shift - 1u);
+    if (rem > half || (rem == half && (q & 1u))) ++q;
+    return (uint16_t)(sign | q);
+  }
+  uint32_t e = (ax >> 23) - 112u;                            // rebias 127 -> 15
+  uint32_t m = ax & 0x7fffffu;
+  uint32_t q = (e << 10) | (m >> 13);
+  uint32_t rem = m & 0x1fffu;
+  if (rem > 0x1000u || (rem == 0x1000u && (q & 1u))) ++q;   // carries into the exponent correctly
+  return (uint16_t)(sign | q);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------------------ C ABI
+extern "C" {
+
+#define OR_HYDRO_ARGS(R)                                                                                        \
+  const R *rho, const R *mx, const R *my, const R *mz, const R *xi, const R *r0, const R *mx0, const R *my0,   \
+      const R *mz0, R *o_r, R *o_mx, R *o_my, R *o_mz, int nx, int ny, int nz, const R *stars, int nstars,      \
+      R wind, const R *jets, R jet_cos, R jet_r, R pulse, R dt, R dx, R a, R b, R floor, R cn2, R ci2, R vmax, \
+      int solver, int lim, int nthreads
+#define OR_HYDRO_PASS                                                                                          \
+  rho, mx, my, mz, xi, r0, mx0, my0, mz0, o_r, o_mx, o_my, o_mz, nx, ny, nz, stars, nstars, wind, jets, jet_cos, \
+      jet_r, pulse, dt, dx, a, b, floor, cn2, ci2, vmax, solver, lim, nthreads
+
+void or_hydro_stage_f32(OR_HYDRO_ARGS(float)) { hydro_stage<float>(OR_HYDRO_PASS); }
+void or_hydro_stage_f64(OR_HYDRO_ARGS(double)) { hydro_stage<double>(OR_HYDRO_PASS); }
+
+float or_max_speed_f32(const float* rho, const float* mx, const float* my, const float* mz, const float* xi,
+                       long long n, float cn2, float ci2, int nthreads) {
+  return max_speed<float>(rho, mx, my, mz, xi, (size_t)n, cn2, ci2, nthreads);
+}
+double or_max_speed_f64(const double* rho, const double* mx, const double* my, const double* mz,
+                        const double* xi, long long n, double cn2, double ci2, int nthreads) {
+  return max_speed<double>(rho, mx, my, mz, xi, (size_t)n, cn2, ci2, nthreads);
+}
+
+#define OR_RAYS(R, S)                                                                                          \
+  void or_cast_rays_##S(const R* rho, int nx, int ny, int nz, R sx, R sy, R sz, R Q, R dr, int nsteps, R dx,   \
+                        R rcap, R bx, R by, R bz, R* rfront, int nu, int nphi, int nthreads) {                 \
+    const R st[3] = {sx, sy, sz}, bm[3] = {bx, by, bz};                                                        \
+    cast_rays<R>(rho, nx, ny, nz, st, Q, dr, nsteps, dx, rcap, bm, rfront, nu, nphi, nthreads);                \
+  }                                                                                                            \
+  void or_apply_front_##S(const R* rfront, int nu, int nphi, R sx, R sy, R sz, R dx, R width, R* xi, R* dfront, \
+                          int nx, int ny, int nz, int nthreads) {                                              \
+    const R st[3] = {sx, sy, sz};                                                                              \
+    apply_front<R>(rfront, nu, nphi, st, dx, width, xi, dfront, nx, ny, nz, nthreads);                         \
+  }                                                                                                            \
+  void or_smooth_xi_##S(const R* src, R* dst, int nx, int ny, int nz, int nthreads) {                          \
+    smooth_xi<R>(src, dst, nx, ny, nz, nthreads);                                                              \
+  }                                                                                                            \
+  void or_cast_fuv_##S(const R* rho, const R* xi, int nx, int ny, int nz, R sx, R sy, R sz, R kappa, R dr,      \
+                       int nsteps, R dx, R binr, R bx, R by, R bz, R* att, int nr, int nu, int nphi,           \
+                       int nthreads) {                                                                         \
+    const R st[3] = {sx, sy, sz}, bm[3] = {bx, by, bz};                                                        \
+    cast_fuv<R>(rho, xi, nx, ny, nz, st, kappa, dr, nsteps, dx, binr, bm, att, nr, nu, nphi, nthreads);       \
+  }                                                                                                            \
+  void or_apply_fuv_##S(const R* att, int nr, int nu, int nphi, R sx, R sy, R sz, R lum, R dx, R binr, R* g0,  \
+                        int nx, int ny, int nz, int nthreads) {                                                \
+    const R st[3] = {sx, sy, sz};                                                                              \
+    apply_fuv<R>(att, nr, nu, nphi, st, lum, dx, binr, g0, nx, ny, nz, nthreads);                              \
+  }                                                                                                            \
+  R or_sample_n_##S(const R* f, int nx, int ny, int nz, R px, R py, R pz, R dx) {                              \
+    Grid g{nx, ny, nz};                                                                                        \
+    return sample_n<R>(f, g, px, py, pz, dx);                                                                  \
+  }                                                                                                            \
+  void or_box_range_##S(R ox, R oy, R oz, R dx_, R dy_, R dz_, R bx, R by, R bz, R* t01) {                     \
+    const R o[3] = {ox, oy, oz}, d[3] = {dx_, dy_, dz_}, bm[3] = {bx, by, bz};                                 \
+    box_range<R>(o, d, bm, t01, t01 + 1);                                                                      \
+  }                                                                                                            \
+  void or_face_flux_##S(const R* q16, R c2b, R c2c, int axis, R floor, int solver, int lim, R* f4) {           \
+    V4<R> qa{{q16[0], q16[1], q16[2], q16[3]}}, qb{{q16[4], q16[5], q16[6], q16[7]}};                          \
+    V4<R> qc{{q16[8], q16[9], q16[10], q16[11]}}, qd{{q16[12], q16[13], q16[14], q16[15]}};                    \
+    R e[4] = {R(0), R(axis == 0), R(axis == 1), R(axis == 2)};                                                 \
+    V4<R> F = face_flux<R>(qa, qb, qc, qd, c2b, c2c, e, floor, solver, lim);                                   \
+    for (int c = 0; c < 4; ++c) f4[c] = F.c[c];                                                                \
+  }                                                                                                            \
+  R or_slope_##S(R a, R b, int lim) { return slope<R>(a, b, lim); }
+
+OR_RAYS(float, f32)
+OR_RAYS(double, f64)
+
+void or_to_half(const float* src, uint16_t* dst, long long n, int nthreads) {
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+  for (long long q = 0; q < n; ++q) dst[q] = f32_to_f16(src[q]);
+}
+
+// deterministic transcendentals, exported so the tests can bound their error against libm
+float or_sinf_det(float x) { return nvm::sinf_det(x); }
+float or_cosf_det(float x) { return nvm::cosf_det(x); }
+float or_atan2f_det(float y, float x) { return nvm::atan2f_det(y, x); }
+float or_expf_det(float x) { return nvm::expf_det(x); }
+
+}  // extern "C"
