@@ -118,8 +118,9 @@ def test_hydro_stage_no_stars(env, okern32):
     okern32.hydro_stage(U, xi, U, ref, shape, np.zeros((1, 4), F32), 0, 0.0, np.zeros((1, 4), F32), 0.9, 0.04, 1.0,
                         0.001, 1 / 70, 0.0, 1.0, 2e-3, 1 / 144, 1.0, 6.0, 1, 0)
     gU = [T(env, u) for u in U]
+    gxi = T(env, xi)
     gout = [torch.zeros(shape, dtype=torch.float32, device=dev) for _ in range(4)]
-    L.call("nv_hydro_stage", *[t.data_ptr() for t in gU], T(env, xi).data_ptr(), *[t.data_ptr() for t in gU],
+    L.call("nv_hydro_stage", *[t.data_ptr() for t in gU], gxi.data_ptr(), *[t.data_ptr() for t in gU],
            *[t.data_ptr() for t in gout], *shape, None, 0, f(0), None, f(0.9), f(0.04), f(1), f(0.001), f(1 / 70),
            f(0), f(1), f(2e-3), f(1 / 144), f(1), f(6), 1, 0, None, stream(env))
     torch.cuda.synchronize()
@@ -161,7 +162,8 @@ def test_cast_rays_bitwise(env, okern32, star, Q):
     ref = np.zeros((nu, nphi), F32)
     okern32.cast_rays(rho, star, Q, dr, nsteps, dx, diag, Lb, ref)
     g = torch.zeros((nu, nphi), dtype=torch.float32, device=dev)
-    L.call("nv_cast_rays", T(env, rho).data_ptr(), *shape, *(f(v) for v in star), f(Q), f(dr), nsteps, f(dx), f(diag),
+    grho = T(env, rho)
+    L.call("nv_cast_rays", grho.data_ptr(), *shape, *(f(v) for v in star), f(Q), f(dr), nsteps, f(dx), f(diag),
            *(f(v) for v in Lb), g.data_ptr(), nu, nphi, stream(env))
     np.testing.assert_array_equal(N(g), ref)
 
@@ -178,7 +180,8 @@ def test_apply_front_smooth_half_fill_bitwise(env, okern32):
     star = (0.45, 0.35, 0.55)
     dx = 1.0 / 44
     okern32.apply_front(rf, star, dx, 1.2 * dx, xi, df)
-    L.call("nv_apply_front", T(env, rf).data_ptr(), 32, 64, *(f(v) for v in star), f(dx), f(1.2 * dx),
+    grf = T(env, rf)
+    L.call("nv_apply_front", grf.data_ptr(), 32, 64, *(f(v) for v in star), f(dx), f(1.2 * dx),
            gxi.data_ptr(), gdf.data_ptr(), *shape, stream(env))
     np.testing.assert_array_equal(N(gxi), xi)
     np.testing.assert_array_equal(N(gdf), df)
@@ -190,7 +193,8 @@ def test_apply_front_smooth_half_fill_bitwise(env, okern32):
     vals = np.concatenate([rng.standard_normal(200000) * 10.0 ** rng.uniform(-9, 5.5, 200000),
                            [65504, 65520, 2 ** -25, 3 * 2 ** -26, 1e-30, 7e4, -7e4, 0.0]]).astype(F32)
     h = torch.zeros(vals.size, dtype=torch.int16, device=dev)
-    L.call("nv_to_half", T(env, vals).data_ptr(), h.data_ptr(), vals.size, stream(env))
+    gvals = T(env, vals)
+    L.call("nv_to_half", gvals.data_ptr(), h.data_ptr(), vals.size, stream(env))
     np.testing.assert_array_equal(N(h).view(np.uint16), okern32.to_half(vals).view(np.uint16))
     L.call("nv_fill_f32", gout.data_ptr(), f(1e3), gout.numel(), stream(env))
     assert bool((gout == 1e3).all())
@@ -271,8 +275,11 @@ def test_fuv_bitwise(env, okern32):
                gacc.data_ptr(), *shape, stream(env))
     np.testing.assert_array_equal(N(gacc), acc)
     g0 = torch.zeros(shape, dtype=torch.float16, device=dev)
-    L.call("nv_fuv_apply", gatt.data_ptr(), *fg, T(env, np.array(stars, F32)).data_ptr(),
-           T(env, np.array(lums, F32)).data_ptr(), 2, f(dx), f(binr), g0.data_ptr(), None, *shape, stream(env))
+    # keep the uploaded tables referenced until the kernel has run: a temporary tensor's block goes back to the
+    # caching allocator at once and the next upload may reuse it before the launch reads it
+    gxyz, glum = T(env, np.array(stars, F32)), T(env, np.array(lums, F32))
+    L.call("nv_fuv_apply", gatt.data_ptr(), *fg, gxyz.data_ptr(), glum.data_ptr(), 2, f(dx), f(binr), g0.data_ptr(),
+           None, *shape, stream(env))
     np.testing.assert_array_equal(N(g0).view(np.uint16), okern32.to_half(acc).view(np.uint16))
 
 
