@@ -5,27 +5,35 @@
 // max_speed (nebula3d.py:154-162).
 //
 // Layout: structure-of-arrays fp32 fields, C-order [nx][ny][nz] (z fastest), as the reference stores them.
-// Tiling: a workgroup owns a TK x TJ column of cells in (z, y) and marches along x over `ci` planes. Each
-// x-plane is loaded once (with a 2-cell halo in y and z), converted to primitives (the 3 divisions of
-// nebula3d.prim happen once per loaded cell, not 15 times per cell as in the reference stencil) and kept in
-// an LDS ring of 5 planes; the x-direction stencil reads the ring, y and z read the current plane. The x-face
-// flux is computed once per face (the right face of plane i is the left face of plane i+1, carried in
-// registers); the y/z faces are evaluated per cell.
 //
-// Arithmetic follows the reference expression by expression with no FMA contraction (-ffp-contract=off) and
-// IEEE division/sqrt, so results equal the CPU oracle bit for bit (up to the sign of zero, which never feeds
-// a nonzero result: every divisor here is strictly positive).
+// Work decomposition. A workgroup owns a TK x TJ column of cells in (z, y) - one wavefront per y row, the 64
+// lanes along the contiguous z axis - and marches along x over `ci` planes. Per plane:
+//   * every thread keeps its own cell column's primitives (rho floored, u, v, w) and c^2, c for planes
+//     i-1 .. i+2 in registers (a sliding window: the x stencil needs no neighbours and no halo), so the three
+//     divisions of nebula3d.prim and the square root of c^2 happen once per cell;
+//   * the plane's primitives go to LDS with a 2-cell y/z halo (the halo cells are the only ones converted
+//     twice: (TJ+4)(TK+4) - TJ TK of them per TJ TK cells);
+//   * each face flux is evaluated once: the x face i+1/2 in registers (it is the left face of plane i+1), the
+//     low y and z faces of every cell from LDS into LDS face buffers, plus the TK + TJ faces on the tile's
+//     high y and z boundaries; a cell then differences its two faces per axis.
+// The reference evaluates six face fluxes per cell (each face twice) from 63 clamped loads; the arithmetic per
+// face is identical, so the results are the same bits.
+//
+// Exactness: no FMA contraction (-ffp-contract=off), IEEE square root, and every division either IEEE or the
+// shared-reciprocal form of nv_divide.h, which returns the identical correctly rounded quotient. Results equal
+// the CPU oracle bit for bit (up to the sign of zero, which never feeds a nonzero result here).
 #include "nv_common.h"
+#include "nv_divide.h"
 
 namespace nv {
 
-constexpr int TK = 64;              // z extent of a tile: one wavefront along the contiguous axis
-constexpr int TJ = 4;               // y rows of a tile (one wavefront each)
-constexpr int NT = TK * TJ;         // threads per workgroup
-constexpr int PK = TK + 4, PJ = TJ + 4, PN = PK * PJ;   // primitive plane with a 2-cell halo
-constexpr int CK = TK + 2, CJ = TJ + 2, CN = CK * CJ;   // c^2 plane with a 1-cell halo
-constexpr int NQ = 5;               // primitive plane ring (planes i-2 .. i+2 live)
-constexpr int NC = 3;               // c^2 plane ring (planes i, i+1 read; i+1 written)
+constexpr int TK = 64;                      // z extent of a tile: one wavefront along the contiguous axis
+constexpr int TJ = 8;                       // y rows per tile, one wavefront each
+constexpr int NT = TK * TJ;                 // threads per workgroup
+constexpr int PK = TK + 4, PJ = TJ + 4;     // primitive plane with a 2-cell halo
+constexpr int CK = TK + 2, CJ = TJ + 2;     // c^2 / c plane with a 1-cell halo
+constexpr int NHALO = 4 * PK + 4 * TJ;      // halo cells of the primitive plane
+constexpr int NCHALO = 2 * CK + 2 * TJ;     // halo cells of the c plane
 
 struct HydroArgs {
   const float *rho, *mx, *my, *mz, *xi, *r0, *mx0, *my0, *mz0;
@@ -53,18 +61,25 @@ __device__ __forceinline__ float slope(float a, float b) {
   }
 }
 
-// Face flux between cells b and c along axis AX (0=x, 1=y, 2=z) from the primitive states of cells a, b, c,
-// d. HLL (sim.py:397-420) or Rusanov (nebula3d.py:54-72; always minmod). The one-hot axis vector e of the
-// reference is folded in at compile time: dot(v, e) = v[AX+1], e * (r c^2) touches component AX+1 only.
+__device__ __forceinline__ float comp(const float4& q, int c) {
+  return c == 0 ? q.x : (c == 1 ? q.y : (c == 2 ? q.z : q.w));
+}
+
+// Face flux between cells b and c along axis AX (0=x, 1=y, 2=z) from the primitive states of cells a, b, c, d
+// and c^2, c of cells b and c. HLL (sim.py:397-420) or Rusanov (nebula3d.py:54-72, always minmod). The
+// reference's one-hot axis vector e is folded in at compile time: dot(v, e) = v[AX+1], e*(r c^2) touches
+// component AX+1 only.
 template <int AX, int SOLVER, int LIM>
-__device__ __forceinline__ void face_flux(const float* qa, const float* qb, const float* qc, const float* qd,
-                                          float c2b, float c2c, float floor_, float* F) {
+__device__ __forceinline__ float4 face_flux(const float4& qa, const float4& qb, const float4& qc, const float4& qd,
+                                            float c2b, float c2c, float cb, float cc, float floor_) {
   constexpr int LM = SOLVER == NV_SOLVER_HLL ? LIM : NV_LIM_MINMOD;
   float L[4], R[4];
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
-    L[c] = qb[c] + 0.5f * slope<LM>(qb[c] - qa[c], qc[c] - qb[c]);
-    R[c] = qc[c] - 0.5f * slope<LM>(qc[c] - qb[c], qd[c] - qc[c]);
+    const float a = comp(qa, c), b = comp(qb, c), cv = comp(qc, c), d = comp(qd, c);
+    const float d1 = b - a, d2 = cv - b, d3 = d - cv;
+    L[c] = b + 0.5f * slope<LM>(d1, d2);
+    R[c] = cv - 0.5f * slope<LM>(d2, d3);
   }
   const float rL = fmaxf(L[0], floor_), rR = fmaxf(R[0], floor_);
   const float unL = L[AX + 1], unR = R[AX + 1];
@@ -78,13 +93,13 @@ __device__ __forceinline__ void face_flux(const float* qa, const float* qb, cons
   }
   FL[AX + 1] = FL[AX + 1] + rL * c2b;
   FR[AX + 1] = FR[AX + 1] + rR * c2c;
+  float F[4];
   if constexpr (SOLVER == NV_SOLVER_RUSANOV) {
-    const float amax = fmaxf(fabsf(unL) + sqrtf(c2b), fabsf(unR) + sqrtf(c2c));
+    const float amax = fmaxf(fabsf(unL) + cb, fabsf(unR) + cc);
     const float ha = 0.5f * amax;
 #pragma unroll
     for (int c = 0; c < 4; ++c) F[c] = 0.5f * (FL[c] + FR[c]) - ha * (UR[c] - UL[c]);
   } else {
-    const float cb = sqrtf(c2b), cc = sqrtf(c2c);
     const float SL = fminf(unL - cb, unR - cc);
     const float SR = fmaxf(unL + cb, unR + cc);
     if (SL >= 0.0f) {
@@ -94,19 +109,28 @@ __device__ __forceinline__ void face_flux(const float* qa, const float* qb, cons
 #pragma unroll
       for (int c = 0; c < 4; ++c) F[c] = FR[c];
     } else {
-      const float ss = SL * SR, den = SR - SL;
+      const float ss = SL * SR;
+      const double rden = recip_d(SR - SL);
 #pragma unroll
-      for (int c = 0; c < 4; ++c) F[c] = ((SR * FL[c] - SL * FR[c]) + ss * (UR[c] - UL[c])) / den;
+      for (int c = 0; c < 4; ++c) F[c] = div_r((SR * FL[c] - SL * FR[c]) + ss * (UR[c] - UL[c]), rden);
     }
   }
+  return make_float4(F[0], F[1], F[2], F[3]);
 }
 
-__device__ __forceinline__ int ring(int p, int n) { return ((p % n) + n) % n; }
+// primitive state of a cell (nebula3d.prim, nebula3d.py:28-32)
+__device__ __forceinline__ float4 prim(float rho, float mx, float my, float mz, float floor_) {
+  const float r = fmaxf(rho, floor_);
+  const double rr = recip_d(r);
+  return make_float4(r, div_r(mx, rr), div_r(my, rr), div_r(mz, rr));
+}
 
 template <int SOLVER, int LIM, bool SMAX>
-__global__ __launch_bounds__(NT) void hydro_stage_kernel(HydroArgs A) {
-  __shared__ float sq[NQ][4][PN];
-  __shared__ float sc[NC][CN];
+__global__ __launch_bounds__(NT, 2) void hydro_stage_kernel(HydroArgs A) {
+  __shared__ float4 sq[PJ * PK];            // primitives of plane i, 2-cell halo
+  __shared__ float2 sc[CJ * CK];            // (c^2, c) of plane i, 1-cell halo
+  __shared__ float4 fy[(TJ + 1) * TK];      // y-face fluxes: row r is the face between rows r-1 and r
+  __shared__ float4 fz[TJ * (TK + 1)];      // z-face fluxes: column t is the face between columns t-1 and t
   __shared__ float red[NT / 64];
 
   const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TK + tx;
@@ -115,100 +139,157 @@ __global__ __launch_bounds__(NT) void hydro_stage_kernel(HydroArgs A) {
   const int i1 = min(i0 + A.ci, nx);
   const int k = k0 + tx, j = j0 + ty;
   const bool active = (k < nz) && (j < ny);
-  const float floor_ = A.floor_, cn2 = A.cn2, ci2 = A.ci2, dci = ci2 - cn2;
+  const long long plane = (long long)ny * nz;
+  // own column, clamped (threads past the domain edge still supply the clamped ghost values of ldc)
+  const long long col = (long long)min(j, ny - 1) * nz + min(k, nz - 1);
+  const float floor_ = A.floor_, cn2 = A.cn2, dci = A.ci2 - A.cn2;
 
-  // primitive plane p (clamped as ldc, nebula3d.py:23-25) into its ring slot
-  auto load_q = [&](int p) {
-    const int s = ring(p, NQ);
-    const long long base = (long long)clampi(p, 0, nx - 1) * ny * nz;
-    for (int t = tid; t < PN; t += NT) {
-      const int jj = t / PK, kk = t - jj * PK;
-      const long long o = base + (long long)clampi(j0 + jj - 2, 0, ny - 1) * nz + clampi(k0 + kk - 2, 0, nz - 1);
-      const float r = fmaxf(A.rho[o], floor_);
-      sq[s][0][t] = r;
-      sq[s][1][t] = A.mx[o] / r;
-      sq[s][2][t] = A.my[o] / r;
-      sq[s][3][t] = A.mz[o] / r;
-    }
+  // register window of the own column: planes i-1, i, i+1, i+2
+  float4 qw[4];
+  float c2w[2], cw[2];          // c^2 and c of planes i, i+1
+  float4 Uraw[3];               // conserved state of planes i, i+1, i+2
+  float xraw[3];                // xi of planes i, i+1, i+2
+  auto load_col = [&](int p, float4& U, float& x) {
+    const long long o = (long long)clampi(p, 0, nx - 1) * plane + col;
+    U = make_float4(A.rho[o], A.mx[o], A.my[o], A.mz[o]);
+    x = A.xi[o];
   };
-  // c^2 plane (csq, nebula3d.py:35-37)
-  auto load_c = [&](int p) {
-    const int s = ring(p, NC);
-    const long long base = (long long)clampi(p, 0, nx - 1) * ny * nz;
-    for (int t = tid; t < CN; t += NT) {
-      const int jj = t / CK, kk = t - jj * CK;
-      const long long o = base + (long long)clampi(j0 + jj - 1, 0, ny - 1) * nz + clampi(k0 + kk - 1, 0, nz - 1);
-      sc[s][t] = cn2 + A.xi[o] * dci;
-    }
-  };
-  auto getq = [&](int p, int dj, int dk, float* q) {
-    const int s = ring(p, NQ), t = (ty + 2 + dj) * PK + tx + 2 + dk;
-#pragma unroll
-    for (int c = 0; c < 4; ++c) q[c] = sq[s][c][t];
-  };
-  auto getc = [&](int p, int dj, int dk) { return sc[ring(p, NC)][(ty + 1 + dj) * CK + tx + 1 + dk]; };
 
-  load_q(i0 - 2);
-  load_q(i0 - 1);
-  load_q(i0);
-  load_q(i0 + 1);
-  load_c(i0 - 1);
-  load_c(i0);
-  __syncthreads();
-
-  float fLx[4];
+  // prologue: planes i0-2 .. i0+1 give the left x face of plane i0
+  float4 fLx;
   {
-    float qa[4], qb[4], qc[4], qd[4];
-    getq(i0 - 2, 0, 0, qa);
-    getq(i0 - 1, 0, 0, qb);
-    getq(i0, 0, 0, qc);
-    getq(i0 + 1, 0, 0, qd);
-    face_flux<0, SOLVER, LIM>(qa, qb, qc, qd, getc(i0 - 1, 0, 0), getc(i0, 0, 0), floor_, fLx);
+    float4 U;
+    float x;
+    float4 q4[4];
+    float c2p[2], cp[2];
+    for (int s = 0; s < 4; ++s) {
+      load_col(i0 - 2 + s, U, x);
+      q4[s] = prim(U.x, U.y, U.z, U.w, floor_);
+      if (s >= 1 && s <= 3) {
+        const float c2 = cn2 + x * dci;
+        if (s < 3) {
+          c2p[s - 1] = c2;
+          cp[s - 1] = sqrtf(c2);
+        } else {
+          c2w[1] = c2;   // plane i0+1
+          cw[1] = sqrtf(c2);
+        }
+        if (s >= 2) {
+          Uraw[s - 2] = U;
+          xraw[s - 2] = x;
+        }
+      }
+    }
+    fLx = face_flux<0, SOLVER, LIM>(q4[0], q4[1], q4[2], q4[3], c2p[0], c2p[1], cp[0], cp[1], floor_);
+    qw[0] = q4[1];
+    qw[1] = q4[2];
+    qw[2] = q4[3];
+    c2w[0] = c2p[1];
+    cw[0] = cp[1];
+  }
+  // plane i0+2 completes the window (qw[3])
+  {
+    load_col(i0 + 2, Uraw[2], xraw[2]);
+    qw[3] = prim(Uraw[2].x, Uraw[2].y, Uraw[2].z, Uraw[2].w, floor_);
   }
 
-  float tmax = 0.0f;
+  const double rdx = recip_d(A.dx);
   const float dx = A.dx;
-  for (int i = i0; i < i1; ++i) {
-    load_q(i + 2);
-    load_c(i + 1);
-    __syncthreads();
-    if (active) {
-      float div[4], fR[4], fL[4];
-      float q[5][4];
-      // x faces: right face from the ring, left face carried from the previous plane
-      getq(i - 1, 0, 0, q[0]);
-      getq(i, 0, 0, q[1]);
-      getq(i + 1, 0, 0, q[2]);
-      getq(i + 2, 0, 0, q[3]);
-      const float c0 = getc(i, 0, 0);
-      face_flux<0, SOLVER, LIM>(q[0], q[1], q[2], q[3], c0, getc(i + 1, 0, 0), floor_, fR);
-#pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        div[c] = 0.0f + (fR[c] - fLx[c]);
-        fLx[c] = fR[c];
-      }
-      // y faces
-#pragma unroll
-      for (int d = 0; d < 5; ++d) getq(i, d - 2, 0, q[d]);
-      face_flux<1, SOLVER, LIM>(q[1], q[2], q[3], q[4], c0, getc(i, 1, 0), floor_, fR);
-      face_flux<1, SOLVER, LIM>(q[0], q[1], q[2], q[3], getc(i, -1, 0), c0, floor_, fL);
-#pragma unroll
-      for (int c = 0; c < 4; ++c) div[c] = div[c] + (fR[c] - fL[c]);
-      // z faces
-#pragma unroll
-      for (int d = 0; d < 5; ++d) getq(i, 0, d - 2, q[d]);
-      face_flux<2, SOLVER, LIM>(q[1], q[2], q[3], q[4], c0, getc(i, 0, 1), floor_, fR);
-      face_flux<2, SOLVER, LIM>(q[0], q[1], q[2], q[3], getc(i, 0, -1), c0, floor_, fL);
-#pragma unroll
-      for (int c = 0; c < 4; ++c) div[c] = div[c] + (fR[c] - fL[c]);
+  float tmax = 0.0f;
 
-      const long long o = cell(i, j, k, ny, nz);
-      const float U[4] = {A.rho[o], A.mx[o], A.my[o], A.mz[o]};
+  for (int i = i0; i < i1; ++i) {
+    // next window plane (i+3): issued now, consumed at the end of this iteration
+    float4 Unext;
+    float xnext;
+    load_col(i + 3, Unext, xnext);
+    float4 U0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const long long o = (long long)i * plane + col;
+    if (!A.u0_is_u && active) U0 = make_float4(A.r0[o], A.mx0[o], A.my0[o], A.mz0[o]);
+
+    // ---- A: plane i into LDS (own cell from the window, halo from global)
+    sq[(ty + 2) * PK + tx + 2] = qw[1];
+    sc[(ty + 1) * CK + tx + 1] = make_float2(c2w[0], cw[0]);
+    const long long pbase = (long long)i * plane;
+    for (int h = tid; h < NHALO; h += NT) {
+      int jj, kk;
+      if (h < 2 * PK) {
+        jj = h / PK;
+        kk = h - jj * PK;
+      } else if (h < 4 * PK) {
+        const int hh = h - 2 * PK;
+        jj = TJ + 2 + hh / PK;
+        kk = hh - (hh / PK) * PK;
+      } else {
+        const int hh = h - 4 * PK;
+        jj = 2 + (hh >> 2);
+        const int c = hh & 3;
+        kk = c < 2 ? c : TK + c;
+      }
+      const long long oh = pbase + (long long)clampi(j0 + jj - 2, 0, ny - 1) * nz + clampi(k0 + kk - 2, 0, nz - 1);
+      sq[jj * PK + kk] = prim(A.rho[oh], A.mx[oh], A.my[oh], A.mz[oh], floor_);
+    }
+    for (int h = tid; h < NCHALO; h += NT) {
+      int jj, kk;
+      if (h < 2 * CK) {
+        jj = h < CK ? 0 : TJ + 1;
+        kk = h < CK ? h : h - CK;
+      } else {
+        const int hh = h - 2 * CK;
+        jj = 1 + (hh >> 1);
+        kk = (hh & 1) ? TK + 1 : 0;
+      }
+      const long long oh = pbase + (long long)clampi(j0 + jj - 1, 0, ny - 1) * nz + clampi(k0 + kk - 1, 0, nz - 1);
+      const float c2 = cn2 + A.xi[oh] * dci;
+      sc[jj * CK + kk] = make_float2(c2, sqrtf(c2));
+    }
+    __syncthreads();
+
+    // ---- C: face fluxes. x in registers; low y and z faces of every cell (+ tile-boundary faces) to LDS
+    const float4 fRx = face_flux<0, SOLVER, LIM>(qw[0], qw[1], qw[2], qw[3], c2w[0], c2w[1], cw[0], cw[1], floor_);
+    {
+      const int cq = (ty + 2) * PK + tx + 2;
+      const int cc = (ty + 1) * CK + tx + 1;
+      const float2 cy0 = sc[cc - CK], cy1 = sc[cc];
+      fy[ty * TK + tx] = face_flux<1, SOLVER, LIM>(sq[cq - 2 * PK], sq[cq - PK], sq[cq], sq[cq + PK], cy0.x, cy1.x,
+                                                   cy0.y, cy1.y, floor_);
+      const float2 cz0 = sc[cc - 1];
+      fz[ty * (TK + 1) + tx] = face_flux<2, SOLVER, LIM>(sq[cq - 2], sq[cq - 1], sq[cq], sq[cq + 1], cz0.x, cy1.x,
+                                                         cz0.y, cy1.y, floor_);
+      if (tid < TK) {
+        // y face between rows TJ-1 and TJ of the tile, column tid
+        const int bq = (TJ + 2) * PK + tid + 2;
+        const int bc = (TJ + 1) * CK + tid + 1;
+        const float2 a0 = sc[bc - CK], a1 = sc[bc];
+        fy[TJ * TK + tid] = face_flux<1, SOLVER, LIM>(sq[bq - 2 * PK], sq[bq - PK], sq[bq], sq[bq + PK], a0.x,
+                                                      a1.x, a0.y, a1.y, floor_);
+      } else if (tid < TK + TJ) {
+        // z face between columns TK-1 and TK of row r
+        const int r = tid - TK;
+        const int bq = (r + 2) * PK + TK + 2;
+        const int bc = (r + 1) * CK + TK + 1;
+        const float2 a0 = sc[bc - 1], a1 = sc[bc];
+        fz[r * (TK + 1) + TK] = face_flux<2, SOLVER, LIM>(sq[bq - 2], sq[bq - 1], sq[bq], sq[bq + 1], a0.x, a1.x,
+                                                          a0.y, a1.y, floor_);
+      }
+    }
+    __syncthreads();
+
+    // ---- E: update of the own cell
+    if (active) {
+      const float4 fLy = fy[ty * TK + tx], fRy = fy[(ty + 1) * TK + tx];
+      const float4 fLz = fz[ty * (TK + 1) + tx], fRz = fz[ty * (TK + 1) + tx + 1];
+      float div[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+        div[c] = ((0.0f + (comp(fRx, c) - comp(fLx, c))) + (comp(fRy, c) - comp(fLy, c))) +
+                 (comp(fRz, c) - comp(fLz, c));
+      const float4 Uc = Uraw[0];
+      const float U[4] = {Uc.x, Uc.y, Uc.z, Uc.w};
       float dU[4];
 #pragma unroll
-      for (int c = 0; c < 4; ++c) dU[c] = -div[c] / dx;
+      for (int c = 0; c < 4; ++c) dU[c] = div_r(-div[c], rdx);
       const float p0 = ((float)i + 0.5f) * dx, p1 = ((float)j + 0.5f) * dx, p2 = ((float)k + 0.5f) * dx;
-      const float x_ion = A.xi[o];
+      const float x_ion = xraw[0];
       if (A.wind > 0.0f && x_ion > 0.01f) {
         // radiation pressure / wind on ionized gas (sim.py:463-473)
         float acc0 = 0.0f, acc1 = 0.0f, acc2 = 0.0f;
@@ -251,21 +332,14 @@ __global__ __launch_bounds__(NT) void hydro_stage_kernel(HydroArgs A) {
       dU[2] = dU[2] + U[0] * aj1;
       dU[3] = dU[3] + U[0] * aj2;
 
-      float U0[4];
-      if (A.u0_is_u) {
-#pragma unroll
-        for (int c = 0; c < 4; ++c) U0[c] = U[c];
-      } else {
-        U0[0] = A.r0[o];
-        U0[1] = A.mx0[o];
-        U0[2] = A.my0[o];
-        U0[3] = A.mz0[o];
-      }
+      const float U0v[4] = {A.u0_is_u ? U[0] : U0.x, A.u0_is_u ? U[1] : U0.y, A.u0_is_u ? U[2] : U0.z,
+                            A.u0_is_u ? U[3] : U0.w};
       float out[4];
 #pragma unroll
-      for (int c = 0; c < 4; ++c) out[c] = A.a * U0[c] + A.b * (U[c] + A.dt * dU[c]);
+      for (int c = 0; c < 4; ++c) out[c] = A.a * U0v[c] + A.b * (U[c] + A.dt * dU[c]);
       const float r = fmaxf(out[0], floor_);
-      float v0 = out[1] / r, v1 = out[2] / r, v2 = out[3] / r;
+      const double rr = recip_d(r);
+      float v0 = div_r(out[1], rr), v1 = div_r(out[2], rr), v2 = div_r(out[3], rr);
       const float sp = sqrtf((v0 * v0 + v1 * v1) + v2 * v2);
       if (sp > A.vmax) {
         const float f = A.vmax / sp;
@@ -280,10 +354,28 @@ __global__ __launch_bounds__(NT) void hydro_stage_kernel(HydroArgs A) {
       A.o_mz[o] = m2;
       if constexpr (SMAX) {
         // max_speed of the written state (nebula3d.py:154-162): raw stored rho, L1 norm of v
-        const float s = ((fabsf(m0 / r) + fabsf(m1 / r)) + fabsf(m2 / r)) + sqrtf(cn2 + x_ion * dci);
+        const float s = ((fabsf(div_r(m0, rr)) + fabsf(div_r(m1, rr))) + fabsf(div_r(m2, rr))) +
+                        sqrtf(cn2 + x_ion * dci);
         tmax = fmaxf(tmax, s);
       }
     }
+    fLx = fRx;
+
+    // ---- shift the window by one plane
+    qw[0] = qw[1];
+    qw[1] = qw[2];
+    qw[2] = qw[3];
+    qw[3] = prim(Unext.x, Unext.y, Unext.z, Unext.w, floor_);
+    c2w[0] = c2w[1];
+    cw[0] = cw[1];
+    c2w[1] = cn2 + xraw[2] * dci;   // plane i+2 becomes plane (i+1)+1
+    cw[1] = sqrtf(c2w[1]);
+    Uraw[0] = Uraw[1];
+    Uraw[1] = Uraw[2];
+    Uraw[2] = Unext;
+    xraw[0] = xraw[1];
+    xraw[1] = xraw[2];
+    xraw[2] = xnext;
   }
   if constexpr (SMAX) block_max_atomic(tmax, red, A.smax);
 }
@@ -331,6 +423,7 @@ extern "C" int nv_hydro_stage(const float* rho, const float* mx, const float* my
   NV_REQUIRE(solver == NV_SOLVER_RUSANOV || solver == NV_SOLVER_HLL, "nv_hydro_stage: bad solver %d", solver);
   NV_REQUIRE(lim >= NV_LIM_MINMOD && lim <= NV_LIM_MC, "nv_hydro_stage: bad limiter %d", lim);
   NV_REQUIRE(o_r != rho && o_mx != mx && o_my != my && o_mz != mz, "nv_hydro_stage: output aliases the input U");
+  NV_REQUIRE(floor_ > 0.0f && dx > 0.0f, "nv_hydro_stage: floor and dx must be > 0");
   HydroArgs A;
   A.rho = rho; A.mx = mx; A.my = my; A.mz = mz; A.xi = xi;
   A.r0 = r0; A.mx0 = mx0; A.my0 = my0; A.mz0 = mz0;

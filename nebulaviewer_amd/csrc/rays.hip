@@ -12,6 +12,7 @@
 // Transcendentals come from nv_math.h so the CPU oracle reproduces them bit for bit.
 #include "nv_common.h"
 #include "nv_math.h"
+#include "nv_divide.h"
 
 namespace nv {
 
@@ -72,14 +73,50 @@ __device__ __forceinline__ void ray_dir(int iu, int ip, int nu, int nphi, float 
   d[2] = u;
 }
 
-// cast_rays (nebula3d.py:247-270): one thread per direction, phi fastest
+// sample_n with the division p/dx done through a shared reciprocal (the identical correctly rounded
+// quotient, nv_divide.h) and 32-bit cell indices (grids up to 2^31 cells)
+__device__ __forceinline__ float sample_r(const float* __restrict__ f, int nx, int ny, int nz, float px, float py,
+                                          float pz, double rdx) {
+  const float gx = div_r(px, rdx) - 0.5f, gy = div_r(py, rdx) - 0.5f, gz = div_r(pz, rdx) - 0.5f;
+  if (gx < -0.5f || gy < -0.5f || gz < -0.5f) return 0.0f;
+  if (gx > (float)nx - 0.5f || gy > (float)ny - 0.5f || gz > (float)nz - 0.5f) return 0.0f;
+  const int i0 = (int)floorf(gx), j0 = (int)floorf(gy), k0 = (int)floorf(gz);
+  const float fx = gx - (float)i0, fy = gy - (float)j0, fz = gz - (float)k0;
+  const unsigned ia = clampi(i0, 0, nx - 1), ib = clampi(i0 + 1, 0, nx - 1);
+  const unsigned ja = clampi(j0, 0, ny - 1), jb = clampi(j0 + 1, 0, ny - 1);
+  const unsigned ka = clampi(k0, 0, nz - 1), kb = clampi(k0 + 1, 0, nz - 1);
+  const unsigned r00 = (ia * ny + ja) * nz, r10 = (ib * ny + ja) * nz;
+  const unsigned r01 = (ia * ny + jb) * nz, r11 = (ib * ny + jb) * nz;
+  const float c000 = f[r00 + ka], c100 = f[r10 + ka], c010 = f[r01 + ka], c110 = f[r11 + ka];
+  const float c001 = f[r00 + kb], c101 = f[r10 + kb], c011 = f[r01 + kb], c111 = f[r11 + kb];
+  const float c00 = c000 + (c100 - c000) * fx;
+  const float c10 = c010 + (c110 - c010) * fx;
+  const float c01 = c001 + (c101 - c001) * fx;
+  const float c11 = c011 + (c111 - c011) * fx;
+  const float c0 = c00 + (c10 - c00) * fy;
+  const float c1 = c01 + (c11 - c01) * fy;
+  return c0 + (c1 - c0) * fz;
+}
+
+// Direction cell of thread t: each wavefront takes an 8 x 8 patch of (u, phi) cells, so its 64 rays stay
+// within a few cells of each other and their corner gathers share cache lines.
+__device__ __forceinline__ bool ray_of_thread(long long t, int nu, int nphi, int* iu, int* ip) {
+  const int tp = (nphi + 7) >> 3;
+  const long long patch = t >> 6;
+  const int lane = (int)(t & 63);
+  *iu = (int)(patch / tp) * 8 + (lane >> 3);
+  *ip = (int)(patch % tp) * 8 + (lane & 7);
+  return *iu < nu && *ip < nphi;
+}
+
+// cast_rays (nebula3d.py:247-270). Two samples per iteration: the corner loads of sample n+1 are in flight
+// while sample n is tested, and the column is still accumulated sample by sample in the reference order.
 __global__ __launch_bounds__(256) void cast_rays_kernel(const float* __restrict__ rho, int nx, int ny, int nz,
                                                         float sx, float sy, float sz, float Q, float dr,
                                                         int nsteps, float dx, float rcap, float bx, float by,
                                                         float bz, float* __restrict__ rfront, int nu, int nphi) {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long long)nu * nphi) return;
-  const int iu = (int)(t / nphi), ip = (int)(t - (long long)iu * nphi);
+  int iu, ip;
+  if (!ray_of_thread((long long)blockIdx.x * blockDim.x + threadIdx.x, nu, nphi, &iu, &ip)) return;
   float d[3];
   ray_dir(iu, ip, nu, nphi, d);
   const float st[3] = {sx, sy, sz}, bm[3] = {bx, by, bz};
@@ -87,18 +124,28 @@ __global__ __launch_bounds__(256) void cast_rays_kernel(const float* __restrict_
   box_range(st, d, bm, &t0, &t1);
   const int n0 = (int)(t0 / dr);
   const int n1 = min(nsteps, (int)(t1 / dr) + 2);
+  const double rdx = recip_d(dx);
   float col = 0.0f, rf = rcap;
-  for (int n = n0; n < n1; ++n) {
-    const float r = ((float)n + 0.5f) * dr;
-    const float nv = sample_n(rho, nx, ny, nz, sx + d[0] * r, sy + d[1] * r, sz + d[2] * r, dx);
-    const float inc = nv * nv * r * r * dr;
-    if (col + inc > Q) {
-      rf = r - 0.5f * dr + dr * (Q - col) / fmaxf(inc, 1e-20f);
+  for (int n = n0; n < n1; n += 2) {
+    const float ra = ((float)n + 0.5f) * dr;
+    const float rb = ((float)(n + 1) + 0.5f) * dr;
+    const float na = sample_r(rho, nx, ny, nz, sx + d[0] * ra, sy + d[1] * ra, sz + d[2] * ra, rdx);
+    const float nb = sample_r(rho, nx, ny, nz, sx + d[0] * rb, sy + d[1] * rb, sz + d[2] * rb, rdx);
+    const float ia = na * na * ra * ra * dr;
+    if (col + ia > Q) {
+      rf = ra - 0.5f * dr + dr * (Q - col) / fmaxf(ia, 1e-20f);
       break;
     }
-    col = col + inc;
+    col = col + ia;
+    if (n + 1 >= n1) break;
+    const float ib = nb * nb * rb * rb * dr;
+    if (col + ib > Q) {
+      rf = rb - 0.5f * dr + dr * (Q - col) / fmaxf(ib, 1e-20f);
+      break;
+    }
+    col = col + ib;
   }
-  rfront[t] = rf;
+  rfront[(long long)iu * nphi + ip] = rf;
 }
 
 // cell centre seen from a star: r, u = cos(theta), phi in [0, 2pi) (nebula3d.py:284-290)
@@ -367,7 +414,8 @@ extern "C" int nv_cast_rays(const float* rho, int nx, int ny, int nz, float sx, 
   NV_REQUIRE(rho && rfront, "nv_cast_rays: null pointer");
   NV_REQUIRE(nx > 0 && ny > 0 && nz > 0 && nu > 0 && nphi > 0, "nv_cast_rays: bad sizes");
   NV_REQUIRE(dr > 0.0f && dx > 0.0f, "nv_cast_rays: dr and dx must be > 0");
-  const long long n = (long long)nu * nphi;
+  NV_REQUIRE((long long)nx * ny * nz < (1LL << 32), "nv_cast_rays: grids above 2^32 cells are not supported");
+  const long long n = (long long)((nu + 7) / 8) * ((nphi + 7) / 8) * 64;
   hipLaunchKernelGGL(cast_rays_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, as_stream(stream), rho, nx, ny, nz, sx,
                      sy, sz, Q, dr, nsteps, dx, rcap, bx, by, bz, rfront, nu, nphi);
   return check_launch("nv_cast_rays");
