@@ -65,6 +65,21 @@ def cpu_baseline(setup, ion_every):
                        f"per step = hydro + ionization/{ion_every}")
 
 
+def copy_calibration(nbytes):
+    """A plain copy kernel of known size (reads and writes nbytes each launch), to scale the HBM byte counters."""
+    import torch
+    n = nbytes // 4
+    src = torch.rand(n, device="cuda:0")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / 3
+    print(json.dumps({"copy_bytes": n * 4, "ms": dt * 1e3, "GBps_read_plus_write": 2 * n * 4 / dt / 1e9}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -73,7 +88,12 @@ def main():
     ap.add_argument("--config", type=int, default=2)
     ap.add_argument("--n", type=int, default=None, help="override the grid size (testing only)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="counter calibration: run a plain device-to-device copy of this many bytes 3 times, exit")
     a = ap.parse_args()
+    if a.copy_bytes:
+        copy_calibration(a.copy_bytes)
+        return
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not supported (the step has no "
               "multi-GPU path)", file=sys.stderr)

@@ -41,7 +41,9 @@ struct HydroArgs {
   const float4 *stars, *jets;
   float* smax;
   int nx, ny, nz, nstars, ci, u0_is_u;
+  int tiles_k, tiles_j, tiles, per_xcd;   // tile grid and the number of tiles each XCD takes
   float wind, jet_cos, jet_r, pulse, dt, dx, a, b, floor_, cn2, ci2, vmax;
+  float vmax2_lo;                         // largest float <= vmax^2: |v|^2 <= vmax2_lo means no cap
 };
 
 // slope limiters: minmod (nebula3d.py:40-46), van Leer (sim.py:375-380), MC (sim.py:366-372)
@@ -56,7 +58,9 @@ __device__ __forceinline__ float slope(float a, float b) {
     const float v = s * fminf(fminf(2.0f * fabsf(a), 2.0f * fabsf(b)), 0.5f * fabsf(a + b));
     return p <= 0.0f ? 0.0f : v;
   } else {
-    const float m = fabsf(a) < fabsf(b) ? a : b;
+    // a*b > 0: a and b are nonzero with one sign, and the median of (a, b, 0) is the one nearer zero, which is
+    // the reference's choice (ties |a| == |b| mean a == b)
+    const float m = __builtin_amdgcn_fmed3f(a, b, 0.0f);
     return p <= 0.0f ? 0.0f : m;
   }
 }
@@ -65,22 +69,34 @@ __device__ __forceinline__ float comp(const float4& q, int c) {
   return c == 0 ? q.x : (c == 1 ? q.y : (c == 2 ? q.z : q.w));
 }
 
-// Face flux between cells b and c along axis AX (0=x, 1=y, 2=z) from the primitive states of cells a, b, c, d
-// and c^2, c of cells b and c. HLL (sim.py:397-420) or Rusanov (nebula3d.py:54-72, always minmod). The
-// reference's one-hot axis vector e is folded in at compile time: dot(v, e) = v[AX+1], e*(r c^2) touches
-// component AX+1 only.
-template <int AX, int SOLVER, int LIM>
-__device__ __forceinline__ float4 face_flux(const float4& qa, const float4& qb, const float4& qc, const float4& qd,
-                                            float c2b, float c2c, float cb, float cc, float floor_) {
-  constexpr int LM = SOLVER == NV_SOLVER_HLL ? LIM : NV_LIM_MINMOD;
-  float L[4], R[4];
+// Edge states of a cell from its own and its two neighbours' primitive states along one axis: the limited
+// slope s = lim(q - q-, q+ - q) is the same for the cell's low and high faces, so lo = q - s/2 is the right
+// state R of the face below (nebula3d.py:61, sim.py:402) and hi = q + s/2 the left state L of the face above
+// (nebula3d.py:60, sim.py:401). Computing both once per cell gives the reference's bits with half the limiter
+// evaluations.
+template <int LM>
+__device__ __forceinline__ void edges(const float4& qm, const float4& q, const float4& qp, float4& lo, float4& hi) {
+  float l[4], h[4];
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
-    const float a = comp(qa, c), b = comp(qb, c), cv = comp(qc, c), d = comp(qd, c);
-    const float d1 = b - a, d2 = cv - b, d3 = d - cv;
-    L[c] = b + 0.5f * slope<LM>(d1, d2);
-    R[c] = cv - 0.5f * slope<LM>(d2, d3);
+    const float v = comp(q, c);
+    const float sl = slope<LM>(v - comp(qm, c), comp(qp, c) - v);
+    l[c] = v - 0.5f * sl;
+    h[c] = v + 0.5f * sl;
   }
+  lo = make_float4(l[0], l[1], l[2], l[3]);
+  hi = make_float4(h[0], h[1], h[2], h[3]);
+}
+
+// Face flux along axis AX (0=x, 1=y, 2=z) from the reconstructed states L (high edge of the cell below) and R
+// (low edge of the cell above), with c^2 and c of those two cells. HLL (sim.py:397-420) or Rusanov
+// (nebula3d.py:54-72). The reference's one-hot axis vector e is folded in at compile time: dot(v, e) = v[AX+1],
+// e*(r c^2) touches component AX+1 only.
+template <int AX, int SOLVER>
+__device__ __forceinline__ float4 face_flux(const float4& Lq, const float4& Rq, float c2b, float c2c, float cb,
+                                            float cc, float floor_) {
+  const float L[4] = {Lq.x, Lq.y, Lq.z, Lq.w};
+  const float R[4] = {Rq.x, Rq.y, Rq.z, Rq.w};
   const float rL = fmaxf(L[0], floor_), rR = fmaxf(R[0], floor_);
   const float unL = L[AX + 1], unR = R[AX + 1];
   const float UL[4] = {rL, rL * L[1], rL * L[2], rL * L[3]};
@@ -125,17 +141,28 @@ __device__ __forceinline__ float4 prim(float rho, float mx, float my, float mz, 
   return make_float4(r, div_r(mx, rr), div_r(my, rr), div_r(mz, rr));
 }
 
-template <int SOLVER, int LIM, bool SMAX>
-__global__ __launch_bounds__(NT, 2) void hydro_stage_kernel(HydroArgs A) {
+// STAGE1: a = 0, b = 1 and U0 = U (sim.py:1025), so out = 0*U0 + 1*Un is Un (up to the sign of zero).
+template <int SOLVER, int LIM, bool SMAX, bool STAGE1>
+__global__ __launch_bounds__(NT, 4) void hydro_stage_kernel(HydroArgs A) {
+  constexpr int LM = SOLVER == NV_SOLVER_HLL ? LIM : NV_LIM_MINMOD;   // Rusanov always uses minmod
   __shared__ float4 sq[PJ * PK];            // primitives of plane i, 2-cell halo
   __shared__ float2 sc[CJ * CK];            // (c^2, c) of plane i, 1-cell halo
+  __shared__ float4 ehy[(TJ + 1) * TK];     // high y edge of rows -1 .. TJ-1 (row r+1 holds row r)
+  __shared__ float4 ehz[TJ * (TK + 1)];     // high z edge of columns -1 .. TK-1
+  __shared__ float4 ely[TK];                // low y edge of row TJ (the next tile's first row)
+  __shared__ float4 elz[TJ];                // low z edge of column TK
   __shared__ float4 fy[(TJ + 1) * TK];      // y-face fluxes: row r is the face between rows r-1 and r
   __shared__ float4 fz[TJ * (TK + 1)];      // z-face fluxes: column t is the face between columns t-1 and t
   __shared__ float red[NT / 64];
 
   const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TK + tx;
   const int nx = A.nx, ny = A.ny, nz = A.nz;
-  const int k0 = blockIdx.x * TK, j0 = blockIdx.y * TJ, i0 = blockIdx.z * A.ci;
+  // Workgroups are dealt to the 8 XCDs round robin (id % 8); give each XCD a contiguous run of tiles (z
+  // fastest, then y, then the x chunk) so the tiles whose halos overlap run on one XCD and share its L2.
+  const int tile = (blockIdx.x & 7) * A.per_xcd + (blockIdx.x >> 3);
+  if (tile >= A.tiles) return;
+  const int bk = tile % A.tiles_k, bj = (tile / A.tiles_k) % A.tiles_j, bi = tile / (A.tiles_k * A.tiles_j);
+  const int k0 = bk * TK, j0 = bj * TJ, i0 = bi * A.ci;
   const int i1 = min(i0 + A.ci, nx);
   const int k = k0 + tx, j = j0 + ty;
   const bool active = (k < nz) && (j < ny);
@@ -144,53 +171,39 @@ __global__ __launch_bounds__(NT, 2) void hydro_stage_kernel(HydroArgs A) {
   const long long col = (long long)min(j, ny - 1) * nz + min(k, nz - 1);
   const float floor_ = A.floor_, cn2 = A.cn2, dci = A.ci2 - A.cn2;
 
-  // register window of the own column: planes i-1, i, i+1, i+2
-  float4 qw[4];
-  float c2w[2], cw[2];          // c^2 and c of planes i, i+1
-  float4 Uraw[3];               // conserved state of planes i, i+1, i+2
-  float xraw[3];                // xi of planes i, i+1, i+2
   auto load_col = [&](int p, float4& U, float& x) {
     const long long o = (long long)clampi(p, 0, nx - 1) * plane + col;
     U = make_float4(A.rho[o], A.mx[o], A.my[o], A.mz[o]);
     x = A.xi[o];
   };
 
-  // prologue: planes i0-2 .. i0+1 give the left x face of plane i0
-  float4 fLx;
+  // register window of the own column: primitives of planes i, i+1, i+2; c^2, c of planes i, i+1; the
+  // conserved state and xi of planes i, i+1, i+2; the high x edge of plane i; the flux of face i-1/2
+  float4 qw[3], Uraw[3], hix, fLx;
+  float xraw[3], c2w[2], cw[2];
   {
-    float4 U;
-    float x;
-    float4 q4[4];
-    float c2p[2], cp[2];
-    for (int s = 0; s < 4; ++s) {
+    float4 U, q[4], lo, hm1;
+    float x, c2m1, cm1;
+    for (int s = 0; s < 4; ++s) {      // planes i0-2 .. i0+1
       load_col(i0 - 2 + s, U, x);
-      q4[s] = prim(U.x, U.y, U.z, U.w, floor_);
-      if (s >= 1 && s <= 3) {
-        const float c2 = cn2 + x * dci;
-        if (s < 3) {
-          c2p[s - 1] = c2;
-          cp[s - 1] = sqrtf(c2);
-        } else {
-          c2w[1] = c2;   // plane i0+1
-          cw[1] = sqrtf(c2);
-        }
-        if (s >= 2) {
-          Uraw[s - 2] = U;
-          xraw[s - 2] = x;
-        }
+      q[s] = prim(U.x, U.y, U.z, U.w, floor_);
+      if (s == 1) {
+        c2m1 = cn2 + x * dci;
+        cm1 = sqrtf(c2m1);
+      } else if (s >= 2) {
+        Uraw[s - 2] = U;
+        xraw[s - 2] = x;
+        c2w[s - 2] = cn2 + x * dci;
+        cw[s - 2] = sqrtf(c2w[s - 2]);
       }
     }
-    fLx = face_flux<0, SOLVER, LIM>(q4[0], q4[1], q4[2], q4[3], c2p[0], c2p[1], cp[0], cp[1], floor_);
-    qw[0] = q4[1];
-    qw[1] = q4[2];
-    qw[2] = q4[3];
-    c2w[0] = c2p[1];
-    cw[0] = cp[1];
-  }
-  // plane i0+2 completes the window (qw[3])
-  {
     load_col(i0 + 2, Uraw[2], xraw[2]);
-    qw[3] = prim(Uraw[2].x, Uraw[2].y, Uraw[2].z, Uraw[2].w, floor_);
+    qw[0] = q[2];
+    qw[1] = q[3];
+    qw[2] = prim(Uraw[2].x, Uraw[2].y, Uraw[2].z, Uraw[2].w, floor_);
+    edges<LM>(q[0], q[1], q[2], lo, hm1);   // plane i0-1
+    edges<LM>(q[1], q[2], q[3], lo, hix);   // plane i0
+    fLx = face_flux<0, SOLVER>(hm1, lo, c2m1, c2w[0], cm1, cw[0], floor_);
   }
 
   const double rdx = recip_d(A.dx);
@@ -204,11 +217,13 @@ __global__ __launch_bounds__(NT, 2) void hydro_stage_kernel(HydroArgs A) {
     load_col(i + 3, Unext, xnext);
     float4 U0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     const long long o = (long long)i * plane + col;
-    if (!A.u0_is_u && active) U0 = make_float4(A.r0[o], A.mx0[o], A.my0[o], A.mz0[o]);
+    if (!STAGE1 && !A.u0_is_u && active) U0 = make_float4(A.r0[o], A.mx0[o], A.my0[o], A.mz0[o]);
 
     // ---- A: plane i into LDS (own cell from the window, halo from global)
-    sq[(ty + 2) * PK + tx + 2] = qw[1];
-    sc[(ty + 1) * CK + tx + 1] = make_float2(c2w[0], cw[0]);
+    const int cq = (ty + 2) * PK + tx + 2;
+    const int cc = (ty + 1) * CK + tx + 1;
+    sq[cq] = qw[0];
+    sc[cc] = make_float2(c2w[0], cw[0]);
     const long long pbase = (long long)i * plane;
     for (int h = tid; h < NHALO; h += NT) {
       int jj, kk;
@@ -244,37 +259,61 @@ __global__ __launch_bounds__(NT, 2) void hydro_stage_kernel(HydroArgs A) {
     }
     __syncthreads();
 
-    // ---- C: face fluxes. x in registers; low y and z faces of every cell (+ tile-boundary faces) to LDS
-    const float4 fRx = face_flux<0, SOLVER, LIM>(qw[0], qw[1], qw[2], qw[3], c2w[0], c2w[1], cw[0], cw[1], floor_);
+    // ---- B: edge states. Own cell in y and z; the tile's outer neighbours for the boundary faces.
+    float4 loy, loz, hi;
+    edges<LM>(sq[cq - PK], sq[cq], sq[cq + PK], loy, hi);
+    ehy[(ty + 1) * TK + tx] = hi;
+    edges<LM>(sq[cq - 1], sq[cq], sq[cq + 1], loz, hi);
+    ehz[ty * (TK + 1) + tx + 1] = hi;
     {
-      const int cq = (ty + 2) * PK + tx + 2;
-      const int cc = (ty + 1) * CK + tx + 1;
-      const float2 cy0 = sc[cc - CK], cy1 = sc[cc];
-      fy[ty * TK + tx] = face_flux<1, SOLVER, LIM>(sq[cq - 2 * PK], sq[cq - PK], sq[cq], sq[cq + PK], cy0.x, cy1.x,
-                                                   cy0.y, cy1.y, floor_);
-      const float2 cz0 = sc[cc - 1];
-      fz[ty * (TK + 1) + tx] = face_flux<2, SOLVER, LIM>(sq[cq - 2], sq[cq - 1], sq[cq], sq[cq + 1], cz0.x, cy1.x,
-                                                         cz0.y, cy1.y, floor_);
-      if (tid < TK) {
-        // y face between rows TJ-1 and TJ of the tile, column tid
-        const int bq = (TJ + 2) * PK + tid + 2;
-        const int bc = (TJ + 1) * CK + tid + 1;
-        const float2 a0 = sc[bc - CK], a1 = sc[bc];
-        fy[TJ * TK + tid] = face_flux<1, SOLVER, LIM>(sq[bq - 2 * PK], sq[bq - PK], sq[bq], sq[bq + PK], a0.x,
-                                                      a1.x, a0.y, a1.y, floor_);
-      } else if (tid < TK + TJ) {
-        // z face between columns TK-1 and TK of row r
-        const int r = tid - TK;
+      float4 lo_, hi_;
+      if (tid < TK) {                        // row -1: high y edge
+        const int bq = PK + tid + 2;
+        edges<LM>(sq[bq - PK], sq[bq], sq[bq + PK], lo_, hi_);
+        ehy[tid] = hi_;
+      } else if (tid < 2 * TK) {             // row TJ: low y edge
+        const int t = tid - TK;
+        const int bq = (TJ + 2) * PK + t + 2;
+        edges<LM>(sq[bq - PK], sq[bq], sq[bq + PK], lo_, hi_);
+        ely[t] = lo_;
+      } else if (tid < 2 * TK + TJ) {        // column -1: high z edge
+        const int r = tid - 2 * TK;
+        const int bq = (r + 2) * PK + 1;
+        edges<LM>(sq[bq - 1], sq[bq], sq[bq + 1], lo_, hi_);
+        ehz[r * (TK + 1)] = hi_;
+      } else if (tid < 2 * TK + 2 * TJ) {    // column TK: low z edge
+        const int r = tid - 2 * TK - TJ;
         const int bq = (r + 2) * PK + TK + 2;
-        const int bc = (r + 1) * CK + TK + 1;
-        const float2 a0 = sc[bc - 1], a1 = sc[bc];
-        fz[r * (TK + 1) + TK] = face_flux<2, SOLVER, LIM>(sq[bq - 2], sq[bq - 1], sq[bq], sq[bq + 1], a0.x, a1.x,
-                                                          a0.y, a1.y, floor_);
+        edges<LM>(sq[bq - 1], sq[bq], sq[bq + 1], lo_, hi_);
+        elz[r] = lo_;
+      }
+    }
+    // x: the window gives plane i+1's edges; its low edge closes face i+1/2, its high edge is carried
+    float4 lox1, hix1;
+    edges<LM>(qw[0], qw[1], qw[2], lox1, hix1);
+    const float4 fRx = face_flux<0, SOLVER>(hix, lox1, c2w[0], c2w[1], cw[0], cw[1], floor_);
+    __syncthreads();
+
+    // ---- C: the low y and z faces of every cell, and the tile's high y and z boundary faces
+    {
+      const float2 cy0 = sc[cc - CK], c1 = sc[cc], cz0 = sc[cc - 1];
+      fy[ty * TK + tx] = face_flux<1, SOLVER>(ehy[ty * TK + tx], loy, cy0.x, c1.x, cy0.y, c1.y, floor_);
+      fz[ty * (TK + 1) + tx] = face_flux<2, SOLVER>(ehz[ty * (TK + 1) + tx], loz, cz0.x, c1.x, cz0.y, c1.y, floor_);
+      if (tid < TK) {
+        const int bc = TJ * CK + tid + 1;    // c plane: row TJ-1 at bc, row TJ at bc + CK
+        const float2 a0 = sc[bc], a1 = sc[bc + CK];
+        fy[TJ * TK + tid] = face_flux<1, SOLVER>(ehy[TJ * TK + tid], ely[tid], a0.x, a1.x, a0.y, a1.y, floor_);
+      } else if (tid < TK + TJ) {
+        const int r = tid - TK;
+        const int bc = (r + 1) * CK + TK;    // column TK-1 at bc, column TK at bc + 1
+        const float2 a0 = sc[bc], a1 = sc[bc + 1];
+        fz[r * (TK + 1) + TK] = face_flux<2, SOLVER>(ehz[r * (TK + 1) + TK], elz[r], a0.x, a1.x, a0.y, a1.y,
+                                                     floor_);
       }
     }
     __syncthreads();
 
-    // ---- E: update of the own cell
+    // ---- D: update of the own cell
     if (active) {
       const float4 fLy = fy[ty * TK + tx], fRy = fy[(ty + 1) * TK + tx];
       const float4 fLz = fz[ty * (TK + 1) + tx], fRz = fz[ty * (TK + 1) + tx + 1];
@@ -309,8 +348,9 @@ __global__ __launch_bounds__(NT, 2) void hydro_stage_kernel(HydroArgs A) {
         dU[2] = dU[2] + U[0] * (acc1 * wx);
         dU[3] = dU[3] + U[0] * (acc2 * wx);
       }
-      // bipolar jets (sim.py:474-485)
+      // bipolar jets (sim.py:474-485); where no cone contributes the term is +0 and adding it is skipped
       float aj0 = 0.0f, aj1 = 0.0f, aj2 = 0.0f;
+      bool jet_hit = false;
       for (int s = 0; s < A.nstars; ++s) {
         const float4 jt = A.jets[s];
         const float4 st = A.stars[s];
@@ -324,28 +364,40 @@ __global__ __launch_bounds__(NT, 2) void hydro_stage_kernel(HydroArgs A) {
               aj0 = aj0 + jt.x * f;
               aj1 = aj1 + jt.y * f;
               aj2 = aj2 + jt.z * f;
+              jet_hit = true;
             }
           }
         }
       }
-      dU[1] = dU[1] + U[0] * aj0;
-      dU[2] = dU[2] + U[0] * aj1;
-      dU[3] = dU[3] + U[0] * aj2;
+      if (jet_hit) {
+        dU[1] = dU[1] + U[0] * aj0;
+        dU[2] = dU[2] + U[0] * aj1;
+        dU[3] = dU[3] + U[0] * aj2;
+      }
 
-      const float U0v[4] = {A.u0_is_u ? U[0] : U0.x, A.u0_is_u ? U[1] : U0.y, A.u0_is_u ? U[2] : U0.z,
-                            A.u0_is_u ? U[3] : U0.w};
       float out[4];
+      if constexpr (STAGE1) {
 #pragma unroll
-      for (int c = 0; c < 4; ++c) out[c] = A.a * U0v[c] + A.b * (U[c] + A.dt * dU[c]);
+        for (int c = 0; c < 4; ++c) out[c] = U[c] + A.dt * dU[c];
+      } else {
+        const float U0v[4] = {A.u0_is_u ? U[0] : U0.x, A.u0_is_u ? U[1] : U0.y, A.u0_is_u ? U[2] : U0.z,
+                              A.u0_is_u ? U[3] : U0.w};
+#pragma unroll
+        for (int c = 0; c < 4; ++c) out[c] = A.a * U0v[c] + A.b * (U[c] + A.dt * dU[c]);
+      }
       const float r = fmaxf(out[0], floor_);
       const double rr = recip_d(r);
       float v0 = div_r(out[1], rr), v1 = div_r(out[2], rr), v2 = div_r(out[3], rr);
-      const float sp = sqrtf((v0 * v0 + v1 * v1) + v2 * v2);
-      if (sp > A.vmax) {
-        const float f = A.vmax / sp;
-        v0 = v0 * f;
-        v1 = v1 * f;
-        v2 = v2 * f;
+      const float s2 = (v0 * v0 + v1 * v1) + v2 * v2;
+      if (s2 > A.vmax2_lo) {
+        // |v| = sqrt(s2) may exceed vmax: the reference test and cap (sim.py:491-494)
+        const float sp = sqrtf(s2);
+        if (sp > A.vmax) {
+          const float f = A.vmax / sp;
+          v0 = v0 * f;
+          v1 = v1 * f;
+          v2 = v2 * f;
+        }
       }
       const float m0 = r * v0, m1 = r * v1, m2 = r * v2;
       A.o_r[o] = r;
@@ -353,19 +405,19 @@ __global__ __launch_bounds__(NT, 2) void hydro_stage_kernel(HydroArgs A) {
       A.o_my[o] = m1;
       A.o_mz[o] = m2;
       if constexpr (SMAX) {
-        // max_speed of the written state (nebula3d.py:154-162): raw stored rho, L1 norm of v
-        const float s = ((fabsf(div_r(m0, rr)) + fabsf(div_r(m1, rr))) + fabsf(div_r(m2, rr))) +
-                        sqrtf(cn2 + x_ion * dci);
+        // max_speed of the written state (nebula3d.py:154-162): raw stored rho, L1 norm of v; the sound speed
+        // sqrt(cn2 + xi (ci2 - cn2)) of this cell is cw[0]
+        const float s = ((fabsf(div_r(m0, rr)) + fabsf(div_r(m1, rr))) + fabsf(div_r(m2, rr))) + cw[0];
         tmax = fmaxf(tmax, s);
       }
     }
     fLx = fRx;
+    hix = hix1;
 
     // ---- shift the window by one plane
     qw[0] = qw[1];
     qw[1] = qw[2];
-    qw[2] = qw[3];
-    qw[3] = prim(Unext.x, Unext.y, Unext.z, Unext.w, floor_);
+    qw[2] = prim(Unext.x, Unext.y, Unext.z, Unext.w, floor_);
     c2w[0] = c2w[1];
     cw[0] = cw[1];
     c2w[1] = cn2 + xraw[2] * dci;   // plane i+2 becomes plane (i+1)+1
@@ -397,12 +449,12 @@ __global__ __launch_bounds__(256) void max_speed_kernel(const float* rho, const 
 
 using HydroKernel = void (*)(HydroArgs);
 
-template <bool SMAX>
+template <bool SMAX, bool STAGE1>
 static HydroKernel pick_hydro(int solver, int lim) {
-  if (solver == NV_SOLVER_RUSANOV) return hydro_stage_kernel<NV_SOLVER_RUSANOV, NV_LIM_MINMOD, SMAX>;
-  if (lim == NV_LIM_VANLEER) return hydro_stage_kernel<NV_SOLVER_HLL, NV_LIM_VANLEER, SMAX>;
-  if (lim == NV_LIM_MC) return hydro_stage_kernel<NV_SOLVER_HLL, NV_LIM_MC, SMAX>;
-  return hydro_stage_kernel<NV_SOLVER_HLL, NV_LIM_MINMOD, SMAX>;
+  if (solver == NV_SOLVER_RUSANOV) return hydro_stage_kernel<NV_SOLVER_RUSANOV, NV_LIM_MINMOD, SMAX, STAGE1>;
+  if (lim == NV_LIM_VANLEER) return hydro_stage_kernel<NV_SOLVER_HLL, NV_LIM_VANLEER, SMAX, STAGE1>;
+  if (lim == NV_LIM_MC) return hydro_stage_kernel<NV_SOLVER_HLL, NV_LIM_MC, SMAX, STAGE1>;
+  return hydro_stage_kernel<NV_SOLVER_HLL, NV_LIM_MINMOD, SMAX, STAGE1>;
 }
 
 }  // namespace nv
@@ -435,13 +487,25 @@ extern "C" int nv_hydro_stage(const float* rho, const float* mx, const float* my
   A.u0_is_u = (r0 == rho && mx0 == mx && my0 == my && mz0 == mz) ? 1 : 0;
   A.wind = wind; A.jet_cos = jet_cos; A.jet_r = jet_r; A.pulse = pulse; A.dt = dt; A.dx = dx;
   A.a = a; A.b = b; A.floor_ = floor_; A.cn2 = cn2; A.ci2 = ci2; A.vmax = vmax;
+  {
+    const double v2 = (double)vmax * (double)vmax;
+    float lo = (float)v2;
+    if ((double)lo > v2) lo = nextafterf(lo, 0.0f);
+    A.vmax2_lo = lo;
+  }
   // march depth: long enough to amortise the 4-plane prologue, short enough to fill the chip
   const long long tiles_jk = (long long)((nz + TK - 1) / TK) * ((ny + TJ - 1) / TJ);
   int ci = (int)std::min<long long>(64, std::max<long long>(4, (long long)nx * tiles_jk / 2048));
   A.ci = ci;
-  dim3 grid((nz + TK - 1) / TK, (ny + TJ - 1) / TJ, (nx + ci - 1) / ci);
+  A.tiles_k = (nz + TK - 1) / TK;
+  A.tiles_j = (ny + TJ - 1) / TJ;
+  A.tiles = A.tiles_k * A.tiles_j * ((nx + ci - 1) / ci);
+  A.per_xcd = (A.tiles + 7) / 8;
+  dim3 grid(8 * A.per_xcd);
   dim3 block(TK, TJ, 1);
-  HydroKernel kern = smax_out ? pick_hydro<true>(solver, lim) : pick_hydro<false>(solver, lim);
+  const bool stage1 = A.u0_is_u && a == 0.0f && b == 1.0f;
+  HydroKernel kern = smax_out ? (stage1 ? pick_hydro<true, true>(solver, lim) : pick_hydro<true, false>(solver, lim))
+                              : (stage1 ? pick_hydro<false, true>(solver, lim) : pick_hydro<false, false>(solver, lim));
   hipLaunchKernelGGL(kern, grid, block, 0, as_stream(stream), A);
   return check_launch("nv_hydro_stage");
 }
