@@ -27,8 +27,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-BYTES_PER_CELL_STEP = 88.0      # 5 S_U + 2 S_xi with S_U = 16, S_xi = 4
-BYTES_PER_CELL_LAUNCH = BYTES_PER_CELL_STEP / 2
+# algorithmic bytes per cell per RK2 step: 5 S_U + 2 S_xi (S_xi = 4): S_U = 16 for the float32 state,
+# 10 for float32 density with binary16 velocity
+BYTES_PER_CELL_STEP = {"fp32": 88.0, "fp32-fast": 88.0, "fp16-vel": 58.0}
 
 
 def log(*a):
@@ -88,6 +89,9 @@ def main():
     ap.add_argument("--config", type=int, default=2)
     ap.add_argument("--n", type=int, default=None, help="override the grid size (testing only)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp32-fast", "fp16-vel"],
+                    help="the form whose throughput is `value` (fp32 = the bit-exact drop-in)")
+    ap.add_argument("--no-variants", action="store_true", help="do not also time the other two forms")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="counter calibration: run a plain device-to-device copy of this many bytes 3 times, exit")
     a = ap.parse_args()
@@ -109,35 +113,48 @@ def main():
     t0 = time.perf_counter()
     setup = S.make_setup(scenario, n, seed)
     log(f"[bench] {scenario} {n}^3 initial conditions in {time.perf_counter() - t0:.1f} s")
-    sim = Sim(setup=setup, verbose=False)
-    cells = math.prod(sim.shape)
-    log(f"[bench] warmup {a.warmup} steps")
-    sim.step(a.warmup)
-    torch.cuda.synchronize()
+    def measure(precision):
+        sim = Sim(setup=setup, verbose=False, precision=precision)
+        log(f"[bench] {precision}: warmup {a.warmup} steps, timing {a.steps} steps")
+        sim.step(a.warmup)
+        torch.cuda.synchronize()
+        timer = KernelTimer()
+        sim.timer = timer
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sim.step(a.steps)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        sim.timer = None
+        ks = timer.summary()
+        cells = math.prod(sim.shape)
+        hyd_s = ks["hydro_stage"]["avg_ms"] / 1e3
+        achieved = BYTES_PER_CELL_STEP[precision] / 2 * cells / hyd_s / 1e9
+        r = dict(value=cells * a.steps / wall, ms_per_step=wall / a.steps * 1e3, hydro_launch_ms=hyd_s * 1e3,
+                 hydro_only_cell_updates_per_s=cells / (2 * hyd_s), hydro_achieved_GBps=achieved,
+                 kernels_ms={k: round(v["avg_ms"], 4) for k, v in ks.items()}, ion_every=sim.ion_every,
+                 rays=tuple(sim.rfronts.shape[1:]), nstars=len(sim.stars))
+        log(f"[bench] {precision}: {r['value'] / 1e9:.2f} G cell-updates/s, {r['ms_per_step']:.3f} ms/step, "
+            f"hydro {hyd_s * 1e3:.3f} ms/launch = {achieved:.0f} GB/s; kernels {r['kernels_ms']}")
+        del sim
+        torch.cuda.empty_cache()
+        return r
 
-    timer = KernelTimer()
-    sim.timer = timer
-    log(f"[bench] timing {a.steps} steps")
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    sim.step(a.steps)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    sim.timer = None
-    ks = timer.summary()
-    hyd = ks["hydro_stage"]
-    hyd_launch_s = hyd["avg_ms"] / 1e3
-    achieved = BYTES_PER_CELL_LAUNCH * cells / hyd_launch_s / 1e9
-    traffic = measured_traffic()
-    value = cells * a.steps / wall
-    hydro_rate = cells / (2 * hyd_launch_s)
-    log(f"[bench] {value / 1e9:.2f} G cell-updates/s full step, hydro {hydro_rate / 1e9:.2f} G/s, "
-        f"hydro launch {hyd['avg_ms']:.3f} ms = {achieved:.0f} GB/s; kernels {json.dumps(ks)}")
+    main_r = measure(a.precision)
+    variants = {}
+    if not a.no_variants:
+        for p in ("fp32", "fp32-fast", "fp16-vel"):
+            if p != a.precision:
+                variants[p] = measure(p)
+    cells = math.prod(setup.shape)
+    value = main_r["value"]
+    achieved = main_r["hydro_achieved_GBps"]
+    traffic = measured_traffic() if a.precision == "fp32" else None
 
     cpu = None
     if not a.no_cpu:
         log("[bench] CPU baseline (C++ oracle)")
-        cpu = cpu_baseline(setup, sim.ion_every)
+        cpu = cpu_baseline(setup, main_r["ion_every"])
         log(f"[bench] CPU baseline {cpu['value'] / 1e6:.2f} M cell-updates/s")
 
     out = {
@@ -147,20 +164,23 @@ def main():
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": wall / a.steps * 1e3,
+        "ms_per_step": main_r["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
+        "precision": a.precision,
         "data": "synthetic (seeded initial conditions generated on the host)",
-        "config": {"workload": f"{scenario} {n}^3 FP32, HLL+minmod RK2, CFL 0.3, ionization every "
-                               f"{sim.ion_every} steps with {sim.rfronts.shape[1]}x{sim.rfronts.shape[2]} rays "
-                               f"(dr = dx/2), seed {seed}", "cells": cells, "stars": len(sim.stars)},
+        "config": {"workload": f"{scenario} {n}^3, HLL+minmod RK2, CFL 0.3, ionization every "
+                               f"{main_r['ion_every']} steps with {main_r['rays'][0]}x{main_r['rays'][1]} rays "
+                               f"(dr = dx/2), seed {seed}", "cells": cells, "stars": main_r["nstars"]},
         "roofline": {"bound": "hbm", "kernel": "hydro_stage_kernel", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": traffic, "algorithmic_bytes_per_launch": BYTES_PER_CELL_LAUNCH * cells},
-        "hydro_only_cell_updates_per_s": hydro_rate,
-        "kernels_ms": {k: round(v["avg_ms"], 4) for k, v in ks.items()},
+                     "traffic": traffic,
+                     "algorithmic_bytes_per_launch": BYTES_PER_CELL_STEP[a.precision] / 2 * cells},
+        "hydro_only_cell_updates_per_s": main_r["hydro_only_cell_updates_per_s"],
+        "kernels_ms": main_r["kernels_ms"],
+        "variants": variants,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

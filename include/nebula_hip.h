@@ -50,10 +50,34 @@ int nv_hydro_stage(const float* rho, const float* mx, const float* my, const flo
                    float dx, float a, float b, float floor_, float cn2, float ci2, float vmax, int solver,
                    int lim, float* smax_out, void* stream);
 
+/* The same stage with fused multiply-adds, hardware reciprocals (one Newton step) and hardware square roots:
+ * each operation within about one ulp instead of correctly rounded. Same arguments as nv_hydro_stage. */
+int nv_hydro_stage_fast(const float* rho, const float* mx, const float* my, const float* mz, const float* xi,
+                        const float* r0, const float* mx0, const float* my0, const float* mz0, float* o_r,
+                        float* o_mx, float* o_my, float* o_mz, int nx, int ny, int nz, const float* stars,
+                        int nstars, float wind, const float* jets, float jet_cos, float jet_r, float pulse, float dt,
+                        float dx, float a, float b, float floor_, float cn2, float ci2, float vmax, int solver,
+                        int lim, float* smax_out, void* stream);
+
+/* The stage on the reduced-storage state: float32 rho with binary16 velocity (vx, vy, vz) for U, U0 and the
+ * output (velocity, not momentum; arithmetic in float32 with the fast operations above). Replaces the same
+ * reference launch as nv_hydro_stage with a 58 B/cell-step instead of an 88 B/cell-step state. smax_out
+ * reduces |v| + c of the stored (rounded) state. */
+int nv_hydro_stage_h(const float* rho, const uint16_t* vx, const uint16_t* vy, const uint16_t* vz,
+                     const float* xi, const float* r0, const uint16_t* vx0, const uint16_t* vy0,
+                     const uint16_t* vz0, float* o_r, uint16_t* o_vx, uint16_t* o_vy, uint16_t* o_vz, int nx,
+                     int ny, int nz, const float* stars, int nstars, float wind, const float* jets, float jet_cos,
+                     float jet_r, float pulse, float dt, float dx, float a, float b, float floor_, float cn2,
+                     float ci2, float vmax, int solver, int lim, float* smax_out, void* stream);
+
 /* CFL signal speed: out[0] = max(out[0], max over cells of |u|+|v|+|w| + sqrt(cn2 + xi (ci2 - cn2))), raw rho.
  * Replaces nebula3d.max_speed (nebula3d.py:154-162) as launched by Sim._compute_dt (sim.py:1009-1012). */
 int nv_max_speed(const float* rho, const float* mx, const float* my, const float* mz, const float* xi, int nx,
                  int ny, int nz, float cn2, float ci2, float* out, void* stream);
+
+/* nv_max_speed for the binary16-velocity state: max of |vx|+|vy|+|vz| + c (max_speed, nebula3d.py:154-162). */
+int nv_max_speed_h(const uint16_t* vx, const uint16_t* vy, const uint16_t* vz, const float* xi, int nx, int ny,
+                   int nz, float cn2, float ci2, float* out, void* stream);
 
 /* Stromgren front radius per direction: rfront[nu][nphi]. Replaces nebula3d.cast_rays (nebula3d.py:247-270),
  * launched per star by Sim.ionize (sim.py:936-937). */
@@ -85,6 +109,12 @@ int nv_ionize_apply(const float* rfronts, int nu, int nphi, const float* star_xy
                     float width, float* xi, uint16_t* dfront_h, float* dfront_f, int nx, int ny, int nz,
                     const float* rho, const float* mx, const float* my, const float* mz, float cn2, float ci2,
                     float* smax_out, void* stream);
+
+/* nv_ionize_apply with the CFL maximum taken over the binary16-velocity state (rho is not read). */
+int nv_ionize_apply_h(const float* rfronts, int nu, int nphi, const float* star_xyz, int nstars, float dx,
+                      float width, float* xi, uint16_t* dfront_h, float* dfront_f, int nx, int ny, int nz,
+                      const float* rho, const uint16_t* vx, const uint16_t* vy, const uint16_t* vz, float cn2,
+                      float ci2, float* smax_out, void* stream);
 
 /* FUV dust attenuation att[nr][nu][nphi] along rays from one star. Replaces nebula3d.cast_fuv
  * (nebula3d.py:320-352), launched per star by Sim.fuv_field (sim.py:964-965). */

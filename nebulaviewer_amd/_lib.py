@@ -19,6 +19,10 @@ SIGNATURES = {
     "nv_last_error": [],
     "nv_abi_version": [],
     "nv_hydro_stage": [P] * 13 + [I, I, I, P, I, F, P, F, F, F, F, F, F, F, F, F, F, F, I, I, P, P],
+    "nv_hydro_stage_fast": [P] * 13 + [I, I, I, P, I, F, P, F, F, F, F, F, F, F, F, F, F, F, I, I, P, P],
+    "nv_hydro_stage_h": [P] * 13 + [I, I, I, P, I, F, P, F, F, F, F, F, F, F, F, F, F, F, I, I, P, P],
+    "nv_max_speed_h": [P] * 4 + [I, I, I, F, F, P, P],
+    "nv_ionize_apply_h": [P, I, I, P, I, F, F, P, P, P, I, I, I, P, P, P, P, F, F, P, P],
     "nv_max_speed": [P] * 5 + [I, I, I, F, F, P, P],
     "nv_cast_rays": [P, I, I, I, F, F, F, F, F, I, F, F, F, F, F, P, I, I, P],
     "nv_apply_front": [P, I, I, F, F, F, F, F, P, P, I, I, I, P],

@@ -4,24 +4,34 @@
 // (sim.py:397-420), face_flux (nebula3d.py:54-72), limiters (nebula3d.py:40-51, sim.py:366-394),
 // max_speed (nebula3d.py:154-162).
 //
-// Layout: structure-of-arrays fp32 fields, C-order [nx][ny][nz] (z fastest), as the reference stores them.
+// Layout: structure-of-arrays fields, C-order [nx][ny][nz] (z fastest), as the reference stores them.
 //
 // Work decomposition. A workgroup owns a TK x TJ column of cells in (z, y) - one wavefront per y row, the 64
 // lanes along the contiguous z axis - and marches along x over `ci` planes. Per plane:
-//   * every thread keeps its own cell column's primitives (rho floored, u, v, w) and c^2, c for planes
-//     i-1 .. i+2 in registers (a sliding window: the x stencil needs no neighbours and no halo), so the three
-//     divisions of nebula3d.prim and the square root of c^2 happen once per cell;
+//   * every thread keeps its own cell column's primitives, c^2, c and conserved state for the planes it needs
+//     in registers (a sliding window: the x stencil needs no neighbours and no halo), so the divisions of
+//     nebula3d.prim and the square root of c^2 happen once per cell;
 //   * the plane's primitives go to LDS with a 2-cell y/z halo (the halo cells are the only ones converted
 //     twice: (TJ+4)(TK+4) - TJ TK of them per TJ TK cells);
-//   * each face flux is evaluated once: the x face i+1/2 in registers (it is the left face of plane i+1), the
-//     low y and z faces of every cell from LDS into LDS face buffers, plus the TK + TJ faces on the tile's
-//     high y and z boundaries; a cell then differences its two faces per axis.
+//   * every cell computes its MUSCL edge states once per axis (the limited slope is shared by its two faces);
+//   * each face flux is evaluated once: x in registers (the right face of plane i is the left face of plane
+//     i+1), the low y and z faces of every cell into LDS, plus the TK + TJ faces on the tile's high boundary.
 // The reference evaluates six face fluxes per cell (each face twice) from 63 clamped loads; the arithmetic per
-// face is identical, so the results are the same bits.
+// face is the same.
 //
-// Exactness: no FMA contraction (-ffp-contract=off), IEEE square root, and every division either IEEE or the
-// shared-reciprocal form of nv_divide.h, which returns the identical correctly rounded quotient. Results equal
-// the CPU oracle bit for bit (up to the sign of zero, which never feeds a nonzero result here).
+// Two compile-time policies:
+//   Arithmetic  Exact: no FMA contraction (-ffp-contract=off), IEEE square root, and every division IEEE or
+//               through the shared double reciprocal of nv_divide.h (the identical correctly rounded
+//               quotient): results equal the CPU oracle bit for bit (up to the sign of zero, which never feeds
+//               a nonzero result here).
+//               Fast: the same expressions with explicit fused multiply-adds, the hardware reciprocal refined
+//               by one Newton step, and the hardware square root (each within about one ulp).
+//   Storage     F32: (rho, mx, my, mz) float32, the reference layout.
+//               F16V: rho float32, (vx, vy, vz) binary16. |v| <= vmax lies in the normal fp16 range, so each
+//               velocity keeps a relative precision of 2^-11 with no scale factor; the primitive state is read
+//               directly (no division) and the conserved state is rho * v. Density stays float32: rounding it
+//               to binary16 every stage stalls updates below half an fp16 ulp (measured: rel-L1 density error
+//               4e-3 after 100 steps of the 64^3 torus against 1.4e-4 with float32 density).
 #include "nv_common.h"
 #include "nv_divide.h"
 
@@ -36,16 +46,101 @@ constexpr int NHALO = 4 * PK + 4 * TJ;      // halo cells of the primitive plane
 constexpr int NCHALO = 2 * CK + 2 * TJ;     // halo cells of the c plane
 
 struct HydroArgs {
-  const float *rho, *mx, *my, *mz, *xi, *r0, *mx0, *my0, *mz0;
-  float *o_r, *o_mx, *o_my, *o_mz;
+  const void *f0, *f1, *f2, *f3;            // U: rho and momentum (F32) or rho and velocity (F16V)
+  const float* xi;
+  const void *g0, *g1, *g2, *g3;            // U0, same storage
+  void *o0, *o1, *o2, *o3;                  // output, same storage
   const float4 *stars, *jets;
   float* smax;
   int nx, ny, nz, nstars, ci, u0_is_u;
-  int tiles_k, tiles_j, tiles, per_xcd;   // tile grid and the number of tiles each XCD takes
+  int tiles_k, tiles_j, tiles, per_xcd;     // tile grid and the number of tiles each XCD takes
   float wind, jet_cos, jet_r, pulse, dt, dx, a, b, floor_, cn2, ci2, vmax;
-  float vmax2_lo;                         // largest float <= vmax^2: |v|^2 <= vmax2_lo means no cap
+  float vmax2_lo;                           // largest float <= vmax^2: |v|^2 <= vmax2_lo means no cap
 };
 
+// ------------------------------------------------------------------------------------------ arithmetic
+struct Exact {
+  using Rcp = double;
+  static __device__ __forceinline__ Rcp rcp(float b) { return recip_d(b); }
+  static __device__ __forceinline__ float div(float a, Rcp r) { return div_r(a, r); }
+  static __device__ __forceinline__ float mad(float a, float b, float c) { return a * b + c; }
+  static __device__ __forceinline__ float sqrt(float x) { return sqrtf(x); }
+};
+
+struct Fast {
+  using Rcp = float;
+  static __device__ __forceinline__ Rcp rcp(float b) {
+    const float r = __builtin_amdgcn_rcpf(b);
+    return fmaf(r, fmaf(-b, r, 1.0f), r);
+  }
+  static __device__ __forceinline__ float div(float a, Rcp r) { return a * r; }
+  static __device__ __forceinline__ float mad(float a, float b, float c) { return fmaf(a, b, c); }
+  static __device__ __forceinline__ float sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+};
+
+// ------------------------------------------------------------------------------------------ storage
+__device__ __forceinline__ float h2f(uint16_t h) { return (float)__builtin_bit_cast(_Float16, h); }
+
+struct F32 {
+  static constexpr bool VEL = false;   // stores momentum
+  static __device__ __forceinline__ float4 load(const void* f0, const void* f1, const void* f2, const void* f3,
+                                                long long o) {
+    return make_float4(((const float*)f0)[o], ((const float*)f1)[o], ((const float*)f2)[o], ((const float*)f3)[o]);
+  }
+  // stores r and r*v; returns the velocity the CFL reduction sees (stored m / stored rho)
+  template <class AR>
+  static __device__ __forceinline__ float3 store(void* o0, void* o1, void* o2, void* o3, long long o, float r,
+                                                 typename AR::Rcp rr, float v0, float v1, float v2, bool want_v) {
+    const float m0 = r * v0, m1 = r * v1, m2 = r * v2;
+    ((float*)o0)[o] = r;
+    ((float*)o1)[o] = m0;
+    ((float*)o2)[o] = m1;
+    ((float*)o3)[o] = m2;
+    if (!want_v) return make_float3(0.0f, 0.0f, 0.0f);
+    return make_float3(AR::div(m0, rr), AR::div(m1, rr), AR::div(m2, rr));
+  }
+};
+
+struct F16V {
+  static constexpr bool VEL = true;    // stores velocity
+  static __device__ __forceinline__ float4 load(const void* f0, const void* f1, const void* f2, const void* f3,
+                                                long long o) {
+    return make_float4(((const float*)f0)[o], h2f(((const uint16_t*)f1)[o]), h2f(((const uint16_t*)f2)[o]),
+                       h2f(((const uint16_t*)f3)[o]));
+  }
+  template <class AR>
+  static __device__ __forceinline__ float3 store(void* o0, void* o1, void* o2, void* o3, long long o, float r,
+                                                 typename AR::Rcp, float v0, float v1, float v2, bool) {
+    const uint16_t h0 = f32_to_f16_bits(v0), h1 = f32_to_f16_bits(v1), h2 = f32_to_f16_bits(v2);
+    ((float*)o0)[o] = r;
+    ((uint16_t*)o1)[o] = h0;
+    ((uint16_t*)o2)[o] = h1;
+    ((uint16_t*)o3)[o] = h2;
+    return make_float3(h2f(h0), h2f(h1), h2f(h2));   // the velocity the next step will read
+  }
+};
+
+// primitive state (nebula3d.prim, nebula3d.py:28-32) and conserved state of a stored cell
+template <class ST, class AR>
+__device__ __forceinline__ void decode(const float4& raw, float floor_, float4& q, float4& U) {
+  if constexpr (ST::VEL) {
+    q = make_float4(fmaxf(raw.x, floor_), raw.y, raw.z, raw.w);
+    U = make_float4(raw.x, raw.x * raw.y, raw.x * raw.z, raw.x * raw.w);
+  } else {
+    const float r = fmaxf(raw.x, floor_);
+    const typename AR::Rcp rr = AR::rcp(r);
+    q = make_float4(r, AR::div(raw.y, rr), AR::div(raw.z, rr), AR::div(raw.w, rr));
+    U = raw;
+  }
+}
+
+template <class ST>
+__device__ __forceinline__ float4 conserved(const float4& raw) {
+  if constexpr (ST::VEL) return make_float4(raw.x, raw.x * raw.y, raw.x * raw.z, raw.x * raw.w);
+  else return raw;
+}
+
+// ------------------------------------------------------------------------------------------ physics
 // slope limiters: minmod (nebula3d.py:40-46), van Leer (sim.py:375-380), MC (sim.py:366-372)
 template <int LIM>
 __device__ __forceinline__ float slope(float a, float b) {
@@ -72,17 +167,16 @@ __device__ __forceinline__ float comp(const float4& q, int c) {
 // Edge states of a cell from its own and its two neighbours' primitive states along one axis: the limited
 // slope s = lim(q - q-, q+ - q) is the same for the cell's low and high faces, so lo = q - s/2 is the right
 // state R of the face below (nebula3d.py:61, sim.py:402) and hi = q + s/2 the left state L of the face above
-// (nebula3d.py:60, sim.py:401). Computing both once per cell gives the reference's bits with half the limiter
-// evaluations.
-template <int LM>
+// (nebula3d.py:60, sim.py:401). In the exact policy mad(-1/2, s, v) is (-s/2) + v, i.e. v - s/2 bit for bit.
+template <class AR, int LM>
 __device__ __forceinline__ void edges(const float4& qm, const float4& q, const float4& qp, float4& lo, float4& hi) {
   float l[4], h[4];
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
     const float v = comp(q, c);
     const float sl = slope<LM>(v - comp(qm, c), comp(qp, c) - v);
-    l[c] = v - 0.5f * sl;
-    h[c] = v + 0.5f * sl;
+    l[c] = AR::mad(-0.5f, sl, v);
+    h[c] = AR::mad(0.5f, sl, v);
   }
   lo = make_float4(l[0], l[1], l[2], l[3]);
   hi = make_float4(h[0], h[1], h[2], h[3]);
@@ -91,8 +185,9 @@ __device__ __forceinline__ void edges(const float4& qm, const float4& q, const f
 // Face flux along axis AX (0=x, 1=y, 2=z) from the reconstructed states L (high edge of the cell below) and R
 // (low edge of the cell above), with c^2 and c of those two cells. HLL (sim.py:397-420) or Rusanov
 // (nebula3d.py:54-72). The reference's one-hot axis vector e is folded in at compile time: dot(v, e) = v[AX+1],
-// e*(r c^2) touches component AX+1 only.
-template <int AX, int SOLVER>
+// e*(r c^2) touches component AX+1 only. Sums are written as mad(x, y, z) = x*y + z, which in the exact
+// policy is the reference expression with its two addends swapped (the same IEEE result).
+template <class AR, int AX, int SOLVER>
 __device__ __forceinline__ float4 face_flux(const float4& Lq, const float4& Rq, float c2b, float c2c, float cb,
                                             float cc, float floor_) {
   const float L[4] = {Lq.x, Lq.y, Lq.z, Lq.w};
@@ -107,14 +202,14 @@ __device__ __forceinline__ float4 face_flux(const float4& Lq, const float4& Rq, 
     FL[c] = UL[c] * unL;
     FR[c] = UR[c] * unR;
   }
-  FL[AX + 1] = FL[AX + 1] + rL * c2b;
-  FR[AX + 1] = FR[AX + 1] + rR * c2c;
+  FL[AX + 1] = AR::mad(rL, c2b, FL[AX + 1]);
+  FR[AX + 1] = AR::mad(rR, c2c, FR[AX + 1]);
   float F[4];
   if constexpr (SOLVER == NV_SOLVER_RUSANOV) {
     const float amax = fmaxf(fabsf(unL) + cb, fabsf(unR) + cc);
     const float ha = 0.5f * amax;
 #pragma unroll
-    for (int c = 0; c < 4; ++c) F[c] = 0.5f * (FL[c] + FR[c]) - ha * (UR[c] - UL[c]);
+    for (int c = 0; c < 4; ++c) F[c] = AR::mad(-ha, UR[c] - UL[c], 0.5f * (FL[c] + FR[c]));
   } else {
     const float SL = fminf(unL - cb, unR - cc);
     const float SR = fmaxf(unL + cb, unR + cc);
@@ -126,25 +221,20 @@ __device__ __forceinline__ float4 face_flux(const float4& Lq, const float4& Rq, 
       for (int c = 0; c < 4; ++c) F[c] = FR[c];
     } else {
       const float ss = SL * SR;
-      const double rden = recip_d(SR - SL);
+      const typename AR::Rcp rden = AR::rcp(SR - SL);
 #pragma unroll
-      for (int c = 0; c < 4; ++c) F[c] = div_r((SR * FL[c] - SL * FR[c]) + ss * (UR[c] - UL[c]), rden);
+      for (int c = 0; c < 4; ++c)
+        F[c] = AR::div(AR::mad(ss, UR[c] - UL[c], AR::mad(SR, FL[c], -(SL * FR[c]))), rden);
     }
   }
   return make_float4(F[0], F[1], F[2], F[3]);
 }
 
-// primitive state of a cell (nebula3d.prim, nebula3d.py:28-32)
-__device__ __forceinline__ float4 prim(float rho, float mx, float my, float mz, float floor_) {
-  const float r = fmaxf(rho, floor_);
-  const double rr = recip_d(r);
-  return make_float4(r, div_r(mx, rr), div_r(my, rr), div_r(mz, rr));
-}
-
 // STAGE1: a = 0, b = 1 and U0 = U (sim.py:1025), so out = 0*U0 + 1*Un is Un (up to the sign of zero).
-template <int SOLVER, int LIM, bool SMAX, bool STAGE1>
+template <class ST, class AR, int SOLVER, int LIM, bool SMAX, bool STAGE1>
 __global__ __launch_bounds__(NT, 4) void hydro_stage_kernel(HydroArgs A) {
   constexpr int LM = SOLVER == NV_SOLVER_HLL ? LIM : NV_LIM_MINMOD;   // Rusanov always uses minmod
+  using Rcp = typename AR::Rcp;
   __shared__ float4 sq[PJ * PK];            // primitives of plane i, 2-cell halo
   __shared__ float2 sc[CJ * CK];            // (c^2, c) of plane i, 1-cell halo
   __shared__ float4 ehy[(TJ + 1) * TK];     // high y edge of rows -1 .. TJ-1 (row r+1 holds row r)
@@ -171,9 +261,9 @@ __global__ __launch_bounds__(NT, 4) void hydro_stage_kernel(HydroArgs A) {
   const long long col = (long long)min(j, ny - 1) * nz + min(k, nz - 1);
   const float floor_ = A.floor_, cn2 = A.cn2, dci = A.ci2 - A.cn2;
 
-  auto load_col = [&](int p, float4& U, float& x) {
+  auto load_col = [&](int p, float4& raw, float& x) {
     const long long o = (long long)clampi(p, 0, nx - 1) * plane + col;
-    U = make_float4(A.rho[o], A.mx[o], A.my[o], A.mz[o]);
+    raw = ST::load(A.f0, A.f1, A.f2, A.f3, o);
     x = A.xi[o];
   };
 
@@ -182,42 +272,42 @@ __global__ __launch_bounds__(NT, 4) void hydro_stage_kernel(HydroArgs A) {
   float4 qw[3], Uraw[3], hix, fLx;
   float xraw[3], c2w[2], cw[2];
   {
-    float4 U, q[4], lo, hm1;
+    float4 raw, q[4], Uc, lo, hm1;
     float x, c2m1, cm1;
     for (int s = 0; s < 4; ++s) {      // planes i0-2 .. i0+1
-      load_col(i0 - 2 + s, U, x);
-      q[s] = prim(U.x, U.y, U.z, U.w, floor_);
+      load_col(i0 - 2 + s, raw, x);
+      decode<ST, AR>(raw, floor_, q[s], Uc);
       if (s == 1) {
-        c2m1 = cn2 + x * dci;
-        cm1 = sqrtf(c2m1);
+        c2m1 = AR::mad(x, dci, cn2);
+        cm1 = AR::sqrt(c2m1);
       } else if (s >= 2) {
-        Uraw[s - 2] = U;
+        Uraw[s - 2] = Uc;
         xraw[s - 2] = x;
-        c2w[s - 2] = cn2 + x * dci;
-        cw[s - 2] = sqrtf(c2w[s - 2]);
+        c2w[s - 2] = AR::mad(x, dci, cn2);
+        cw[s - 2] = AR::sqrt(c2w[s - 2]);
       }
     }
-    load_col(i0 + 2, Uraw[2], xraw[2]);
+    load_col(i0 + 2, raw, xraw[2]);
     qw[0] = q[2];
     qw[1] = q[3];
-    qw[2] = prim(Uraw[2].x, Uraw[2].y, Uraw[2].z, Uraw[2].w, floor_);
-    edges<LM>(q[0], q[1], q[2], lo, hm1);   // plane i0-1
-    edges<LM>(q[1], q[2], q[3], lo, hix);   // plane i0
-    fLx = face_flux<0, SOLVER>(hm1, lo, c2m1, c2w[0], cm1, cw[0], floor_);
+    decode<ST, AR>(raw, floor_, qw[2], Uraw[2]);
+    edges<AR, LM>(q[0], q[1], q[2], lo, hm1);   // plane i0-1
+    edges<AR, LM>(q[1], q[2], q[3], lo, hix);   // plane i0
+    fLx = face_flux<AR, 0, SOLVER>(hm1, lo, c2m1, c2w[0], cm1, cw[0], floor_);
   }
 
-  const double rdx = recip_d(A.dx);
+  const Rcp rdx = AR::rcp(A.dx);
   const float dx = A.dx;
   float tmax = 0.0f;
 
   for (int i = i0; i < i1; ++i) {
     // next window plane (i+3): issued now, consumed at the end of this iteration
-    float4 Unext;
+    float4 rnext;
     float xnext;
-    load_col(i + 3, Unext, xnext);
-    float4 U0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    load_col(i + 3, rnext, xnext);
+    float4 U0raw = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     const long long o = (long long)i * plane + col;
-    if (!STAGE1 && !A.u0_is_u && active) U0 = make_float4(A.r0[o], A.mx0[o], A.my0[o], A.mz0[o]);
+    if (!STAGE1 && !A.u0_is_u && active) U0raw = ST::load(A.g0, A.g1, A.g2, A.g3, o);
 
     // ---- A: plane i into LDS (own cell from the window, halo from global)
     const int cq = (ty + 2) * PK + tx + 2;
@@ -241,7 +331,9 @@ __global__ __launch_bounds__(NT, 4) void hydro_stage_kernel(HydroArgs A) {
         kk = c < 2 ? c : TK + c;
       }
       const long long oh = pbase + (long long)clampi(j0 + jj - 2, 0, ny - 1) * nz + clampi(k0 + kk - 2, 0, nz - 1);
-      sq[jj * PK + kk] = prim(A.rho[oh], A.mx[oh], A.my[oh], A.mz[oh], floor_);
+      float4 qh, Uh;
+      decode<ST, AR>(ST::load(A.f0, A.f1, A.f2, A.f3, oh), floor_, qh, Uh);
+      sq[jj * PK + kk] = qh;
     }
     for (int h = tid; h < NCHALO; h += NT) {
       int jj, kk;
@@ -254,61 +346,62 @@ __global__ __launch_bounds__(NT, 4) void hydro_stage_kernel(HydroArgs A) {
         kk = (hh & 1) ? TK + 1 : 0;
       }
       const long long oh = pbase + (long long)clampi(j0 + jj - 1, 0, ny - 1) * nz + clampi(k0 + kk - 1, 0, nz - 1);
-      const float c2 = cn2 + A.xi[oh] * dci;
-      sc[jj * CK + kk] = make_float2(c2, sqrtf(c2));
+      const float c2 = AR::mad(A.xi[oh], dci, cn2);
+      sc[jj * CK + kk] = make_float2(c2, AR::sqrt(c2));
     }
     __syncthreads();
 
     // ---- B: edge states. Own cell in y and z; the tile's outer neighbours for the boundary faces.
     float4 loy, loz, hi;
-    edges<LM>(sq[cq - PK], sq[cq], sq[cq + PK], loy, hi);
+    edges<AR, LM>(sq[cq - PK], sq[cq], sq[cq + PK], loy, hi);
     ehy[(ty + 1) * TK + tx] = hi;
-    edges<LM>(sq[cq - 1], sq[cq], sq[cq + 1], loz, hi);
+    edges<AR, LM>(sq[cq - 1], sq[cq], sq[cq + 1], loz, hi);
     ehz[ty * (TK + 1) + tx + 1] = hi;
     {
       float4 lo_, hi_;
       if (tid < TK) {                        // row -1: high y edge
         const int bq = PK + tid + 2;
-        edges<LM>(sq[bq - PK], sq[bq], sq[bq + PK], lo_, hi_);
+        edges<AR, LM>(sq[bq - PK], sq[bq], sq[bq + PK], lo_, hi_);
         ehy[tid] = hi_;
       } else if (tid < 2 * TK) {             // row TJ: low y edge
         const int t = tid - TK;
         const int bq = (TJ + 2) * PK + t + 2;
-        edges<LM>(sq[bq - PK], sq[bq], sq[bq + PK], lo_, hi_);
+        edges<AR, LM>(sq[bq - PK], sq[bq], sq[bq + PK], lo_, hi_);
         ely[t] = lo_;
       } else if (tid < 2 * TK + TJ) {        // column -1: high z edge
         const int r = tid - 2 * TK;
         const int bq = (r + 2) * PK + 1;
-        edges<LM>(sq[bq - 1], sq[bq], sq[bq + 1], lo_, hi_);
+        edges<AR, LM>(sq[bq - 1], sq[bq], sq[bq + 1], lo_, hi_);
         ehz[r * (TK + 1)] = hi_;
       } else if (tid < 2 * TK + 2 * TJ) {    // column TK: low z edge
         const int r = tid - 2 * TK - TJ;
         const int bq = (r + 2) * PK + TK + 2;
-        edges<LM>(sq[bq - 1], sq[bq], sq[bq + 1], lo_, hi_);
+        edges<AR, LM>(sq[bq - 1], sq[bq], sq[bq + 1], lo_, hi_);
         elz[r] = lo_;
       }
     }
     // x: the window gives plane i+1's edges; its low edge closes face i+1/2, its high edge is carried
     float4 lox1, hix1;
-    edges<LM>(qw[0], qw[1], qw[2], lox1, hix1);
-    const float4 fRx = face_flux<0, SOLVER>(hix, lox1, c2w[0], c2w[1], cw[0], cw[1], floor_);
+    edges<AR, LM>(qw[0], qw[1], qw[2], lox1, hix1);
+    const float4 fRx = face_flux<AR, 0, SOLVER>(hix, lox1, c2w[0], c2w[1], cw[0], cw[1], floor_);
     __syncthreads();
 
     // ---- C: the low y and z faces of every cell, and the tile's high y and z boundary faces
     {
       const float2 cy0 = sc[cc - CK], c1 = sc[cc], cz0 = sc[cc - 1];
-      fy[ty * TK + tx] = face_flux<1, SOLVER>(ehy[ty * TK + tx], loy, cy0.x, c1.x, cy0.y, c1.y, floor_);
-      fz[ty * (TK + 1) + tx] = face_flux<2, SOLVER>(ehz[ty * (TK + 1) + tx], loz, cz0.x, c1.x, cz0.y, c1.y, floor_);
+      fy[ty * TK + tx] = face_flux<AR, 1, SOLVER>(ehy[ty * TK + tx], loy, cy0.x, c1.x, cy0.y, c1.y, floor_);
+      fz[ty * (TK + 1) + tx] = face_flux<AR, 2, SOLVER>(ehz[ty * (TK + 1) + tx], loz, cz0.x, c1.x, cz0.y, c1.y,
+                                                        floor_);
       if (tid < TK) {
         const int bc = TJ * CK + tid + 1;    // c plane: row TJ-1 at bc, row TJ at bc + CK
         const float2 a0 = sc[bc], a1 = sc[bc + CK];
-        fy[TJ * TK + tid] = face_flux<1, SOLVER>(ehy[TJ * TK + tid], ely[tid], a0.x, a1.x, a0.y, a1.y, floor_);
+        fy[TJ * TK + tid] = face_flux<AR, 1, SOLVER>(ehy[TJ * TK + tid], ely[tid], a0.x, a1.x, a0.y, a1.y, floor_);
       } else if (tid < TK + TJ) {
         const int r = tid - TK;
         const int bc = (r + 1) * CK + TK;    // column TK-1 at bc, column TK at bc + 1
         const float2 a0 = sc[bc], a1 = sc[bc + 1];
-        fz[r * (TK + 1) + TK] = face_flux<2, SOLVER>(ehz[r * (TK + 1) + TK], elz[r], a0.x, a1.x, a0.y, a1.y,
-                                                     floor_);
+        fz[r * (TK + 1) + TK] = face_flux<AR, 2, SOLVER>(ehz[r * (TK + 1) + TK], elz[r], a0.x, a1.x, a0.y, a1.y,
+                                                         floor_);
       }
     }
     __syncthreads();
@@ -326,7 +419,7 @@ __global__ __launch_bounds__(NT, 4) void hydro_stage_kernel(HydroArgs A) {
       const float U[4] = {Uc.x, Uc.y, Uc.z, Uc.w};
       float dU[4];
 #pragma unroll
-      for (int c = 0; c < 4; ++c) dU[c] = div_r(-div[c], rdx);
+      for (int c = 0; c < 4; ++c) dU[c] = AR::div(-div[c], rdx);
       const float p0 = ((float)i + 0.5f) * dx, p1 = ((float)j + 0.5f) * dx, p2 = ((float)k + 0.5f) * dx;
       const float x_ion = xraw[0];
       if (A.wind > 0.0f && x_ion > 0.01f) {
@@ -378,17 +471,18 @@ __global__ __launch_bounds__(NT, 4) void hydro_stage_kernel(HydroArgs A) {
       float out[4];
       if constexpr (STAGE1) {
 #pragma unroll
-        for (int c = 0; c < 4; ++c) out[c] = U[c] + A.dt * dU[c];
+        for (int c = 0; c < 4; ++c) out[c] = AR::mad(A.dt, dU[c], U[c]);
       } else {
-        const float U0v[4] = {A.u0_is_u ? U[0] : U0.x, A.u0_is_u ? U[1] : U0.y, A.u0_is_u ? U[2] : U0.z,
-                              A.u0_is_u ? U[3] : U0.w};
+        const float4 U0c = conserved<ST>(U0raw);
+        const float U0v[4] = {A.u0_is_u ? U[0] : U0c.x, A.u0_is_u ? U[1] : U0c.y, A.u0_is_u ? U[2] : U0c.z,
+                              A.u0_is_u ? U[3] : U0c.w};
 #pragma unroll
-        for (int c = 0; c < 4; ++c) out[c] = A.a * U0v[c] + A.b * (U[c] + A.dt * dU[c]);
+        for (int c = 0; c < 4; ++c) out[c] = AR::mad(A.a, U0v[c], A.b * AR::mad(A.dt, dU[c], U[c]));
       }
       const float r = fmaxf(out[0], floor_);
-      const double rr = recip_d(r);
-      float v0 = div_r(out[1], rr), v1 = div_r(out[2], rr), v2 = div_r(out[3], rr);
-      const float s2 = (v0 * v0 + v1 * v1) + v2 * v2;
+      const Rcp rr = AR::rcp(r);
+      float v0 = AR::div(out[1], rr), v1 = AR::div(out[2], rr), v2 = AR::div(out[3], rr);
+      const float s2 = AR::mad(v2, v2, AR::mad(v1, v1, v0 * v0));
       if (s2 > A.vmax2_lo) {
         // |v| = sqrt(s2) may exceed vmax: the reference test and cap (sim.py:491-494)
         const float sp = sqrtf(s2);
@@ -399,15 +493,11 @@ __global__ __launch_bounds__(NT, 4) void hydro_stage_kernel(HydroArgs A) {
           v2 = v2 * f;
         }
       }
-      const float m0 = r * v0, m1 = r * v1, m2 = r * v2;
-      A.o_r[o] = r;
-      A.o_mx[o] = m0;
-      A.o_my[o] = m1;
-      A.o_mz[o] = m2;
+      const float3 vs = ST::template store<AR>(A.o0, A.o1, A.o2, A.o3, o, r, rr, v0, v1, v2, SMAX);
       if constexpr (SMAX) {
-        // max_speed of the written state (nebula3d.py:154-162): raw stored rho, L1 norm of v; the sound speed
-        // sqrt(cn2 + xi (ci2 - cn2)) of this cell is cw[0]
-        const float s = ((fabsf(div_r(m0, rr)) + fabsf(div_r(m1, rr))) + fabsf(div_r(m2, rr))) + cw[0];
+        // max_speed of the written state (nebula3d.py:154-162): L1 norm of its velocity plus this cell's
+        // sound speed sqrt(cn2 + xi (ci2 - cn2)), which is cw[0]
+        const float s = ((fabsf(vs.x) + fabsf(vs.y)) + fabsf(vs.z)) + cw[0];
         tmax = fmaxf(tmax, s);
       }
     }
@@ -415,16 +505,17 @@ __global__ __launch_bounds__(NT, 4) void hydro_stage_kernel(HydroArgs A) {
     hix = hix1;
 
     // ---- shift the window by one plane
+    float4 Unext;
     qw[0] = qw[1];
     qw[1] = qw[2];
-    qw[2] = prim(Unext.x, Unext.y, Unext.z, Unext.w, floor_);
-    c2w[0] = c2w[1];
-    cw[0] = cw[1];
-    c2w[1] = cn2 + xraw[2] * dci;   // plane i+2 becomes plane (i+1)+1
-    cw[1] = sqrtf(c2w[1]);
+    decode<ST, AR>(rnext, floor_, qw[2], Unext);
     Uraw[0] = Uraw[1];
     Uraw[1] = Uraw[2];
     Uraw[2] = Unext;
+    c2w[0] = c2w[1];
+    cw[0] = cw[1];
+    c2w[1] = AR::mad(xraw[2], dci, cn2);   // plane i+2 becomes plane (i+1)+1
+    cw[1] = AR::sqrt(c2w[1]);
     xraw[0] = xraw[1];
     xraw[1] = xraw[2];
     xraw[2] = xnext;
@@ -432,7 +523,7 @@ __global__ __launch_bounds__(NT, 4) void hydro_stage_kernel(HydroArgs A) {
   if constexpr (SMAX) block_max_atomic(tmax, red, A.smax);
 }
 
-// max_speed (nebula3d.py:154-162)
+// max_speed (nebula3d.py:154-162) of float32 (rho, m) fields
 __global__ __launch_bounds__(256) void max_speed_kernel(const float* rho, const float* mx, const float* my,
                                                         const float* mz, const float* xi, long long n, float cn2,
                                                         float ci2, float* out) {
@@ -447,44 +538,62 @@ __global__ __launch_bounds__(256) void max_speed_kernel(const float* rho, const 
   block_max_atomic(m, red, out);
 }
 
-using HydroKernel = void (*)(HydroArgs);
-
-template <bool SMAX, bool STAGE1>
-static HydroKernel pick_hydro(int solver, int lim) {
-  if (solver == NV_SOLVER_RUSANOV) return hydro_stage_kernel<NV_SOLVER_RUSANOV, NV_LIM_MINMOD, SMAX, STAGE1>;
-  if (lim == NV_LIM_VANLEER) return hydro_stage_kernel<NV_SOLVER_HLL, NV_LIM_VANLEER, SMAX, STAGE1>;
-  if (lim == NV_LIM_MC) return hydro_stage_kernel<NV_SOLVER_HLL, NV_LIM_MC, SMAX, STAGE1>;
-  return hydro_stage_kernel<NV_SOLVER_HLL, NV_LIM_MINMOD, SMAX, STAGE1>;
+// the same for binary16 (rho, v) fields: the velocity is stored
+__global__ __launch_bounds__(256) void max_speed_h_kernel(const uint16_t* vx, const uint16_t* vy, const uint16_t* vz,
+                                                          const float* xi, long long n, float cn2, float ci2,
+                                                          float* out) {
+  __shared__ float red[4];
+  const float dci = ci2 - cn2;
+  float m = 0.0f;
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long long)gridDim.x * blockDim.x) {
+    const float s = ((fabsf(h2f(vx[q])) + fabsf(h2f(vy[q]))) + fabsf(h2f(vz[q]))) + sqrtf(cn2 + xi[q] * dci);
+    m = fmaxf(m, s);
+  }
+  block_max_atomic(m, red, out);
 }
 
-}  // namespace nv
+using HydroKernel = void (*)(HydroArgs);
 
-extern "C" int nv_hydro_stage(const float* rho, const float* mx, const float* my, const float* mz, const float* xi,
-                              const float* r0, const float* mx0, const float* my0, const float* mz0, float* o_r,
-                              float* o_mx, float* o_my, float* o_mz, int nx, int ny, int nz, const float* stars,
-                              int nstars, float wind, const float* jets, float jet_cos, float jet_r, float pulse,
-                              float dt, float dx, float a, float b, float floor_, float cn2, float ci2, float vmax,
-                              int solver, int lim, float* smax_out, void* stream) {
-  using namespace nv;
+template <class ST, class AR, bool SMAX, bool STAGE1>
+static HydroKernel pick_lim(int solver, int lim) {
+  if (solver == NV_SOLVER_RUSANOV) return hydro_stage_kernel<ST, AR, NV_SOLVER_RUSANOV, NV_LIM_MINMOD, SMAX, STAGE1>;
+  if (lim == NV_LIM_VANLEER) return hydro_stage_kernel<ST, AR, NV_SOLVER_HLL, NV_LIM_VANLEER, SMAX, STAGE1>;
+  if (lim == NV_LIM_MC) return hydro_stage_kernel<ST, AR, NV_SOLVER_HLL, NV_LIM_MC, SMAX, STAGE1>;
+  return hydro_stage_kernel<ST, AR, NV_SOLVER_HLL, NV_LIM_MINMOD, SMAX, STAGE1>;
+}
+
+template <class ST, class AR>
+static HydroKernel pick(int solver, int lim, bool smax, bool stage1) {
+  if (smax) return stage1 ? pick_lim<ST, AR, true, true>(solver, lim) : pick_lim<ST, AR, true, false>(solver, lim);
+  return stage1 ? pick_lim<ST, AR, false, true>(solver, lim) : pick_lim<ST, AR, false, false>(solver, lim);
+}
+
+enum class Mode { F32_EXACT, F32_FAST, F16V_FAST };
+
+static int launch_stage(Mode mode, const char* name, const void* f0, const void* f1, const void* f2, const void* f3,
+                        const float* xi, const void* g0, const void* g1, const void* g2, const void* g3, void* o0,
+                        void* o1, void* o2, void* o3, int nx, int ny, int nz, const float* stars, int nstars,
+                        float wind, const float* jets, float jet_cos, float jet_r, float pulse, float dt, float dx,
+                        float a, float b, float floor_, float cn2, float ci2, float vmax, int solver, int lim,
+                        float* smax_out, void* stream) {
   clear_error();
-  NV_REQUIRE(rho && mx && my && mz && xi && r0 && mx0 && my0 && mz0 && o_r && o_mx && o_my && o_mz,
-             "nv_hydro_stage: null field pointer");
-  NV_REQUIRE(nx > 0 && ny > 0 && nz > 0, "nv_hydro_stage: bad grid %dx%dx%d", nx, ny, nz);
-  NV_REQUIRE(nstars >= 0 && (nstars == 0 || (stars && jets)), "nv_hydro_stage: nstars=%d needs star/jet tables",
-             nstars);
-  NV_REQUIRE(solver == NV_SOLVER_RUSANOV || solver == NV_SOLVER_HLL, "nv_hydro_stage: bad solver %d", solver);
-  NV_REQUIRE(lim >= NV_LIM_MINMOD && lim <= NV_LIM_MC, "nv_hydro_stage: bad limiter %d", lim);
-  NV_REQUIRE(o_r != rho && o_mx != mx && o_my != my && o_mz != mz, "nv_hydro_stage: output aliases the input U");
-  NV_REQUIRE(floor_ > 0.0f && dx > 0.0f, "nv_hydro_stage: floor and dx must be > 0");
+  NV_REQUIRE(f0 && f1 && f2 && f3 && xi && g0 && g1 && g2 && g3 && o0 && o1 && o2 && o3, "%s: null field pointer",
+             name);
+  NV_REQUIRE(nx > 0 && ny > 0 && nz > 0, "%s: bad grid %dx%dx%d", name, nx, ny, nz);
+  NV_REQUIRE(nstars >= 0 && (nstars == 0 || (stars && jets)), "%s: nstars=%d needs star/jet tables", name, nstars);
+  NV_REQUIRE(solver == NV_SOLVER_RUSANOV || solver == NV_SOLVER_HLL, "%s: bad solver %d", name, solver);
+  NV_REQUIRE(lim >= NV_LIM_MINMOD && lim <= NV_LIM_MC, "%s: bad limiter %d", name, lim);
+  NV_REQUIRE(o0 != f0 && o1 != f1 && o2 != f2 && o3 != f3, "%s: output aliases the input U", name);
+  NV_REQUIRE(floor_ > 0.0f && dx > 0.0f, "%s: floor and dx must be > 0", name);
   HydroArgs A;
-  A.rho = rho; A.mx = mx; A.my = my; A.mz = mz; A.xi = xi;
-  A.r0 = r0; A.mx0 = mx0; A.my0 = my0; A.mz0 = mz0;
-  A.o_r = o_r; A.o_mx = o_mx; A.o_my = o_my; A.o_mz = o_mz;
+  A.f0 = f0; A.f1 = f1; A.f2 = f2; A.f3 = f3; A.xi = xi;
+  A.g0 = g0; A.g1 = g1; A.g2 = g2; A.g3 = g3;
+  A.o0 = o0; A.o1 = o1; A.o2 = o2; A.o3 = o3;
   A.stars = reinterpret_cast<const float4*>(stars);
   A.jets = reinterpret_cast<const float4*>(jets);
   A.smax = smax_out;
   A.nx = nx; A.ny = ny; A.nz = nz; A.nstars = nstars;
-  A.u0_is_u = (r0 == rho && mx0 == mx && my0 == my && mz0 == mz) ? 1 : 0;
+  A.u0_is_u = (g0 == f0 && g1 == f1 && g2 == f2 && g3 == f3) ? 1 : 0;
   A.wind = wind; A.jet_cos = jet_cos; A.jet_r = jet_r; A.pulse = pulse; A.dt = dt; A.dx = dx;
   A.a = a; A.b = b; A.floor_ = floor_; A.cn2 = cn2; A.ci2 = ci2; A.vmax = vmax;
   {
@@ -495,19 +604,57 @@ extern "C" int nv_hydro_stage(const float* rho, const float* mx, const float* my
   }
   // march depth: long enough to amortise the 4-plane prologue, short enough to fill the chip
   const long long tiles_jk = (long long)((nz + TK - 1) / TK) * ((ny + TJ - 1) / TJ);
-  int ci = (int)std::min<long long>(64, std::max<long long>(4, (long long)nx * tiles_jk / 2048));
+  const int ci = (int)std::min<long long>(64, std::max<long long>(4, (long long)nx * tiles_jk / 2048));
   A.ci = ci;
   A.tiles_k = (nz + TK - 1) / TK;
   A.tiles_j = (ny + TJ - 1) / TJ;
   A.tiles = A.tiles_k * A.tiles_j * ((nx + ci - 1) / ci);
   A.per_xcd = (A.tiles + 7) / 8;
-  dim3 grid(8 * A.per_xcd);
-  dim3 block(TK, TJ, 1);
   const bool stage1 = A.u0_is_u && a == 0.0f && b == 1.0f;
-  HydroKernel kern = smax_out ? (stage1 ? pick_hydro<true, true>(solver, lim) : pick_hydro<true, false>(solver, lim))
-                              : (stage1 ? pick_hydro<false, true>(solver, lim) : pick_hydro<false, false>(solver, lim));
-  hipLaunchKernelGGL(kern, grid, block, 0, as_stream(stream), A);
-  return check_launch("nv_hydro_stage");
+  const bool sm = smax_out != nullptr;
+  HydroKernel kern = mode == Mode::F32_EXACT  ? pick<F32, Exact>(solver, lim, sm, stage1)
+                     : mode == Mode::F32_FAST ? pick<F32, Fast>(solver, lim, sm, stage1)
+                                              : pick<F16V, Fast>(solver, lim, sm, stage1);
+  hipLaunchKernelGGL(kern, dim3(8 * A.per_xcd), dim3(TK, TJ, 1), 0, as_stream(stream), A);
+  return check_launch(name);
+}
+
+}  // namespace nv
+
+#define NV_STAGE_TAIL                                                                                      \
+  nx, ny, nz, stars, nstars, wind, jets, jet_cos, jet_r, pulse, dt, dx, a, b, floor_, cn2, ci2, vmax, solver, lim, \
+      smax_out, stream
+
+extern "C" int nv_hydro_stage(const float* rho, const float* mx, const float* my, const float* mz, const float* xi,
+                              const float* r0, const float* mx0, const float* my0, const float* mz0, float* o_r,
+                              float* o_mx, float* o_my, float* o_mz, int nx, int ny, int nz, const float* stars,
+                              int nstars, float wind, const float* jets, float jet_cos, float jet_r, float pulse,
+                              float dt, float dx, float a, float b, float floor_, float cn2, float ci2, float vmax,
+                              int solver, int lim, float* smax_out, void* stream) {
+  return nv::launch_stage(nv::Mode::F32_EXACT, "nv_hydro_stage", rho, mx, my, mz, xi, r0, mx0, my0, mz0, o_r, o_mx,
+                          o_my, o_mz, NV_STAGE_TAIL);
+}
+
+extern "C" int nv_hydro_stage_fast(const float* rho, const float* mx, const float* my, const float* mz,
+                                   const float* xi, const float* r0, const float* mx0, const float* my0,
+                                   const float* mz0, float* o_r, float* o_mx, float* o_my, float* o_mz, int nx, int ny,
+                                   int nz, const float* stars, int nstars, float wind, const float* jets,
+                                   float jet_cos, float jet_r, float pulse, float dt, float dx, float a, float b,
+                                   float floor_, float cn2, float ci2, float vmax, int solver, int lim,
+                                   float* smax_out, void* stream) {
+  return nv::launch_stage(nv::Mode::F32_FAST, "nv_hydro_stage_fast", rho, mx, my, mz, xi, r0, mx0, my0, mz0, o_r,
+                          o_mx, o_my, o_mz, NV_STAGE_TAIL);
+}
+
+extern "C" int nv_hydro_stage_h(const float* rho, const uint16_t* vx, const uint16_t* vy, const uint16_t* vz,
+                                const float* xi, const float* r0, const uint16_t* vx0, const uint16_t* vy0,
+                                const uint16_t* vz0, float* o_r, uint16_t* o_vx, uint16_t* o_vy, uint16_t* o_vz,
+                                int nx, int ny, int nz, const float* stars, int nstars, float wind, const float* jets,
+                                float jet_cos, float jet_r, float pulse, float dt, float dx, float a, float b,
+                                float floor_, float cn2, float ci2, float vmax, int solver, int lim, float* smax_out,
+                                void* stream) {
+  return nv::launch_stage(nv::Mode::F16V_FAST, "nv_hydro_stage_h", rho, vx, vy, vz, xi, r0, vx0, vy0, vz0, o_r, o_vx,
+                          o_vy, o_vz, NV_STAGE_TAIL);
 }
 
 extern "C" int nv_max_speed(const float* rho, const float* mx, const float* my, const float* mz, const float* xi,
@@ -521,4 +668,17 @@ extern "C" int nv_max_speed(const float* rho, const float* mx, const float* my, 
   hipLaunchKernelGGL(max_speed_kernel, dim3(blocks), dim3(256), 0, as_stream(stream), rho, mx, my, mz, xi, n, cn2,
                      ci2, out);
   return check_launch("nv_max_speed");
+}
+
+extern "C" int nv_max_speed_h(const uint16_t* vx, const uint16_t* vy, const uint16_t* vz, const float* xi, int nx,
+                              int ny, int nz, float cn2, float ci2, float* out, void* stream) {
+  using namespace nv;
+  clear_error();
+  NV_REQUIRE(vx && vy && vz && xi && out, "nv_max_speed_h: null pointer");
+  NV_REQUIRE(nx > 0 && ny > 0 && nz > 0, "nv_max_speed_h: bad grid %dx%dx%d", nx, ny, nz);
+  const long long n = (long long)nx * ny * nz;
+  const int blocks = (int)std::min<long long>((n + 255) / 256, 8192);
+  hipLaunchKernelGGL(max_speed_h_kernel, dim3(blocks), dim3(256), 0, as_stream(stream), vx, vy, vz, xi, n, cn2, ci2,
+                     out);
+  return check_launch("nv_max_speed_h");
 }

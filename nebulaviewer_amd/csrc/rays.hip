@@ -18,30 +18,6 @@ namespace nv {
 
 constexpr float PI_F = 3.14159265358979323846f;
 
-// trilinear density at world p, 0 outside [-1/2, n-1/2] in cell units (nebula3d.py:195-226)
-__device__ __forceinline__ float sample_n(const float* __restrict__ f, int nx, int ny, int nz, float px, float py,
-                                          float pz, float dx) {
-  const float gx = px / dx - 0.5f, gy = py / dx - 0.5f, gz = pz / dx - 0.5f;
-  if (gx < -0.5f || gy < -0.5f || gz < -0.5f) return 0.0f;
-  if (gx > (float)nx - 0.5f || gy > (float)ny - 0.5f || gz > (float)nz - 0.5f) return 0.0f;
-  const int i0 = (int)floorf(gx), j0 = (int)floorf(gy), k0 = (int)floorf(gz);
-  const float fx = gx - (float)i0, fy = gy - (float)j0, fz = gz - (float)k0;
-  const int ia = clampi(i0, 0, nx - 1), ib = clampi(i0 + 1, 0, nx - 1);
-  const int ja = clampi(j0, 0, ny - 1), jb = clampi(j0 + 1, 0, ny - 1);
-  const int ka = clampi(k0, 0, nz - 1), kb = clampi(k0 + 1, 0, nz - 1);
-  const float c000 = f[cell(ia, ja, ka, ny, nz)], c100 = f[cell(ib, ja, ka, ny, nz)];
-  const float c010 = f[cell(ia, jb, ka, ny, nz)], c110 = f[cell(ib, jb, ka, ny, nz)];
-  const float c001 = f[cell(ia, ja, kb, ny, nz)], c101 = f[cell(ib, ja, kb, ny, nz)];
-  const float c011 = f[cell(ia, jb, kb, ny, nz)], c111 = f[cell(ib, jb, kb, ny, nz)];
-  const float c00 = c000 + (c100 - c000) * fx;
-  const float c10 = c010 + (c110 - c010) * fx;
-  const float c01 = c001 + (c101 - c001) * fx;
-  const float c11 = c011 + (c111 - c011) * fx;
-  const float c0 = c00 + (c10 - c00) * fy;
-  const float c1 = c01 + (c11 - c01) * fy;
-  return c0 + (c1 - c0) * fz;
-}
-
 // slab test against [0, bmax] (nebula3d.py:229-244)
 __device__ __forceinline__ void box_range(const float o[3], const float d[3], const float bm[3], float* t0o,
                                           float* t1o) {
@@ -73,9 +49,16 @@ __device__ __forceinline__ void ray_dir(int iu, int ip, int nu, int nphi, float 
   d[2] = u;
 }
 
-// sample_n with the division p/dx done through a shared reciprocal (the identical correctly rounded
-// quotient, nv_divide.h) and 32-bit cell indices (grids up to 2^31 cells)
-__device__ __forceinline__ float sample_r(const float* __restrict__ f, int nx, int ny, int nz, float px, float py,
+__device__ __forceinline__ float ldf(const float* __restrict__ f, unsigned i) { return f[i]; }
+__device__ __forceinline__ float ldf(const uint16_t* __restrict__ f, unsigned i) {
+  return (float)__builtin_bit_cast(_Float16, f[i]);
+}
+
+// trilinear sample at world p, 0 outside [-1/2, n-1/2] in cell units (sample_n, nebula3d.py:195-226), with the
+// division p/dx done through a shared reciprocal (the identical correctly rounded quotient, nv_divide.h) and
+// 32-bit cell indices (grids up to 2^32 cells); T = float or binary16 bits
+template <class T>
+__device__ __forceinline__ float sample_r(const T* __restrict__ f, int nx, int ny, int nz, float px, float py,
                                           float pz, double rdx) {
   const float gx = div_r(px, rdx) - 0.5f, gy = div_r(py, rdx) - 0.5f, gz = div_r(pz, rdx) - 0.5f;
   if (gx < -0.5f || gy < -0.5f || gz < -0.5f) return 0.0f;
@@ -87,8 +70,8 @@ __device__ __forceinline__ float sample_r(const float* __restrict__ f, int nx, i
   const unsigned ka = clampi(k0, 0, nz - 1), kb = clampi(k0 + 1, 0, nz - 1);
   const unsigned r00 = (ia * ny + ja) * nz, r10 = (ib * ny + ja) * nz;
   const unsigned r01 = (ia * ny + jb) * nz, r11 = (ib * ny + jb) * nz;
-  const float c000 = f[r00 + ka], c100 = f[r10 + ka], c010 = f[r01 + ka], c110 = f[r11 + ka];
-  const float c001 = f[r00 + kb], c101 = f[r10 + kb], c011 = f[r01 + kb], c111 = f[r11 + kb];
+  const float c000 = ldf(f, r00 + ka), c100 = ldf(f, r10 + ka), c010 = ldf(f, r01 + ka), c110 = ldf(f, r11 + ka);
+  const float c001 = ldf(f, r00 + kb), c101 = ldf(f, r10 + kb), c011 = ldf(f, r01 + kb), c111 = ldf(f, r11 + kb);
   const float c00 = c000 + (c100 - c000) * fx;
   const float c10 = c010 + (c110 - c010) * fx;
   const float c01 = c001 + (c101 - c001) * fx;
@@ -111,7 +94,8 @@ __device__ __forceinline__ bool ray_of_thread(long long t, int nu, int nphi, int
 
 // cast_rays (nebula3d.py:247-270). Two samples per iteration: the corner loads of sample n+1 are in flight
 // while sample n is tested, and the column is still accumulated sample by sample in the reference order.
-__global__ __launch_bounds__(256) void cast_rays_kernel(const float* __restrict__ rho, int nx, int ny, int nz,
+template <class T>
+__global__ __launch_bounds__(256) void cast_rays_kernel(const T* __restrict__ rho, int nx, int ny, int nz,
                                                         float sx, float sy, float sz, float Q, float dr,
                                                         int nsteps, float dx, float rcap, float bx, float by,
                                                         float bz, float* __restrict__ rfront, int nu, int nphi) {
@@ -229,13 +213,13 @@ struct IonArgs {
   float* xi;
   uint16_t* df_h;
   float* df_f;
-  const float *rho, *mx, *my, *mz;
+  const void *f0, *f1, *f2, *f3;   // state for the CFL max: (rho, m) float32, or (rho, v) binary16
   float* smax;
   int nu, nphi, nstars, nx, ny, nz, ci;
   float dx, width, cn2, ci2;
 };
 
-template <bool SMAX>
+template <bool SMAX, bool VEL>
 __global__ __launch_bounds__(INT_) void ionize_apply_kernel(IonArgs A) {
   __shared__ float sx_[XR][XN];   // unsmoothed xi (max over stars of the front ramp)
   __shared__ float sd_[XR][XN];   // fp32 dfront (min over stars of r - rf)
@@ -290,9 +274,15 @@ __global__ __launch_bounds__(INT_) void ionize_apply_kernel(IonArgs A) {
       if (A.df_h) A.df_h[o] = f32_to_f16_bits(df);
       else A.df_f[o] = df;
       if constexpr (SMAX) {
-        const float rr = A.rho[o];
-        const float s = ((fabsf(A.mx[o] / rr) + fabsf(A.my[o] / rr)) + fabsf(A.mz[o] / rr)) +
-                        sqrtf(A.cn2 + xs * (A.ci2 - A.cn2));
+        float s;
+        if constexpr (VEL) {
+          s = ((fabsf(ldf((const uint16_t*)A.f1, o)) + fabsf(ldf((const uint16_t*)A.f2, o))) +
+               fabsf(ldf((const uint16_t*)A.f3, o))) + sqrtf(A.cn2 + xs * (A.ci2 - A.cn2));
+        } else {
+          const float rr = ((const float*)A.f0)[o];
+          s = ((fabsf(((const float*)A.f1)[o] / rr) + fabsf(((const float*)A.f2)[o] / rr)) +
+               fabsf(((const float*)A.f3)[o] / rr)) + sqrtf(A.cn2 + xs * (A.ci2 - A.cn2));
+        }
         tmax = fmaxf(tmax, s);
       }
     }
@@ -302,7 +292,8 @@ __global__ __launch_bounds__(INT_) void ionize_apply_kernel(IonArgs A) {
 
 // ---------------------------------------------------------------------------------------- FUV
 // cast_fuv (nebula3d.py:320-352)
-__global__ __launch_bounds__(256) void cast_fuv_kernel(const float* __restrict__ rho, const float* __restrict__ xi,
+template <class T>
+__global__ __launch_bounds__(256) void cast_fuv_kernel(const T* __restrict__ rho, const float* __restrict__ xi,
                                                        int nx, int ny, int nz, float sx, float sy, float sz,
                                                        float kappa, float dr, int nsteps, float dx, float binr,
                                                        float bx, float by, float bz, float* __restrict__ att, int nr,
@@ -320,11 +311,12 @@ __global__ __launch_bounds__(256) void cast_fuv_kernel(const float* __restrict__
   const int n1 = min(nsteps, (int)(t1 / dr) + 2);
   float tau = 0.0f;
   int ib = 0;
+  const double rdx = recip_d(dx);
   for (int n = n0; n < n1; ++n) {
     const float r = ((float)n + 0.5f) * dr;
     const float qx = sx + d[0] * r, qy = sy + d[1] * r, qz = sz + d[2] * r;
-    const float n_tot = sample_n(rho, nx, ny, nz, qx, qy, qz, dx);
-    const float x_i = sample_n(xi, nx, ny, nz, qx, qy, qz, dx);
+    const float n_tot = sample_r(rho, nx, ny, nz, qx, qy, qz, rdx);
+    const float x_i = sample_r(xi, nx, ny, nz, qx, qy, qz, rdx);
     tau = tau + kappa * n_tot * (1.0f - 0.92f * x_i) * dr;
     const int nb = (int)(r / binr);
     if (nb > ib) {
@@ -407,18 +399,26 @@ inline int blocks_for(long long n, int bs) { return (int)((n + bs - 1) / bs); }
 
 using namespace nv;
 
+template <class T>
+static int cast_rays_launch(const char* name, const T* rho, int nx, int ny, int nz, float sx, float sy, float sz,
+                            float Q, float dr, int nsteps, float dx, float rcap, float bx, float by, float bz,
+                            float* rfront, int nu, int nphi, void* stream) {
+  clear_error();
+  NV_REQUIRE(rho && rfront, "%s: null pointer", name);
+  NV_REQUIRE(nx > 0 && ny > 0 && nz > 0 && nu > 0 && nphi > 0, "%s: bad sizes", name);
+  NV_REQUIRE(dr > 0.0f && dx > 0.0f, "%s: dr and dx must be > 0", name);
+  NV_REQUIRE((long long)nx * ny * nz < (1LL << 32), "%s: grids above 2^32 cells are not supported", name);
+  const long long n = (long long)((nu + 7) / 8) * ((nphi + 7) / 8) * 64;
+  hipLaunchKernelGGL(cast_rays_kernel<T>, dim3(blocks_for(n, 256)), dim3(256), 0, as_stream(stream), rho, nx, ny, nz,
+                     sx, sy, sz, Q, dr, nsteps, dx, rcap, bx, by, bz, rfront, nu, nphi);
+  return check_launch(name);
+}
+
 extern "C" int nv_cast_rays(const float* rho, int nx, int ny, int nz, float sx, float sy, float sz, float Q, float dr,
                             int nsteps, float dx, float rcap, float bx, float by, float bz, float* rfront, int nu,
                             int nphi, void* stream) {
-  clear_error();
-  NV_REQUIRE(rho && rfront, "nv_cast_rays: null pointer");
-  NV_REQUIRE(nx > 0 && ny > 0 && nz > 0 && nu > 0 && nphi > 0, "nv_cast_rays: bad sizes");
-  NV_REQUIRE(dr > 0.0f && dx > 0.0f, "nv_cast_rays: dr and dx must be > 0");
-  NV_REQUIRE((long long)nx * ny * nz < (1LL << 32), "nv_cast_rays: grids above 2^32 cells are not supported");
-  const long long n = (long long)((nu + 7) / 8) * ((nphi + 7) / 8) * 64;
-  hipLaunchKernelGGL(cast_rays_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, as_stream(stream), rho, nx, ny, nz, sx,
-                     sy, sz, Q, dr, nsteps, dx, rcap, bx, by, bz, rfront, nu, nphi);
-  return check_launch("nv_cast_rays");
+  return cast_rays_launch("nv_cast_rays", rho, nx, ny, nz, sx, sy, sz, Q, dr, nsteps, dx, rcap, bx, by, bz, rfront,
+                          nu, nphi, stream);
 }
 
 extern "C" int nv_apply_front(const float* rfront, int nu, int nphi, float sx, float sy, float sz, float dx,
@@ -460,41 +460,65 @@ extern "C" int nv_fill_f32(float* dst, float v, long long n, void* stream) {
   return check_launch("nv_fill_f32");
 }
 
-extern "C" int nv_ionize_apply(const float* rfronts, int nu, int nphi, const float* star_xyz, int nstars, float dx,
-                               float width, float* xi, uint16_t* dfront_h, float* dfront_f, int nx, int ny, int nz,
-                               const float* rho, const float* mx, const float* my, const float* mz, float cn2,
-                               float ci2, float* smax_out, void* stream) {
+static int ionize_apply_launch(const char* name, bool vel, const float* rfronts, int nu, int nphi,
+                               const float* star_xyz, int nstars, float dx, float width, float* xi, uint16_t* dfront_h,
+                               float* dfront_f, int nx, int ny, int nz, const void* f0, const void* f1,
+                               const void* f2, const void* f3, float cn2, float ci2, float* smax_out, void* stream) {
   clear_error();
-  NV_REQUIRE(xi && (nstars == 0 || (rfronts && star_xyz)), "nv_ionize_apply: null pointer");
-  NV_REQUIRE((dfront_h != nullptr) != (dfront_f != nullptr), "nv_ionize_apply: pass exactly one dfront buffer");
-  NV_REQUIRE(nx > 0 && ny > 0 && nz > 0 && nu > 0 && nphi > 0 && nstars >= 0, "nv_ionize_apply: bad sizes");
-  NV_REQUIRE(!smax_out || (rho && mx && my && mz), "nv_ionize_apply: smax needs rho and momentum");
+  NV_REQUIRE(xi && (nstars == 0 || (rfronts && star_xyz)), "%s: null pointer", name);
+  NV_REQUIRE((dfront_h != nullptr) != (dfront_f != nullptr), "%s: pass exactly one dfront buffer", name);
+  NV_REQUIRE(nx > 0 && ny > 0 && nz > 0 && nu > 0 && nphi > 0 && nstars >= 0, "%s: bad sizes", name);
+  NV_REQUIRE(!smax_out || (f0 && f1 && f2 && f3), "%s: smax needs the density and momentum fields", name);
   IonArgs A;
   A.rfronts = rfronts; A.star_xyz = star_xyz; A.xi = xi; A.df_h = dfront_h; A.df_f = dfront_f;
-  A.rho = rho; A.mx = mx; A.my = my; A.mz = mz; A.smax = smax_out;
+  A.f0 = f0; A.f1 = f1; A.f2 = f2; A.f3 = f3; A.smax = smax_out;
   A.nu = nu; A.nphi = nphi; A.nstars = nstars; A.nx = nx; A.ny = ny; A.nz = nz;
   A.dx = dx; A.width = width; A.cn2 = cn2; A.ci2 = ci2;
   const long long tiles_jk = (long long)((nz + IK - 1) / IK) * ((ny + IJ - 1) / IJ);
   A.ci = (int)std::min<long long>(64, std::max<long long>(4, (long long)nx * tiles_jk / 2048));
   dim3 grid((nz + IK - 1) / IK, (ny + IJ - 1) / IJ, (nx + A.ci - 1) / A.ci);
-  if (smax_out)
-    hipLaunchKernelGGL(ionize_apply_kernel<true>, grid, dim3(IK, IJ), 0, as_stream(stream), A);
-  else
-    hipLaunchKernelGGL(ionize_apply_kernel<false>, grid, dim3(IK, IJ), 0, as_stream(stream), A);
-  return check_launch("nv_ionize_apply");
+  auto kern = !smax_out ? ionize_apply_kernel<false, false>
+                        : (vel ? ionize_apply_kernel<true, true> : ionize_apply_kernel<true, false>);
+  hipLaunchKernelGGL(kern, grid, dim3(IK, IJ), 0, as_stream(stream), A);
+  return check_launch(name);
+}
+
+extern "C" int nv_ionize_apply(const float* rfronts, int nu, int nphi, const float* star_xyz, int nstars, float dx,
+                               float width, float* xi, uint16_t* dfront_h, float* dfront_f, int nx, int ny, int nz,
+                               const float* rho, const float* mx, const float* my, const float* mz, float cn2,
+                               float ci2, float* smax_out, void* stream) {
+  return ionize_apply_launch("nv_ionize_apply", false, rfronts, nu, nphi, star_xyz, nstars, dx, width, xi, dfront_h,
+                             dfront_f, nx, ny, nz, rho, mx, my, mz, cn2, ci2, smax_out, stream);
+}
+
+extern "C" int nv_ionize_apply_h(const float* rfronts, int nu, int nphi, const float* star_xyz, int nstars, float dx,
+                                 float width, float* xi, uint16_t* dfront_h, float* dfront_f, int nx, int ny, int nz,
+                                 const float* rho, const uint16_t* vx, const uint16_t* vy, const uint16_t* vz,
+                                 float cn2, float ci2, float* smax_out, void* stream) {
+  return ionize_apply_launch("nv_ionize_apply_h", true, rfronts, nu, nphi, star_xyz, nstars, dx, width, xi, dfront_h,
+                             dfront_f, nx, ny, nz, rho, vx, vy, vz, cn2, ci2, smax_out, stream);
+}
+
+template <class T>
+static int cast_fuv_launch(const char* name, const T* rho, const float* xi, int nx, int ny, int nz, float sx,
+                           float sy, float sz, float kappa, float dr, int nsteps, float dx, float binr, float bx,
+                           float by, float bz, float* att, int nr, int nu, int nphi, void* stream) {
+  clear_error();
+  NV_REQUIRE(rho && xi && att, "%s: null pointer", name);
+  NV_REQUIRE(nx > 0 && ny > 0 && nz > 0 && nr > 0 && nu > 0 && nphi > 0, "%s: bad sizes", name);
+  NV_REQUIRE(dr > 0.0f && dx > 0.0f && binr > 0.0f, "%s: dr, dx, binr must be > 0", name);
+  NV_REQUIRE((long long)nx * ny * nz < (1LL << 32), "%s: grids above 2^32 cells are not supported", name);
+  const long long n = (long long)nu * nphi;
+  hipLaunchKernelGGL(cast_fuv_kernel<T>, dim3(blocks_for(n, 256)), dim3(256), 0, as_stream(stream), rho, xi, nx, ny,
+                     nz, sx, sy, sz, kappa, dr, nsteps, dx, binr, bx, by, bz, att, nr, nu, nphi);
+  return check_launch(name);
 }
 
 extern "C" int nv_cast_fuv(const float* rho, const float* xi, int nx, int ny, int nz, float sx, float sy, float sz,
                            float kappa, float dr, int nsteps, float dx, float binr, float bx, float by, float bz,
                            float* att, int nr, int nu, int nphi, void* stream) {
-  clear_error();
-  NV_REQUIRE(rho && xi && att, "nv_cast_fuv: null pointer");
-  NV_REQUIRE(nx > 0 && ny > 0 && nz > 0 && nr > 0 && nu > 0 && nphi > 0, "nv_cast_fuv: bad sizes");
-  NV_REQUIRE(dr > 0.0f && dx > 0.0f && binr > 0.0f, "nv_cast_fuv: dr, dx, binr must be > 0");
-  const long long n = (long long)nu * nphi;
-  hipLaunchKernelGGL(cast_fuv_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, as_stream(stream), rho, xi, nx, ny, nz,
-                     sx, sy, sz, kappa, dr, nsteps, dx, binr, bx, by, bz, att, nr, nu, nphi);
-  return check_launch("nv_cast_fuv");
+  return cast_fuv_launch("nv_cast_fuv", rho, xi, nx, ny, nz, sx, sy, sz, kappa, dr, nsteps, dx, binr, bx, by, bz, att,
+                         nr, nu, nphi, stream);
 }
 
 extern "C" int nv_apply_fuv(const float* att, int nr, int nu, int nphi, float sx, float sy, float sz, float lum,

@@ -14,6 +14,10 @@ two RK stages, clocks. Two fusions change no bits:
   - the CFL max of the next step is reduced in the epilogue of RK stage 2 (or of the ionize pass when it
     changes xi), so no separate max_speed pass reads the grid; the host still reads smax once per step,
     as the reference does (sim.py:1012), to compute dt in double.
+
+precision: "fp32" reproduces the reference's float32 step bit for bit; "fp32-fast" uses fused multiply-adds and
+hardware reciprocals / square roots in the RK stage; "fp16-vel" stores float32 density with binary16 velocity
+(58 instead of 88 bytes of state traffic per cell and step) and then mx, my, mz are computed on access.
 """
 import math
 import time
@@ -34,8 +38,13 @@ class Sim:
     profile = False
     dt_cap = None
 
+    PRECISIONS = ("fp32", "fp32-fast", "fp16-vel")
+
     def __init__(self, scenario="torus", res=None, params=None, rays=None, fuv_grid=None, ion_every=None,
-                 device="cuda:0", verbose=True, fast=False, setup=None, fused=True, seed=None):
+                 device="cuda:0", verbose=True, fast=False, setup=None, fused=True, seed=None, precision="fp32"):
+        if precision not in self.PRECISIONS:
+            raise ValueError(f"precision must be one of {self.PRECISIONS}")
+        self.precision = precision
         self.device = torch.device(device)
         self.fast = fast
         self.verbose = verbose
@@ -74,10 +83,47 @@ class Sim:
     def _p(t):
         return t.data_ptr() if t is not None else None
 
+    # ------------------------------------------------------------------ fields
+    @property
+    def vel_state(self):
+        return self.precision == "fp16-vel"
+
+    @property
+    def rho(self):
+        """density [nx, ny, nz] float32"""
+        return self.U[0]
+
+    def _mom(self, c):
+        return self.U[0] * self.U[c].float() if self.vel_state else self.U[c]
+
+    @property
+    def mx(self):
+        return self._mom(1)
+
+    @property
+    def my(self):
+        return self._mom(2)
+
+    @property
+    def mz(self):
+        return self._mom(3)
+
+    def set_state(self, rho, mx, my, mz):
+        """overwrite the conserved state from float32 tensors (timeline restores, tests)"""
+        if self.vel_state:
+            self.U[0].copy_(rho)
+            r = self.U[0].clamp_min(self.P["floor"])
+            for dst, m in zip(self.U[1:], (mx, my, mz)):
+                dst.copy_(m / r)
+        else:
+            for dst, src in zip(self.U, (rho, mx, my, mz)):
+                dst.copy_(src)
+        self.fields_modified()
+
     # ------------------------------------------------------------------ setup
     def reset(self, scenario=None, res=None, params=None, setup=None):
         t_setup = time.perf_counter()
-        for k in ("rho", "mx", "my", "mz", "xi", "dfront", "g0", "stage", "rfronts", "atts"):
+        for k in ("U", "xi", "dfront", "g0", "stage", "rfronts", "atts"):
             self.__dict__.pop(k, None)
         if setup is None:
             setup = S.make_setup(scenario or "torus", res, self._seed, params)
@@ -104,12 +150,21 @@ class Sim:
 
         dev = self.device
         f32 = dict(dtype=torch.float32, device=dev)
-        self.rho = torch.from_numpy(np.ascontiguousarray(setup.rho, dtype=np.float32)).to(dev)
+        rho = torch.from_numpy(np.ascontiguousarray(setup.rho, dtype=np.float32)).to(dev)
         z = lambda a: (torch.zeros(self.shape, **f32) if a is None
                        else torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).to(dev))
-        self.mx, self.my, self.mz = z(setup.mx), z(setup.my), z(setup.mz)
+        m = [z(setup.mx), z(setup.my), z(setup.mz)]
+        if self.vel_state:
+            # float32 rho, binary16 velocity (not momentum): the state of nv_hydro_stage_h
+            r = rho.clamp_min(P["floor"])
+            self.U = [rho] + [(c / r).half() for c in m]
+            del r, m
+            h16 = dict(dtype=torch.float16, device=dev)
+            self.stage = [torch.zeros(self.shape, **f32)] + [torch.zeros(self.shape, **h16) for _ in range(3)]
+        else:
+            self.U = [rho] + m
+            self.stage = torch.zeros((4,) + self.shape, **f32)
         self.xi = torch.zeros(self.shape, **f32)
-        self.stage = torch.zeros((4,) + self.shape, **f32)
         hdt = torch.float32 if P.get("fp32_fields") else torch.float16
         self.dfront = torch.full(self.shape, 1e3, dtype=hdt, device=dev)
         self.g0 = torch.zeros(self.shape, dtype=hdt, device=dev)
@@ -223,7 +278,8 @@ class Sim:
         tk = self._t0("cast_rays")
         for slot, i in enumerate(on):
             sx, sy, sz = self.stars[i][:3]
-            _lib.call("nv_cast_rays", self._p(self.rho), nx, ny, nz, _f(sx), _f(sy), _f(sz), _f(self.q_eff(i)),
+            _lib.call("nv_cast_rays", self._p(self.U[0]), nx, ny, nz,
+                      _f(sx), _f(sy), _f(sz), _f(self.q_eff(i)),
                       _f(self.dr), self.nsteps, _f(self.dx), _f(self.diag), _f(Lx), _f(Ly), _f(Lz),
                       self._p(self.rfronts[slot]), nu, nphi, strm)
         self._t1(tk)
@@ -235,14 +291,16 @@ class Sim:
                 self.star_xyz[:len(on)].copy_(torch.from_numpy(xyz))
             self.smax.zero_()
             tk = self._t0("ionize_apply")
-            _lib.call("nv_ionize_apply", self._p(self.rfronts), nu, nphi, self._p(self.star_xyz), len(on),
-                      _f(self.dx), _f(width), self._p(self.xi), self._p(self.dfront) if half else None,
-                      None if half else self._p(self.dfront), nx, ny, nz, self._p(self.rho), self._p(self.mx),
-                      self._p(self.my), self._p(self.mz), _f(self.cn2), _f(self.ci2), self._p(self.smax), strm)
+            _lib.call("nv_ionize_apply_h" if self.vel_state else "nv_ionize_apply", self._p(self.rfronts), nu,
+                      nphi, self._p(self.star_xyz), len(on), _f(self.dx), _f(width), self._p(self.xi),
+                      self._p(self.dfront) if half else None, None if half else self._p(self.dfront), nx, ny, nz,
+                      *[self._p(t) for t in self.U], _f(self.cn2), _f(self.ci2), self._p(self.smax), strm)
             self._t1(tk)
             self._smax_ok = True
         else:
             # the reference launch sequence, kernel by kernel (sim.py:930-943)
+            if self.vel_state:
+                raise ValueError("the unfused ionize sequence needs the float32 state (precision='fp32')")
             n = nx * ny * nz
             df = self.stage[1]
             _lib.call("nv_fill_f32", self._p(self.xi), _f(0.0), n, strm)
@@ -275,7 +333,8 @@ class Sim:
         Lx, Ly, Lz = self.bmax
         for slot, i in enumerate(on):
             sx, sy, sz = self.stars[i][:3]
-            _lib.call("nv_cast_fuv", self._p(self.rho), self._p(self.xi), nx, ny, nz, _f(sx), _f(sy), _f(sz),
+            _lib.call("nv_cast_fuv", self._p(self.U[0]), self._p(self.xi),
+                      nx, ny, nz, _f(sx), _f(sy), _f(sz),
                       _f(self.P["kappa_fuv"]), _f(self.dr), self.nsteps, _f(self.dx), _f(binr), _f(Lx), _f(Ly),
                       _f(Lz), self._p(self.atts[slot]), fnr, fnu, fnp, strm)
         xyz = np.array([self.stars[i][:3] for i in on], dtype=np.float32).reshape(-1, 3)
@@ -314,8 +373,12 @@ class Sim:
         if not self._smax_ok:
             nx, ny, nz = self.shape
             self.smax.zero_()
-            _lib.call("nv_max_speed", self._p(self.rho), self._p(self.mx), self._p(self.my), self._p(self.mz),
-                      self._p(self.xi), nx, ny, nz, _f(self.cn2), _f(self.ci2), self._p(self.smax), self._stream)
+            if self.vel_state:
+                _lib.call("nv_max_speed_h", *[self._p(t) for t in self.U[1:]], self._p(self.xi), nx, ny, nz,
+                          _f(self.cn2), _f(self.ci2), self._p(self.smax), self._stream)
+            else:
+                _lib.call("nv_max_speed", *[self._p(t) for t in self.U], self._p(self.xi), nx, ny, nz, _f(self.cn2),
+                          _f(self.ci2), self._p(self.smax), self._stream)
         smax = float(self.smax.item())
         self._smax_ok = False
         return self.P["cfl"] * self.dx / smax
@@ -328,18 +391,19 @@ class Sim:
         solver = S.SOLVERS[P.get("solver", "rusanov")]
         lim = S.LIMITERS[P.get("limiter", "minmod")] if solver == 1 else 0
         strm = self._stream
-        U = [self._p(t) for t in (self.rho, self.mx, self.my, self.mz)]
+        U = [self._p(t) for t in self.U]
         U1 = [self._p(self.stage[c]) for c in range(4)]
+        entry = {"fp32": "nv_hydro_stage", "fp32-fast": "nv_hydro_stage_fast",
+                 "fp16-vel": "nv_hydro_stage_h"}[self.precision]
         common = [nx, ny, nz, self._p(self.star_arr), nstars, _f(wind), self._p(self.jet_arr), _f(self.C["jet_cos"]),
                   _f(P["jet_r"]), _f(pulse), _f(dt), _f(self.dx)]
         tail = [_f(P["floor"]), _f(self.cn2), _f(self.ci2), _f(6.0), solver, lim]
         tk = self._t0("hydro_stage")
-        _lib.call("nv_hydro_stage", *U, self._p(self.xi), *U, *U1, *common, _f(0.0), _f(1.0), *tail, None, strm)
+        _lib.call(entry, *U, self._p(self.xi), *U, *U1, *common, _f(0.0), _f(1.0), *tail, None, strm)
         self._t1(tk)
         self.smax.zero_()
         tk = self._t0("hydro_stage")
-        _lib.call("nv_hydro_stage", *U1, self._p(self.xi), *U, *U, *common, _f(0.5), _f(0.5), *tail,
-                  self._p(self.smax), strm)
+        _lib.call(entry, *U1, self._p(self.xi), *U, *U, *common, _f(0.5), _f(0.5), *tail, self._p(self.smax), strm)
         self._t1(tk)
         self._smax_ok = True
 
@@ -405,9 +469,12 @@ class Sim:
 
     def render_scratch(self):
         """two (nx, ny, nz, 4) float16 views of the stage block, free between steps (sim.py:821-825)"""
+        if self.vel_state:
+            raise NotImplementedError("render_scratch needs the float32 stage block (precision='fp32')")
         flat = self.stage.view(-1).view(torch.float16)
         n = self.shape[0] * self.shape[1] * self.shape[2]
         return flat[:4 * n].view(self.shape + (4,)), flat[4 * n:8 * n].view(self.shape + (4,))
 
     def stats(self):
-        return dict(ion=float(self.xi.mean()), rho_max=float(self.rho.max()), rho_min=float(self.rho.min()))
+        r = self.U[0]
+        return dict(ion=float(self.xi.mean()), rho_max=float(r.max()), rho_min=float(r.min()))

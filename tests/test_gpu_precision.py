@@ -1,0 +1,160 @@
+"""The fast-arithmetic FP32 stage and the binary16-velocity form against the exact path and the FP64 oracle.
+
+Tolerances (the documented error bars of these forms):
+  fp32-fast  one stage: rel-L1 <= 2e-6 against the exact stage; after the 64^3 torus's 100 steps:
+             rel-L1 <= 1e-5 on rho and momentum, at most 1e-4 of front cells with |dxi| > 1e-3 (the
+             drop-in tolerance of the FP32 path).
+  fp16-vel   one stage from (float32 rho, binary16 v) inputs: rel-L1 <= 1e-6 on rho and <= 5e-4 on the
+             velocity against FP64; after 100 steps of the 64^3 torus: rel-L1 <= 1e-3 on rho against the
+             FP64 oracle (measured 1.4e-4 in an exact CPU emulation of the rounding); over the 512^3 wall's
+             300 steps: rel-L1 <= 1e-3 on rho against the exact FP32 path.
+"""
+import ctypes
+import math
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+F32 = np.float32
+f = ctypes.c_float
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def rel_l1(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.abs(a - b).sum() / max(np.abs(b).sum(), 1e-300))
+
+
+def front_disagreement(xi, xi_ref):
+    """fraction of front cells (0 < xi_ref < 1, or within 2 cells of one) with |xi - xi_ref| > 1e-3"""
+    from scipy import ndimage
+    front = (xi_ref > 0) & (xi_ref < 1)
+    front = ndimage.binary_dilation(front, iterations=2)
+    n = int(front.sum())
+    bad = int((np.abs(np.asarray(xi, np.float64) - xi_ref) > 1e-3)[front].sum())
+    return bad / max(n, 1), n
+
+
+@pytest.fixture(scope="module")
+def env():
+    import torch
+    from nebulaviewer_amd import _lib
+    _lib.load()
+    return torch, torch.device("cuda:0"), _lib
+
+
+def state(shape, seed):
+    rng = np.random.default_rng(seed)
+    rho = np.exp(rng.normal(0, 1.0, shape)).clip(0.01, 500).astype(F32)
+    v = [rng.normal(0, 0.5, shape).astype(F32) for _ in range(3)]
+    xi = rng.uniform(0, 1, shape).astype(F32)
+    return rho, v, xi
+
+
+def call_stage(env, entry, U, xi, U0, out, shape, a, b, smax=None):
+    torch, dev, L = env
+    z = torch.zeros((1, 4), dtype=torch.float32, device=dev)
+    L.call(entry, *[t.data_ptr() for t in U], xi.data_ptr(), *[t.data_ptr() for t in U0],
+           *[t.data_ptr() for t in out], *shape, z.data_ptr(), 0, f(0), z.data_ptr(), f(0.9), f(0.04), f(1),
+           f(1e-3), f(1 / shape[2]), f(a), f(b), f(2e-3), f(1 / 144), f(1), f(6), 1, 0,
+           smax.data_ptr() if smax is not None else None, torch.cuda.current_stream(dev).cuda_stream)
+
+
+def test_fast_stage_close_to_exact(env):
+    torch, dev, L = env
+    shape = (48, 64, 128)
+    say(f"[fast stage] {shape}")
+    rho, v, xi = state(shape, 1)
+    U = [torch.from_numpy(rho).to(dev)] + [torch.from_numpy(rho * c).to(dev) for c in v]
+    gx = torch.from_numpy(xi).to(dev)
+    oe = [torch.empty_like(U[0]) for _ in range(4)]
+    of = [torch.empty_like(U[0]) for _ in range(4)]
+    call_stage(env, "nv_hydro_stage", U, gx, U, oe, shape, 0.0, 1.0)
+    call_stage(env, "nv_hydro_stage_fast", U, gx, U, of, shape, 0.0, 1.0)
+    for c in range(4):
+        assert rel_l1(of[c].cpu().numpy(), oe[c].cpu().numpy()) <= 2e-6, c
+    assert not torch.equal(of[1], oe[1])    # it is a different arithmetic, not a copy of the exact path
+
+
+def test_half_velocity_stage_against_fp64(env, okern64):
+    torch, dev, L = env
+    shape = (40, 48, 64)
+    say(f"[fp16-vel stage] {shape}")
+    rho, v, xi = state(shape, 2)
+    h = [rho.astype(np.float32)] + [c.astype(np.float16) for c in v]
+    # the FP64 stage from the same (decoded) state
+    r64 = h[0].astype(np.float64)
+    U64 = [r64] + [r64 * c.astype(np.float64) for c in h[1:]]
+    out64 = [np.zeros(shape) for _ in range(4)]
+    z = np.zeros((1, 4))
+    okern64.hydro_stage(U64, xi.astype(np.float64), U64, out64, shape, z, 0, 0.0, z, 0.9, 0.04, 1.0, 1e-3,
+                        1 / shape[2], 0.0, 1.0, 2e-3, 1 / 144, 1.0, 6.0, 1, 0)
+    Uh = [torch.from_numpy(a).to(dev) for a in h]
+    out = [torch.empty_like(t) for t in Uh]
+    call_stage(env, "nv_hydro_stage_h", Uh, torch.from_numpy(xi).to(dev), Uh, out, shape, 0.0, 1.0)
+    assert out[0].dtype == torch.float32 and out[1].dtype == torch.float16
+    r = out[0].cpu().numpy()
+    assert rel_l1(r, out64[0]) <= 1e-6
+    vel = [out[c].float().cpu().numpy() for c in (1, 2, 3)]
+    for c in range(3):
+        assert rel_l1(vel[c], out64[c + 1] / out64[0]) <= 5e-4, c
+
+
+def run_torus(env, precision, steps, n=64):
+    from nebulaviewer_amd import scenarios as S
+    from nebulaviewer_amd.sim import Sim
+    say(f"[sim] torus {n}^3 {precision} {steps} steps")
+    sim = Sim(setup=S.make_setup("torus", n), verbose=False, precision=precision)
+    sim.step(steps)
+    return sim
+
+
+def test_fast_sim_within_dropin_tolerance(env):
+    exact = run_torus(env, "fp32", 100)
+    fast = run_torus(env, "fp32-fast", 100)
+    assert rel_l1(fast.rho.cpu().numpy(), exact.rho.cpu().numpy()) <= 1e-5
+    for name in ("mx", "my", "mz"):
+        assert rel_l1(getattr(fast, name).cpu().numpy(), getattr(exact, name).cpu().numpy()) <= 1e-5, name
+    frac, n = front_disagreement(fast.xi.cpu().numpy(), exact.xi.cpu().numpy().astype(np.float64))
+    assert frac <= 1e-4, (frac, n)
+
+
+def test_half_and_fp32_against_fp64_oracle(env):
+    from nebulaviewer_amd import scenarios as S
+    from oracle.oracle import OracleSim
+    n, steps = 64, 100
+    setup = S.make_setup("torus", n)
+    say("[oracle fp64] torus 64^3 100 steps")
+    ref = OracleSim(setup, S.constants(setup), S.ray_grid(n), S.fuv_grid(n), np.float64, threads=16)
+    ref.step(steps)
+    for prec, tol in (("fp32", 1e-5), ("fp16-vel", 1e-3)):
+        sim = run_torus(env, prec, steps)
+        assert abs(sim.t - ref.t) <= 1e-6 * ref.t if prec == "fp32" else abs(sim.t - ref.t) <= 1e-3 * ref.t
+        e = rel_l1(sim.rho.cpu().numpy(), ref.rho)
+        say(f"  {prec}: rel-L1 rho vs FP64 = {e:.3e}")
+        assert e <= tol, (prec, e)
+
+
+def test_half_velocity_wall_512_300_steps(env):
+    from nebulaviewer_amd import scenarios as S
+    from nebulaviewer_amd.sim import Sim
+    torch = env[0]
+    say("[fp16-vel vs fp32] wall 512^3, 300 steps")
+    setup = S.make_setup("wall", 512)
+    ref = Sim(setup=setup, verbose=False, precision="fp32")
+    ref.step(300)
+    r_ref = ref.rho.cpu().numpy()
+    xi_ref = ref.xi.cpu().numpy()
+    del ref
+    torch.cuda.empty_cache()
+    half = Sim(setup=setup, verbose=False, precision="fp16-vel")
+    half.step(300)
+    e = rel_l1(half.rho.cpu().numpy(), r_ref)
+    say(f"  rel-L1 rho fp16-vel vs fp32 after 300 steps = {e:.3e}")
+    assert e <= 1e-3
