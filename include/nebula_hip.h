@@ -85,6 +85,14 @@ int nv_cast_rays(const float* rho, int nx, int ny, int nz, float sx, float sy, f
                  int nsteps, float dx, float rcap, float bx, float by, float bz, float* rfront, int nu, int nphi,
                  void* stream);
 
+/* nv_cast_rays with the recombination column of long rays in FP64: a ray whose marched length exceeds 512 cells
+ * accumulates the column, tests col + inc > Q and interpolates the front radius in double (the radius rounded
+ * to float at the end); shorter rays give nv_cast_rays' result bit for bit. No reference counterpart: the
+ * reference sums in float (nebula3d.py:262-268), whose threshold test drifts on long rays. */
+int nv_cast_rays_wide(const float* rho, int nx, int ny, int nz, float sx, float sy, float sz, float Q, float dr,
+                      int nsteps, float dx, float rcap, float bx, float by, float bz, float* rfront, int nu,
+                      int nphi, void* stream);
+
 /* Per-cell front lookup: xi = max(xi, ramp), dfront = min(dfront, r - rf). Replaces nebula3d.apply_front
  * (nebula3d.py:281-307), launched per star by Sim.ionize (sim.py:938). */
 int nv_apply_front(const float* rfront, int nu, int nphi, float sx, float sy, float sz, float dx, float width,

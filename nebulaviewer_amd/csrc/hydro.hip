@@ -56,6 +56,7 @@ struct HydroArgs {
   int tiles_k, tiles_j, tiles, per_xcd;     // tile grid and the number of tiles each XCD takes
   float wind, jet_cos, jet_r, pulse, dt, dx, a, b, floor_, cn2, ci2, vmax;
   float vmax2_lo;                           // largest float <= vmax^2: |v|^2 <= vmax2_lo means no cap
+  float inv_dx;                             // 1/dx when dx is a power of two (exact), else 0
 };
 
 // ------------------------------------------------------------------------------------------ arithmetic
@@ -419,7 +420,10 @@ __global__ __launch_bounds__(NT, 4) void hydro_stage_kernel(HydroArgs A) {
       const float U[4] = {Uc.x, Uc.y, Uc.z, Uc.w};
       float dU[4];
 #pragma unroll
-      for (int c = 0; c < 4; ++c) dU[c] = AR::div(-div[c], rdx);
+      for (int c = 0; c < 4; ++c) {
+        // dx = 2^e: -div * 2^-e is the same real number as -div / dx, so one rounding gives the same float
+        dU[c] = A.inv_dx != 0.0f ? -div[c] * A.inv_dx : AR::div(-div[c], rdx);
+      }
       const float p0 = ((float)i + 0.5f) * dx, p1 = ((float)j + 0.5f) * dx, p2 = ((float)k + 0.5f) * dx;
       const float x_ion = xraw[0];
       if (A.wind > 0.0f && x_ion > 0.01f) {
@@ -602,6 +606,7 @@ static int launch_stage(Mode mode, const char* name, const void* f0, const void*
     if ((double)lo > v2) lo = nextafterf(lo, 0.0f);
     A.vmax2_lo = lo;
   }
+  A.inv_dx = pow2_reciprocal(dx);
   // march depth: long enough to amortise the 4-plane prologue, short enough to fill the chip
   const long long tiles_jk = (long long)((nz + TK - 1) / TK) * ((ny + TJ - 1) / TJ);
   const int ci = (int)std::min<long long>(64, std::max<long long>(4, (long long)nx * tiles_jk / 2048));

@@ -1,6 +1,7 @@
 // nv_common.h - shared device helpers and the host-side error plumbing of the C ABI.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <math.h>
 #include <stdint.h>
 #include <stdio.h>
 #include <string>
@@ -24,6 +25,15 @@ void clear_error();
 int check_launch(const char* what);
 
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+// 1/x if x is a power of two whose reciprocal is a normal float (then a / x and a * (1/x) round the same real
+// number, so they are equal bit for bit), else 0.
+inline float pow2_reciprocal(float x) {
+  int e = 0;
+  const float m = frexpf(x, &e);
+  if (!(x > 0.0f) || m != 0.5f || e < -124 || e > 126) return 0.0f;
+  return ldexpf(1.0f, 1 - e);
+}
 
 // ------------------------------------------------------------------------------------ device helpers
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }

@@ -54,24 +54,58 @@ __device__ __forceinline__ float ldf(const uint16_t* __restrict__ f, unsigned i)
   return (float)__builtin_bit_cast(_Float16, f[i]);
 }
 
-// trilinear sample at world p, 0 outside [-1/2, n-1/2] in cell units (sample_n, nebula3d.py:195-226), with the
-// division p/dx done through a shared reciprocal (the identical correctly rounded quotient, nv_divide.h) and
-// 32-bit cell indices (grids up to 2^32 cells); T = float or binary16 bits
-template <class T>
-__device__ __forceinline__ float sample_r(const T* __restrict__ f, int nx, int ny, int nz, float px, float py,
-                                          float pz, double rdx) {
-  const float gx = div_r(px, rdx) - 0.5f, gy = div_r(py, rdx) - 0.5f, gz = div_r(pz, rdx) - 0.5f;
+// Two z-neighbours (i, i+1) in one 8-byte load (4-byte aligned, which global loads allow).
+__device__ __forceinline__ float2 ld_pair(const float* __restrict__ f, unsigned i) {
+  typedef float f2u __attribute__((ext_vector_type(2), aligned(4)));
+  const f2u v = *reinterpret_cast<const f2u*>(f + i);
+  return make_float2(v.x, v.y);
+}
+
+// How a sampler turns a world coordinate into cell units: p / dx through the shared double reciprocal (the
+// identical correctly rounded quotient, nv_divide.h), or p * (1/dx) when dx is a power of two, which rounds the
+// same real number and so gives the same float.
+struct CellScale {
+  double rdx;
+  float inv;   // 1/dx if dx = 2^e, else 0
+  __device__ __forceinline__ float operator()(float p) const { return inv != 0.0f ? p * inv : div_r(p, rdx); }
+};
+
+// trilinear sample at world p, 0 outside [-1/2, n-1/2] in cell units (sample_n, nebula3d.py:195-226), with
+// 32-bit cell indices (grids up to 2^32 cells). PAIR (nz >= 2): the corners at ka = clamp(k0) and
+// kb = clamp(k0+1) come from one load of (kp, kp+1), kp = clamp(k0, 0, nz-2): k0 = -1 gives ka = kb = 0 (the
+// pair's first value), k0 = nz-1 gives ka = kb = nz-1 (its second), so every corner is the value the separate
+// clamped loads read.
+template <bool PAIR>
+__device__ __forceinline__ float sample_r(const float* __restrict__ f, int nx, int ny, int nz, float px, float py,
+                                          float pz, const CellScale& cs) {
+  const float gx = cs(px) - 0.5f, gy = cs(py) - 0.5f, gz = cs(pz) - 0.5f;
   if (gx < -0.5f || gy < -0.5f || gz < -0.5f) return 0.0f;
   if (gx > (float)nx - 0.5f || gy > (float)ny - 0.5f || gz > (float)nz - 0.5f) return 0.0f;
   const int i0 = (int)floorf(gx), j0 = (int)floorf(gy), k0 = (int)floorf(gz);
   const float fx = gx - (float)i0, fy = gy - (float)j0, fz = gz - (float)k0;
   const unsigned ia = clampi(i0, 0, nx - 1), ib = clampi(i0 + 1, 0, nx - 1);
   const unsigned ja = clampi(j0, 0, ny - 1), jb = clampi(j0 + 1, 0, ny - 1);
-  const unsigned ka = clampi(k0, 0, nz - 1), kb = clampi(k0 + 1, 0, nz - 1);
   const unsigned r00 = (ia * ny + ja) * nz, r10 = (ib * ny + ja) * nz;
   const unsigned r01 = (ia * ny + jb) * nz, r11 = (ib * ny + jb) * nz;
-  const float c000 = ldf(f, r00 + ka), c100 = ldf(f, r10 + ka), c010 = ldf(f, r01 + ka), c110 = ldf(f, r11 + ka);
-  const float c001 = ldf(f, r00 + kb), c101 = ldf(f, r10 + kb), c011 = ldf(f, r01 + kb), c111 = ldf(f, r11 + kb);
+  float c000, c100, c010, c110, c001, c101, c011, c111;
+  if constexpr (PAIR) {
+    const unsigned kp = clampi(k0, 0, nz - 2);
+    const bool a_second = k0 > nz - 2, b_first = k0 < 0;
+    const float2 p00 = ld_pair(f, r00 + kp), p10 = ld_pair(f, r10 + kp);
+    const float2 p01 = ld_pair(f, r01 + kp), p11 = ld_pair(f, r11 + kp);
+    c000 = a_second ? p00.y : p00.x;
+    c100 = a_second ? p10.y : p10.x;
+    c010 = a_second ? p01.y : p01.x;
+    c110 = a_second ? p11.y : p11.x;
+    c001 = b_first ? p00.x : p00.y;
+    c101 = b_first ? p10.x : p10.y;
+    c011 = b_first ? p01.x : p01.y;
+    c111 = b_first ? p11.x : p11.y;
+  } else {
+    const unsigned ka = clampi(k0, 0, nz - 1), kb = clampi(k0 + 1, 0, nz - 1);
+    c000 = f[r00 + ka]; c100 = f[r10 + ka]; c010 = f[r01 + ka]; c110 = f[r11 + ka];
+    c001 = f[r00 + kb]; c101 = f[r10 + kb]; c011 = f[r01 + kb]; c111 = f[r11 + kb];
+  }
   const float c00 = c000 + (c100 - c000) * fx;
   const float c10 = c010 + (c110 - c010) * fx;
   const float c01 = c001 + (c101 - c001) * fx;
@@ -94,11 +128,14 @@ __device__ __forceinline__ bool ray_of_thread(long long t, int nu, int nphi, int
 
 // cast_rays (nebula3d.py:247-270). Two samples per iteration: the corner loads of sample n+1 are in flight
 // while sample n is tested, and the column is still accumulated sample by sample in the reference order.
-template <class T>
-__global__ __launch_bounds__(256) void cast_rays_kernel(const T* __restrict__ rho, int nx, int ny, int nz,
+// WIDE: a ray whose marched length (n1 - n0) dr exceeds 512 cells keeps its column in FP64 (the test
+// col + inc > Q and the interpolation of rf in double); shorter rays are unchanged.
+template <bool PAIR, bool WIDE>
+__global__ __launch_bounds__(256) void cast_rays_kernel(const float* __restrict__ rho, int nx, int ny, int nz,
                                                         float sx, float sy, float sz, float Q, float dr,
-                                                        int nsteps, float dx, float rcap, float bx, float by,
-                                                        float bz, float* __restrict__ rfront, int nu, int nphi) {
+                                                        int nsteps, CellScale cs, float dx, float rcap, float bx,
+                                                        float by, float bz, float* __restrict__ rfront, int nu,
+                                                        int nphi) {
   int iu, ip;
   if (!ray_of_thread((long long)blockIdx.x * blockDim.x + threadIdx.x, nu, nphi, &iu, &ip)) return;
   float d[3];
@@ -108,26 +145,34 @@ __global__ __launch_bounds__(256) void cast_rays_kernel(const T* __restrict__ rh
   box_range(st, d, bm, &t0, &t1);
   const int n0 = (int)(t0 / dr);
   const int n1 = min(nsteps, (int)(t1 / dr) + 2);
-  const double rdx = recip_d(dx);
+  const bool wide = WIDE && (float)(n1 - n0) * dr > 512.0f * dx;
   float col = 0.0f, rf = rcap;
+  double col64 = 0.0;
+  // one sample's threshold test; true when the front is found (rf set)
+  auto step = [&](float r, float inc) -> bool {
+    if (WIDE && wide) {
+      if (col64 + (double)inc > (double)Q) {
+        rf = (float)(((double)r - 0.5 * (double)dr) + (double)dr * ((double)Q - col64) / (double)fmaxf(inc, 1e-20f));
+        return true;
+      }
+      col64 = col64 + (double)inc;
+      return false;
+    }
+    if (col + inc > Q) {
+      rf = r - 0.5f * dr + dr * (Q - col) / fmaxf(inc, 1e-20f);
+      return true;
+    }
+    col = col + inc;
+    return false;
+  };
   for (int n = n0; n < n1; n += 2) {
     const float ra = ((float)n + 0.5f) * dr;
     const float rb = ((float)(n + 1) + 0.5f) * dr;
-    const float na = sample_r(rho, nx, ny, nz, sx + d[0] * ra, sy + d[1] * ra, sz + d[2] * ra, rdx);
-    const float nb = sample_r(rho, nx, ny, nz, sx + d[0] * rb, sy + d[1] * rb, sz + d[2] * rb, rdx);
-    const float ia = na * na * ra * ra * dr;
-    if (col + ia > Q) {
-      rf = ra - 0.5f * dr + dr * (Q - col) / fmaxf(ia, 1e-20f);
-      break;
-    }
-    col = col + ia;
+    const float na = sample_r<PAIR>(rho, nx, ny, nz, sx + d[0] * ra, sy + d[1] * ra, sz + d[2] * ra, cs);
+    const float nb = sample_r<PAIR>(rho, nx, ny, nz, sx + d[0] * rb, sy + d[1] * rb, sz + d[2] * rb, cs);
+    if (step(ra, na * na * ra * ra * dr)) break;
     if (n + 1 >= n1) break;
-    const float ib = nb * nb * rb * rb * dr;
-    if (col + ib > Q) {
-      rf = rb - 0.5f * dr + dr * (Q - col) / fmaxf(ib, 1e-20f);
-      break;
-    }
-    col = col + ib;
+    if (step(rb, nb * nb * rb * rb * dr)) break;
   }
   rfront[(long long)iu * nphi + ip] = rf;
 }
@@ -292,10 +337,10 @@ __global__ __launch_bounds__(INT_) void ionize_apply_kernel(IonArgs A) {
 
 // ---------------------------------------------------------------------------------------- FUV
 // cast_fuv (nebula3d.py:320-352)
-template <class T>
-__global__ __launch_bounds__(256) void cast_fuv_kernel(const T* __restrict__ rho, const float* __restrict__ xi,
+template <bool PAIR>
+__global__ __launch_bounds__(256) void cast_fuv_kernel(const float* __restrict__ rho, const float* __restrict__ xi,
                                                        int nx, int ny, int nz, float sx, float sy, float sz,
-                                                       float kappa, float dr, int nsteps, float dx, float binr,
+                                                       float kappa, float dr, int nsteps, CellScale cs, float binr,
                                                        float bx, float by, float bz, float* __restrict__ att, int nr,
                                                        int nu, int nphi) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -311,12 +356,11 @@ __global__ __launch_bounds__(256) void cast_fuv_kernel(const T* __restrict__ rho
   const int n1 = min(nsteps, (int)(t1 / dr) + 2);
   float tau = 0.0f;
   int ib = 0;
-  const double rdx = recip_d(dx);
   for (int n = n0; n < n1; ++n) {
     const float r = ((float)n + 0.5f) * dr;
     const float qx = sx + d[0] * r, qy = sy + d[1] * r, qz = sz + d[2] * r;
-    const float n_tot = sample_r(rho, nx, ny, nz, qx, qy, qz, rdx);
-    const float x_i = sample_r(xi, nx, ny, nz, qx, qy, qz, rdx);
+    const float n_tot = sample_r<PAIR>(rho, nx, ny, nz, qx, qy, qz, cs);
+    const float x_i = sample_r<PAIR>(xi, nx, ny, nz, qx, qy, qz, cs);
     tau = tau + kappa * n_tot * (1.0f - 0.92f * x_i) * dr;
     const int nb = (int)(r / binr);
     if (nb > ib) {
@@ -399,26 +443,43 @@ inline int blocks_for(long long n, int bs) { return (int)((n + bs - 1) / bs); }
 
 using namespace nv;
 
-template <class T>
-static int cast_rays_launch(const char* name, const T* rho, int nx, int ny, int nz, float sx, float sy, float sz,
-                            float Q, float dr, int nsteps, float dx, float rcap, float bx, float by, float bz,
-                            float* rfront, int nu, int nphi, void* stream) {
+static CellScale cell_scale(float dx) {
+  CellScale cs;
+  cs.rdx = 1.0 / (double)dx;   // correctly rounded: |rdx dx - 1| <= 2^-53, inside the bound div_r needs
+  cs.inv = pow2_reciprocal(dx);
+  return cs;
+}
+
+static int cast_rays_launch(const char* name, bool wide, const float* rho, int nx, int ny, int nz, float sx,
+                            float sy, float sz, float Q, float dr, int nsteps, float dx, float rcap, float bx,
+                            float by, float bz, float* rfront, int nu, int nphi, void* stream) {
   clear_error();
   NV_REQUIRE(rho && rfront, "%s: null pointer", name);
   NV_REQUIRE(nx > 0 && ny > 0 && nz > 0 && nu > 0 && nphi > 0, "%s: bad sizes", name);
   NV_REQUIRE(dr > 0.0f && dx > 0.0f, "%s: dr and dx must be > 0", name);
   NV_REQUIRE((long long)nx * ny * nz < (1LL << 32), "%s: grids above 2^32 cells are not supported", name);
   const long long n = (long long)((nu + 7) / 8) * ((nphi + 7) / 8) * 64;
-  hipLaunchKernelGGL(cast_rays_kernel<T>, dim3(blocks_for(n, 256)), dim3(256), 0, as_stream(stream), rho, nx, ny, nz,
-                     sx, sy, sz, Q, dr, nsteps, dx, rcap, bx, by, bz, rfront, nu, nphi);
+  const CellScale cs = cell_scale(dx);
+  const bool pair = nz >= 2;
+  auto kern = pair ? (wide ? cast_rays_kernel<true, true> : cast_rays_kernel<true, false>)
+                   : (wide ? cast_rays_kernel<false, true> : cast_rays_kernel<false, false>);
+  hipLaunchKernelGGL(kern, dim3(blocks_for(n, 256)), dim3(256), 0, as_stream(stream), rho, nx, ny, nz, sx, sy, sz, Q,
+                     dr, nsteps, cs, dx, rcap, bx, by, bz, rfront, nu, nphi);
   return check_launch(name);
 }
 
 extern "C" int nv_cast_rays(const float* rho, int nx, int ny, int nz, float sx, float sy, float sz, float Q, float dr,
                             int nsteps, float dx, float rcap, float bx, float by, float bz, float* rfront, int nu,
                             int nphi, void* stream) {
-  return cast_rays_launch("nv_cast_rays", rho, nx, ny, nz, sx, sy, sz, Q, dr, nsteps, dx, rcap, bx, by, bz, rfront,
-                          nu, nphi, stream);
+  return cast_rays_launch("nv_cast_rays", false, rho, nx, ny, nz, sx, sy, sz, Q, dr, nsteps, dx, rcap, bx, by, bz,
+                          rfront, nu, nphi, stream);
+}
+
+extern "C" int nv_cast_rays_wide(const float* rho, int nx, int ny, int nz, float sx, float sy, float sz, float Q,
+                                 float dr, int nsteps, float dx, float rcap, float bx, float by, float bz,
+                                 float* rfront, int nu, int nphi, void* stream) {
+  return cast_rays_launch("nv_cast_rays_wide", true, rho, nx, ny, nz, sx, sy, sz, Q, dr, nsteps, dx, rcap, bx, by,
+                          bz, rfront, nu, nphi, stream);
 }
 
 extern "C" int nv_apply_front(const float* rfront, int nu, int nphi, float sx, float sy, float sz, float dx,
@@ -499,26 +560,20 @@ extern "C" int nv_ionize_apply_h(const float* rfronts, int nu, int nphi, const f
                              dfront_f, nx, ny, nz, rho, vx, vy, vz, cn2, ci2, smax_out, stream);
 }
 
-template <class T>
-static int cast_fuv_launch(const char* name, const T* rho, const float* xi, int nx, int ny, int nz, float sx,
-                           float sy, float sz, float kappa, float dr, int nsteps, float dx, float binr, float bx,
-                           float by, float bz, float* att, int nr, int nu, int nphi, void* stream) {
-  clear_error();
-  NV_REQUIRE(rho && xi && att, "%s: null pointer", name);
-  NV_REQUIRE(nx > 0 && ny > 0 && nz > 0 && nr > 0 && nu > 0 && nphi > 0, "%s: bad sizes", name);
-  NV_REQUIRE(dr > 0.0f && dx > 0.0f && binr > 0.0f, "%s: dr, dx, binr must be > 0", name);
-  NV_REQUIRE((long long)nx * ny * nz < (1LL << 32), "%s: grids above 2^32 cells are not supported", name);
-  const long long n = (long long)nu * nphi;
-  hipLaunchKernelGGL(cast_fuv_kernel<T>, dim3(blocks_for(n, 256)), dim3(256), 0, as_stream(stream), rho, xi, nx, ny,
-                     nz, sx, sy, sz, kappa, dr, nsteps, dx, binr, bx, by, bz, att, nr, nu, nphi);
-  return check_launch(name);
-}
-
 extern "C" int nv_cast_fuv(const float* rho, const float* xi, int nx, int ny, int nz, float sx, float sy, float sz,
                            float kappa, float dr, int nsteps, float dx, float binr, float bx, float by, float bz,
                            float* att, int nr, int nu, int nphi, void* stream) {
-  return cast_fuv_launch("nv_cast_fuv", rho, xi, nx, ny, nz, sx, sy, sz, kappa, dr, nsteps, dx, binr, bx, by, bz, att,
-                         nr, nu, nphi, stream);
+  clear_error();
+  NV_REQUIRE(rho && xi && att, "nv_cast_fuv: null pointer");
+  NV_REQUIRE(nx > 0 && ny > 0 && nz > 0 && nr > 0 && nu > 0 && nphi > 0, "nv_cast_fuv: bad sizes");
+  NV_REQUIRE(dr > 0.0f && dx > 0.0f && binr > 0.0f, "nv_cast_fuv: dr, dx, binr must be > 0");
+  NV_REQUIRE((long long)nx * ny * nz < (1LL << 32), "nv_cast_fuv: grids above 2^32 cells are not supported");
+  const long long n = (long long)nu * nphi;
+  const CellScale cs = cell_scale(dx);
+  auto kern = nz >= 2 ? cast_fuv_kernel<true> : cast_fuv_kernel<false>;
+  hipLaunchKernelGGL(kern, dim3(blocks_for(n, 256)), dim3(256), 0, as_stream(stream), rho, xi, nx, ny, nz, sx, sy, sz,
+                     kappa, dr, nsteps, cs, binr, bx, by, bz, att, nr, nu, nphi);
+  return check_launch("nv_cast_fuv");
 }
 
 extern "C" int nv_apply_fuv(const float* att, int nr, int nu, int nphi, float sx, float sy, float sz, float lum,

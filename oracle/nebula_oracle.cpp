@@ -326,6 +326,46 @@ void cast_rays(const R* rho, int nx, int ny, int nz, const R star[3], R Q, R dr,
     }
 }
 
+// cast_rays with the recombination column of long rays in FP64: a float ray whose marched length (n1 - n0) dr
+// exceeds 512 cells accumulates col, tests col + inc > Q and interpolates rf in double (rf rounded to float at
+// the end); shorter rays follow cast_rays exactly. The per-sample increment stays the float n^2 r^2 dr.
+void cast_rays_wide(const float* rho, int nx, int ny, int nz, const float star[3], float Q, float dr, int nsteps,
+                    float dx, float rcap, const float bmax[3], float* rfront, int nu, int nphi, int nthreads) {
+  Grid g{nx, ny, nz};
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 4)
+  for (int iu = 0; iu < nu; ++iu)
+    for (int ip = 0; ip < nphi; ++ip) {
+      float d[3];
+      ray_dir(iu, ip, nu, nphi, d);
+      float col = 0.0f, rf = rcap, t0, t1;
+      double col64 = 0.0;
+      box_range(star, d, bmax, &t0, &t1);
+      int n0 = (int)(t0 / dr);
+      int n1 = std::min(nsteps, (int)(t1 / dr) + 2);
+      const bool wide = (float)(n1 - n0) * dr > 512.0f * dx;
+      for (int n = n0; n < n1; ++n) {
+        float r = ((float)n + 0.5f) * dr;
+        float nv = sample_n(rho, g, star[0] + d[0] * r, star[1] + d[1] * r, star[2] + d[2] * r, dx);
+        float inc = nv * nv * r * r * dr;
+        if (wide) {
+          if (col64 + (double)inc > (double)Q) {
+            rf = (float)(((double)r - 0.5 * (double)dr) +
+                         (double)dr * ((double)Q - col64) / (double)rmax(inc, 1e-20f));
+            break;
+          }
+          col64 = col64 + (double)inc;
+        } else {
+          if (col + inc > Q) {
+            rf = r - 0.5f * dr + dr * (Q - col) / rmax(inc, 1e-20f);
+            break;
+          }
+          col = col + inc;
+        }
+      }
+      rfront[(size_t)iu * nphi + ip] = rf;
+    }
+}
+
 // rf_at (nebula3d.py:273-278): u clamped, phi wrapped
 template <typename R> inline R rf_at(const R* rfront, int nu, int nphi, int iu, int ip) {
   int ipw = ((ip % nphi) + nphi) % nphi;
@@ -550,6 +590,13 @@ double or_max_speed_f64(const double* rho, const double* mx, const double* my, c
 
 OR_RAYS(float, f32)
 OR_RAYS(double, f64)
+
+void or_cast_rays_wide_f32(const float* rho, int nx, int ny, int nz, float sx, float sy, float sz, float Q, float dr,
+                           int nsteps, float dx, float rcap, float bx, float by, float bz, float* rfront, int nu,
+                           int nphi, int nthreads) {
+  const float st[3] = {sx, sy, sz}, bm[3] = {bx, by, bz};
+  cast_rays_wide(rho, nx, ny, nz, st, Q, dr, nsteps, dx, rcap, bm, rfront, nu, nphi, nthreads);
+}
 
 void or_to_half(const float* src, uint16_t* dst, long long n, int nthreads) {
 #pragma omp parallel for num_threads(nthreads) schedule(static)

@@ -54,6 +54,7 @@ def load():
             getattr(_lib, f"or_face_flux_{s}").argtypes = [P, R, R, I, R, I, I, P]
             getattr(_lib, f"or_slope_{s}").argtypes = [R, R, I]
             getattr(_lib, f"or_slope_{s}").restype = R
+        _lib.or_cast_rays_wide_f32.argtypes = _lib.or_cast_rays_f32.argtypes
         _lib.or_to_half.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_longlong, ctypes.c_int]
         for fn in ("or_sinf_det", "or_cosf_det", "or_expf_det"):
             getattr(_lib, fn).argtypes = [ctypes.c_float]
@@ -93,9 +94,12 @@ class Kernels:
         return float(self.fn("max_speed")(_ptr(rho), _ptr(mx), _ptr(my), _ptr(mz), _ptr(xi), rho.size,
                                           self.R(cn2), self.R(ci2), self.threads))
 
-    def cast_rays(self, rho, star, Q, dr, nsteps, dx, rcap, bmax, rfront):
+    def cast_rays(self, rho, star, Q, dr, nsteps, dx, rcap, bmax, rfront, wide=False):
+        """wide: FP64 recombination column on rays longer than 512 cells (float kernels only)"""
         R = self.R
-        self.fn("cast_rays")(_ptr(rho), *rho.shape, R(star[0]), R(star[1]), R(star[2]), R(Q), R(dr), nsteps, R(dx),
+        if wide and self.s != "f32":
+            raise ValueError("the FP64 long-ray column is a variant of the float kernel")
+        self.fn("cast_rays_wide" if wide else "cast_rays")(_ptr(rho), *rho.shape, R(star[0]), R(star[1]), R(star[2]), R(Q), R(dr), nsteps, R(dx),
                              R(rcap), R(bmax[0]), R(bmax[1]), R(bmax[2]), _ptr(rfront), *rfront.shape, self.threads)
 
     def apply_front(self, rfront, star, dx, width, xi, df):

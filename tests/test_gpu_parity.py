@@ -168,6 +168,49 @@ def test_cast_rays_bitwise(env, okern32, star, Q):
     np.testing.assert_array_equal(N(g), ref)
 
 
+@pytest.mark.parametrize("shape,dx", [((32, 16, 64), 1 / 64), ((20, 24, 1), 1 / 24), ((20, 24, 1), 1 / 32),
+                                      ((9, 7, 2), 0.1)])
+def test_cast_rays_sampler_forms_bitwise(env, okern32, shape, dx):
+    """power-of-two dx (multiply by 1/dx), general dx (shared reciprocal), nz = 1 (no paired z loads), nz = 2"""
+    say(f"[cast_rays forms] shape={shape} dx={dx}")
+    torch, dev, L = env
+    rho = np.random.default_rng(31).lognormal(0, 1.0, shape).astype(F32)
+    Lb = tuple(n * dx for n in shape)
+    diag = math.sqrt(sum(v * v for v in Lb)) + 0.6
+    dr = 0.5 * dx
+    nsteps = int(diag / dr) + 1
+    star = (0.37 * Lb[0], 0.52 * Lb[1], 0.45 * Lb[2])
+    nu, nphi = 24, 40
+    grho = T(env, rho)
+    for Q in (1e-4, 3e-3, 1e3):
+        ref = np.zeros((nu, nphi), F32)
+        okern32.cast_rays(rho, star, Q, dr, nsteps, dx, diag, Lb, ref)
+        g = torch.zeros((nu, nphi), dtype=torch.float32, device=dev)
+        L.call("nv_cast_rays", grho.data_ptr(), *shape, *(f(v) for v in star), f(Q), f(dr), nsteps, f(dx),
+               f(diag), *(f(v) for v in Lb), g.data_ptr(), nu, nphi, stream(env))
+        np.testing.assert_array_equal(N(g), ref)
+
+
+def test_cast_rays_wide_bitwise(env, okern32, okern64):
+    """The FP64 long-ray column against its oracle on the stalled-column case, and the float form beside it."""
+    import long_ray_case as LR
+    say("[cast_rays_wide] stalled-column case 640x24x24")
+    torch, dev, L = env
+    Q = LR.find_q(okern64)
+    rho, star, dr, nsteps, dx, diag, Lb = LR.geometry()
+    grho = T(env, rho)
+    for entry, wide in (("nv_cast_rays_wide", True), ("nv_cast_rays", False)):
+        ref = np.zeros(LR.RAYS, F32)
+        okern32.cast_rays(rho, star, Q, dr, nsteps, dx, diag, Lb, ref, wide=wide)
+        g = torch.zeros(LR.RAYS, dtype=torch.float32, device=dev)
+        L.call(entry, grho.data_ptr(), *LR.SHAPE, *(f(v) for v in star), f(Q), f(dr), nsteps, f(dx), f(diag),
+               *(f(v) for v in Lb), g.data_ptr(), *LR.RAYS, stream(env))
+        np.testing.assert_array_equal(N(g), ref)
+        if wide:
+            wide_rf = N(g)
+    assert abs(float(wide_rf[LR.RAY]) - 2 * LR.X1 * dx) < 12 * dx
+
+
 def test_apply_front_smooth_half_fill_bitwise(env, okern32):
     say("[apply_front/smooth/to_half/fill]")
     torch, dev, L = env
@@ -248,10 +291,10 @@ def test_ionize_apply_fused_bitwise(env, okern32, nst, half, shape):
     assert float(smax.item()) == smax_ref
 
 
-def test_fuv_bitwise(env, okern32):
-    say("[fuv] cast_fuv / apply_fuv / fuv_apply")
+@pytest.mark.parametrize("shape", [(40, 36, 44), (32, 16, 64)])
+def test_fuv_bitwise(env, okern32, shape):
+    say(f"[fuv] cast_fuv / apply_fuv / fuv_apply {shape}")
     torch, dev, L = env
-    shape = (40, 36, 44)
     rho, dx, Lb, diag = ray_case(shape, seed=14)
     xi = np.random.default_rng(15).uniform(0, 1, shape).astype(F32)
     dr = 0.5 * dx
