@@ -14,6 +14,9 @@ U + xi and writes U1: 36 B; stage 2 reads U1 + xi + U^n and writes U^n+1: 52 B),
 on average; achieved = 44 B x cells / (mean launch time from HIP events on the launch stream).
 cpu_baseline: the C++ oracle (oracle/, a restatement of the reference kernels; the reference's Warp CPU device
 cannot run here) on the same 512^3 grid: one ionization and one hydro step on all host cores.
+variants: the same workload in the fast-arithmetic and binary16-velocity forms, and the uneven batch (64
+independent members of 48^3..160^3, scenarios.batch_setups) stepped by SimBatch with HIP-graph replay, with its
+throughput relative to the single 512^3 grid (full step and hydro only).
 """
 import argparse
 import json
@@ -92,6 +95,8 @@ def main():
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp32-fast", "fp16-vel"],
                     help="the form whose throughput is `value` (fp32 = the bit-exact drop-in)")
     ap.add_argument("--no-variants", action="store_true", help="do not also time the other two forms")
+    ap.add_argument("--batch", type=int, default=64,
+                    help="members of the uneven-batch variant (0: skip it; timed with the variants)")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="counter calibration: run a plain device-to-device copy of this many bytes 3 times, exit")
     a = ap.parse_args()
@@ -140,12 +145,51 @@ def main():
         torch.cuda.empty_cache()
         return r
 
+    def measure_batch(count):
+        from nebulaviewer_amd.batch import SimBatch
+        t0 = time.perf_counter()
+        setups = S.batch_setups(count)
+        log(f"[bench] batch of {count}: initial conditions in {time.perf_counter() - t0:.1f} s")
+        b = SimBatch(setups, precision="fp32", graph=True)
+        cells = b.cells
+        b.step(a.warmup)                       # captures the two kinds of step
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        b.step(a.steps)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        timer = KernelTimer()
+        b.timer = timer                        # per-kernel events (eager launches, no graph)
+        b.step(2 * b.ion_every)
+        torch.cuda.synchronize()
+        b.timer = None
+        ks = timer.summary()
+        hyd_s = ks["hydro_stage"]["avg_ms"] / 1e3
+        r = dict(members=count, cells=cells, stars=sum(len(m.stars) for m in b.members),
+                 sizes=sorted({m.shape[0] for m in b.members}), value=cells * a.steps / wall,
+                 ms_per_step=wall / a.steps * 1e3, hydro_launch_ms=hyd_s * 1e3,
+                 hydro_only_cell_updates_per_s=cells / (2 * hyd_s),
+                 hydro_achieved_GBps=BYTES_PER_CELL_STEP["fp32"] / 2 * cells / hyd_s / 1e9,
+                 kernels_ms={k: round(v["avg_ms"], 4) for k, v in ks.items()}, graph_replay=True)
+        log(f"[bench] batch: {r['value'] / 1e9:.2f} G cell-updates/s ({cells / 1e6:.1f} M cells, {r['stars']} stars), "
+            f"{r['ms_per_step']:.3f} ms/step, hydro {hyd_s * 1e3:.3f} ms/launch; kernels {r['kernels_ms']}")
+        del b
+        torch.cuda.empty_cache()
+        return r
+
     main_r = measure(a.precision)
     variants = {}
     if not a.no_variants:
         for p in ("fp32", "fp32-fast", "fp16-vel"):
             if p != a.precision:
                 variants[p] = measure(p)
+        if a.batch > 0:
+            br = measure_batch(a.batch)
+            ref = main_r if a.precision == "fp32" else variants["fp32"]
+            br["vs_single_grid"] = {"step": br["value"] / ref["value"],
+                                    "hydro_only": br["hydro_only_cell_updates_per_s"] /
+                                    ref["hydro_only_cell_updates_per_s"]}
+            variants["batch"] = br
     cells = math.prod(setup.shape)
     value = main_r["value"]
     achieved = main_r["hydro_achieved_GBps"]

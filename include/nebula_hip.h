@@ -30,6 +30,58 @@ extern "C" {
 #define NV_LIM_VANLEER 1
 #define NV_LIM_MC 2
 
+#define NV_PREC_F32 0      /* nv_hydro_stage */
+#define NV_PREC_F32_FAST 1 /* nv_hydro_stage_fast */
+#define NV_PREC_F16V 2     /* nv_hydro_stage_h (float32 rho, binary16 velocity) */
+
+#define NV_TILE_AUTO 0     /* the library's choice from the z extent */
+#define NV_TILE_64x8 1     /* 64 z x 8 y cells per workgroup (one wavefront per y row) */
+#define NV_TILE_32x16 2    /* 32 z x 16 y cells per workgroup (one wavefront per two rows) */
+
+/* One grid of a batched RK stage: the arguments of nv_hydro_stage, with the four fields of U, U0 and the output
+ * as arrays (float32, or float32 rho + binary16 velocity for NV_PREC_F16V). dt_dev: if non-null, the stage reads
+ * its dt from this device float when it runs (nv_cfl_dt writes it), and dt is ignored. */
+typedef struct NvStageDesc {
+  const void* u[4];
+  const float* xi;
+  const void* u0[4];
+  void* out[4];
+  const float* stars;
+  const float* jets;
+  float* smax_out;
+  const float* dt_dev;
+  int nx, ny, nz, nstars;
+  float wind, jet_cos, jet_r, pulse, dt, dx, a, b, floor_, cn2, ci2, vmax;
+} NvStageDesc;
+
+/* One ray-cast job of a batch: the arguments of nv_cast_rays (wide != 0: those of nv_cast_rays_wide). */
+typedef struct NvRayDesc {
+  const float* rho;
+  int nx, ny, nz;
+  float sx, sy, sz, Q, dr;
+  int nsteps;
+  float dx, rcap, bx, by, bz;
+  float* rfront;
+  int nu, nphi, wide;
+} NvRayDesc;
+
+/* One grid of a batched ionize pass: the arguments of nv_ionize_apply (state = rho, mx, my, mz, or rho, vx, vy, vz
+ * binary16 for nv_ionize_apply_h). */
+typedef struct NvIonDesc {
+  const float* rfronts;
+  int nu, nphi;
+  const float* star_xyz;
+  int nstars;
+  float dx, width;
+  float* xi;
+  uint16_t* dfront_h;
+  float* dfront_f;
+  int nx, ny, nz;
+  const void* state[4];
+  float cn2, ci2;
+  float* smax_out;
+} NvIonDesc;
+
 /* Message of the last failed call on this thread ("" if none). */
 const char* nv_last_error(void);
 /* ABI version of this library (bumped on any signature change). */
@@ -70,6 +122,34 @@ int nv_hydro_stage_h(const float* rho, const uint16_t* vx, const uint16_t* vy, c
                      float jet_r, float pulse, float dt, float dx, float a, float b, float floor_, float cn2,
                      float ci2, float vmax, int solver, int lim, float* smax_out, void* stream);
 
+/* nv_hydro_stage (or _fast / _h, by precision) from a descriptor, with an explicit workgroup tile shape
+ * (NV_TILE_*): the shape changes the work decomposition only, never a bit of the result. The entry point an
+ * autotuner uses; nv_hydro_stage and its variants take NV_TILE_AUTO. Replaces the same launch as nv_hydro_stage. */
+int nv_hydro_stage_ex(const NvStageDesc* d, int precision, int solver, int lim, int tile, void* stream);
+
+/* Batched RK stage: many independent grids (uneven sizes) advance in one launch over the concatenated tile
+ * list. No reference counterpart (the reference steps one grid per Sim, sim.py:1014-1028); each member's result
+ * is bit-identical to nv_hydro_stage (or _fast / _h, by precision) called on that member alone.
+ * Two steps, so that a launch needs nothing from the host but a pointer (hipGraph-capturable):
+ *   nv_hydro_stage_batch_plan: validates the n members (as nv_hydro_stage does, plus: all members set smax_out or
+ *     none; solver, limiter, precision and the tile shape, NV_TILE_AUTO for the library's choice, are shared) and
+ *     writes a plan of nv_hydro_stage_batch_plan_size(n) bytes into host memory `plan`;
+ *   nv_hydro_stage_batch: launches the stage from the plan: plan_host (the host blob) and plan_dev (a device
+ *     copy of the same bytes, 8-byte aligned, which must stay alive and unchanged while launches use it). */
+long long nv_hydro_stage_batch_plan_size(int n);
+int nv_hydro_stage_batch_plan(const NvStageDesc* members, int n, int precision, int solver, int lim, int tile,
+                              void* plan, long long plan_bytes);
+int nv_hydro_stage_batch(const void* plan_host, const void* plan_dev, void* stream);
+
+/* Device-side timestep of n grids, so a step needs no host round trip: for each s,
+ *   dt = cfl_dx[s] / (double)smax[s]  (cfl_dx = cfl * dx computed on the host in double, as sim.py:1011 does),
+ *   dt = min(dt, dt_cap[s]) if dt_cap is non-null and dt_cap[s] > 0,
+ *   dt_f[s] = (float)dt (what the stages read through NvStageDesc.dt_dev), dt_d[s] = dt and t[s] += dt when those
+ *   are non-null (the clock of sim.py:1111), and smax[s] = 0 (ready for the next reduction).
+ * Replaces the host side of Sim._compute_dt (sim.py:1009-1012) and its readback. */
+int nv_cfl_dt(float* smax, const double* cfl_dx, const double* dt_cap, float* dt_f, double* dt_d, double* t, int n,
+              void* stream);
+
 /* CFL signal speed: out[0] = max(out[0], max over cells of |u|+|v|+|w| + sqrt(cn2 + xi (ci2 - cn2))), raw rho.
  * Replaces nebula3d.max_speed (nebula3d.py:154-162) as launched by Sim._compute_dt (sim.py:1009-1012). */
 int nv_max_speed(const float* rho, const float* mx, const float* my, const float* mz, const float* xi, int nx,
@@ -92,6 +172,13 @@ int nv_cast_rays(const float* rho, int nx, int ny, int nz, float sx, float sy, f
 int nv_cast_rays_wide(const float* rho, int nx, int ny, int nz, float sx, float sy, float sz, float Q, float dr,
                       int nsteps, float dx, float rcap, float bx, float by, float bz, float* rfront, int nu,
                       int nphi, void* stream);
+
+/* Batched ray casts: every (grid, star) job in one launch, each job's rfront bit-identical to nv_cast_rays
+ * (nv_cast_rays_wide when wide != 0) on that job alone. No reference counterpart (Sim.ionize launches per star,
+ * sim.py:936-937). Plan and launch as for nv_hydro_stage_batch. */
+long long nv_cast_rays_batch_plan_size(int n);
+int nv_cast_rays_batch_plan(const NvRayDesc* jobs, int n, void* plan, long long plan_bytes);
+int nv_cast_rays_batch(const void* plan_host, const void* plan_dev, void* stream);
 
 /* Per-cell front lookup: xi = max(xi, ramp), dfront = min(dfront, r - rf). Replaces nebula3d.apply_front
  * (nebula3d.py:281-307), launched per star by Sim.ionize (sim.py:938). */
@@ -123,6 +210,13 @@ int nv_ionize_apply_h(const float* rfronts, int nu, int nphi, const float* star_
                       float width, float* xi, uint16_t* dfront_h, float* dfront_f, int nx, int ny, int nz,
                       const float* rho, const uint16_t* vx, const uint16_t* vy, const uint16_t* vz, float cn2,
                       float ci2, float* smax_out, void* stream);
+
+/* Batched fused ionize pass over many grids in one launch, each member bit-identical to nv_ionize_apply
+ * (vel = 0) or nv_ionize_apply_h (vel = 1) on that member alone; smax_out set for all members or none. No
+ * reference counterpart (sim.py:927-947 is per Sim). Plan and launch as for nv_hydro_stage_batch. */
+long long nv_ionize_apply_batch_plan_size(int n);
+int nv_ionize_apply_batch_plan(const NvIonDesc* members, int n, int vel, void* plan, long long plan_bytes);
+int nv_ionize_apply_batch(const void* plan_host, const void* plan_dev, void* stream);
 
 /* FUV dust attenuation att[nr][nu][nphi] along rays from one star. Replaces nebula3d.cast_fuv
  * (nebula3d.py:320-352), launched per star by Sim.fuv_field (sim.py:964-965). */

@@ -34,7 +34,44 @@ SIGNATURES = {
     "nv_cast_fuv": [P, P, I, I, I, F, F, F, F, F, I, F, F, F, F, F, P, I, I, I, P],
     "nv_apply_fuv": [P, I, I, I, F, F, F, F, F, F, P, I, I, I, P],
     "nv_fuv_apply": [P, I, I, I, P, P, I, F, F, P, P, I, I, I, P],
+    "nv_hydro_stage_batch_plan_size": [I],
+    "nv_hydro_stage_ex": [P, I, I, I, I, P],
+    "nv_hydro_stage_batch_plan": [P, I, I, I, I, I, P, LL],
+    "nv_hydro_stage_batch": [P, P, P],
+    "nv_cfl_dt": [P, P, P, P, P, P, I, P],
+    "nv_cast_rays_batch_plan_size": [I],
+    "nv_cast_rays_batch_plan": [P, I, P, LL],
+    "nv_cast_rays_batch": [P, P, P],
+    "nv_ionize_apply_batch_plan_size": [I],
+    "nv_ionize_apply_batch_plan": [P, I, I, P, LL],
+    "nv_ionize_apply_batch": [P, P, P],
 }
+
+RESTYPES = {"nv_last_error": ctypes.c_char_p, "nv_hydro_stage_batch_plan_size": LL,
+            "nv_cast_rays_batch_plan_size": LL, "nv_ionize_apply_batch_plan_size": LL}
+
+
+class NvStageDesc(ctypes.Structure):
+    """one grid of nv_hydro_stage_batch_plan (include/nebula_hip.h)"""
+    _fields_ = [("u", P * 4), ("xi", P), ("u0", P * 4), ("out", P * 4), ("stars", P), ("jets", P),
+                ("smax_out", P), ("dt_dev", P), ("nx", I), ("ny", I), ("nz", I), ("nstars", I),
+                ("wind", F), ("jet_cos", F), ("jet_r", F), ("pulse", F), ("dt", F), ("dx", F), ("a", F), ("b", F),
+                ("floor_", F), ("cn2", F), ("ci2", F), ("vmax", F)]
+
+
+class NvRayDesc(ctypes.Structure):
+    """one job of nv_cast_rays_batch_plan"""
+    _fields_ = [("rho", P), ("nx", I), ("ny", I), ("nz", I), ("sx", F), ("sy", F), ("sz", F), ("Q", F), ("dr", F),
+                ("nsteps", I), ("dx", F), ("rcap", F), ("bx", F), ("by", F), ("bz", F), ("rfront", P), ("nu", I),
+                ("nphi", I), ("wide", I)]
+
+
+class NvIonDesc(ctypes.Structure):
+    """one grid of nv_ionize_apply_batch_plan"""
+    _fields_ = [("rfronts", P), ("nu", I), ("nphi", I), ("star_xyz", P), ("nstars", I), ("dx", F), ("width", F),
+                ("xi", P), ("dfront_h", P), ("dfront_f", P), ("nx", I), ("ny", I), ("nz", I), ("state", P * 4),
+                ("cn2", F), ("ci2", F), ("smax_out", P)]
+
 
 _lib = None
 
@@ -54,7 +91,7 @@ def load(path=LIB_PATH):
     for name, args in SIGNATURES.items():
         fn = getattr(lib, name)
         fn.argtypes = args
-        fn.restype = ctypes.c_char_p if name == "nv_last_error" else ctypes.c_int
+        fn.restype = RESTYPES.get(name, ctypes.c_int)
     if lib.nv_abi_version() != 1:
         raise NebulaHipError("libnebula_hip.so ABI version mismatch")
     _lib = lib

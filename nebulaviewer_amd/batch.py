@@ -1,0 +1,333 @@
+"""Many independent simulations advanced together: one launch per RK stage, per ionization pass and per CFL
+step for the whole batch.
+
+    batch = SimBatch(scenarios.batch_setups(64))
+    batch.step(200)
+    batch.members[3].rho          # each member is a Sim with its own fields
+
+The reference steps one grid per Sim (sim.py:1099-1114); there is no batching in it. Member m of a SimBatch
+reproduces Sim(setups[m]) stepped alone bit for bit: the batched kernels (nv_hydro_stage_batch,
+nv_cast_rays_batch, nv_ionize_apply_batch) run the single-grid arithmetic on each member's tiles, and the
+timestep is computed from each member's own CFL maximum in the host's double arithmetic.
+
+Two ways to take a step:
+  - device clock (when no member's host-side state depends on time during the steps: every star already on, no
+    luminosity ramp in progress, no pulsed jet): nv_cfl_dt computes each member's dt and clock on the GPU, so a
+    step is a fixed sequence of launches with no host round trip. With graph=True that sequence is captured
+    once per kind of step (with and without ionization) as a HIP graph and replayed.
+  - host clock (otherwise): one readback of the batch's CFL maxima per step, the dt computed per member on the
+    host as Sim._compute_dt does, and the star tables refreshed as Sim._main_step does.
+"""
+import ctypes
+import math
+
+import numpy as np
+import torch
+
+from . import _lib
+from . import scenarios as S
+from .sim import Sim, _f
+
+PRECISION_CODES = {"fp32": 0, "fp32-fast": 1, "fp16-vel": 2}
+
+
+class _Plan:
+    """A plan blob built on the host and its device copy (kept alive together)."""
+
+    def __init__(self, size_fn, build, n, device):
+        nbytes = int(getattr(_lib.load(), size_fn)(n))
+        self.host = ctypes.create_string_buffer(nbytes)
+        build(self.host, nbytes)
+        self.dev = torch.frombuffer(bytearray(self.host.raw), dtype=torch.uint8).to(device)
+
+    @property
+    def args(self):
+        return ctypes.addressof(self.host), self.dev.data_ptr()
+
+
+class SimBatch:
+    def __init__(self, setups, precision="fp32", device="cuda:0", rays=None, ion_every=None, graph=False,
+                 verbose=False, tile=0):
+        if not setups:
+            raise ValueError("a batch needs at least one member")
+        self.device = torch.device(device)
+        self.precision = precision
+        self.tile = tile          # workgroup tile shape of the stage (0: the library's choice; 1: 64x8; 2: 32x16)
+        self.members = [Sim(setup=st, verbose=verbose, precision=precision, device=device, rays=rays,
+                            ion_every=ion_every) for st in setups]
+        m0 = self.members[0]
+        for m in self.members:
+            if (m.P["solver"], m.P["limiter"]) != (m0.P["solver"], m0.P["limiter"]):
+                raise ValueError("batch members must share the solver and limiter")
+            if m.ion_every != m0.ion_every:
+                raise ValueError("batch members must share the ionization cadence")
+        self.n = len(self.members)
+        self.ion_every = m0.ion_every
+        self.solver = S.SOLVERS[m0.P.get("solver", "rusanov")]
+        self.lim = S.LIMITERS[m0.P.get("limiter", "minmod")] if self.solver == 1 else 0
+        dev = self.device
+        self.smax = torch.zeros(self.n, dtype=torch.float32, device=dev)
+        self.dt_f = torch.zeros(self.n, dtype=torch.float32, device=dev)
+        self.dt_d = torch.zeros(self.n, dtype=torch.float64, device=dev)
+        self.t_dev = torch.tensor([m.t for m in self.members], dtype=torch.float64, device=dev)
+        self.cfl_dx = torch.tensor([m.P["cfl"] * m.dx for m in self.members], dtype=torch.float64, device=dev)
+        self.dt_cap = torch.tensor([m.dt_cap or 0.0 for m in self.members], dtype=torch.float64, device=dev)
+        self.it = m0.it
+        self.graph = graph
+        self._graphs = {}
+        self._ion_key = None
+        self._stage_key = None
+        self.timer = None
+        # the CFL maximum of the current state, as Sim._compute_dt takes it when no fused value is cached
+        for s, m in enumerate(self.members):
+            slot = self.smax.data_ptr() + 4 * s
+            nx, ny, nz = m.shape
+            if m.vel_state:
+                _lib.call("nv_max_speed_h", *[t.data_ptr() for t in m.U[1:]], m.xi.data_ptr(), nx, ny, nz, _f(m.cn2),
+                          _f(m.ci2), slot, self._stream)
+            else:
+                _lib.call("nv_max_speed", *[t.data_ptr() for t in m.U], m.xi.data_ptr(), nx, ny, nz, _f(m.cn2),
+                          _f(m.ci2), slot, self._stream)
+        torch.cuda.synchronize(dev)
+
+    # ------------------------------------------------------------------ helpers
+    @property
+    def _stream(self):
+        return torch.cuda.current_stream(self.device).cuda_stream
+
+    def _t0(self, name):
+        return self.timer.start(name) if self.timer is not None else None
+
+    def _t1(self, tok):
+        if tok is not None:
+            self.timer.stop(tok)
+
+    @property
+    def cells(self):
+        return sum(m.shape[0] * m.shape[1] * m.shape[2] for m in self.members)
+
+    def _on(self, m):
+        return [i for i, s in enumerate(m.stars) if m.t >= s[4]]
+
+    def device_clock_ok(self):
+        """True when no member's host-side state changes with time over the next steps."""
+        for m in self.members:
+            tr = float(m.P.get("q_ramp", 0.0))
+            for i, s in enumerate(m.stars):
+                if m.t < s[4]:
+                    return False                      # a star still to switch on
+                if tr > 0.0 and m.t - s[4] < tr:
+                    return False                      # luminosity ramp in progress
+                if m.jet_of(i) > 0.0 and m.P["jet_pulse"] != 0.0:
+                    return False                      # pulsed jet: the pulse follows the clock
+        return True
+
+    @staticmethod
+    def _pulse(m):
+        """the jet pulse of Sim._main_step (sim.py:1108); 1 for a member without an active jet, whose stage never
+        reads it, so that its plans (and captured graphs) do not change with the clock"""
+        if not any(m.jet_of(i) > 0.0 for i in range(len(m.stars))):
+            return 1.0
+        return max(0.0, 1.0 + m.P["jet_pulse"] * math.sin(2 * math.pi * m.t / m.P["jet_period"]))
+
+    # ------------------------------------------------------------------ plans
+    def _stage_plans(self, pulses):
+        key = tuple(pulses)
+        if key == self._stage_key:
+            return
+        P = _lib.P
+        code = PRECISION_CODES[self.precision]
+        plans = []
+        for stage in (1, 2):
+            descs = (_lib.NvStageDesc * self.n)()
+            for s, m in enumerate(self.members):
+                d = descs[s]
+                U = [t.data_ptr() for t in m.U]
+                U1 = [m.stage[c].data_ptr() for c in range(4)]
+                src, dst = (U, U1) if stage == 1 else (U1, U)
+                for c in range(4):
+                    d.u[c] = src[c]
+                    d.u0[c] = U[c]
+                    d.out[c] = dst[c]
+                d.xi = m.xi.data_ptr()
+                d.stars = m.star_arr.data_ptr()
+                d.jets = m.jet_arr.data_ptr()
+                d.smax_out = (self.smax.data_ptr() + 4 * s) if stage == 2 else None
+                d.dt_dev = self.dt_f.data_ptr() + 4 * s
+                d.nx, d.ny, d.nz = m.shape
+                d.nstars = len(m.stars)
+                d.wind = m.P["wind"]
+                d.jet_cos = m.C["jet_cos"]
+                d.jet_r = m.P["jet_r"]
+                d.pulse = pulses[s]
+                d.dt = 0.0
+                d.dx = m.dx
+                d.a, d.b = (0.0, 1.0) if stage == 1 else (0.5, 0.5)
+                d.floor_ = m.P["floor"]
+                d.cn2, d.ci2, d.vmax = m.cn2, m.ci2, 6.0
+
+            def build(buf, nbytes, descs=descs):
+                _lib.call("nv_hydro_stage_batch_plan", ctypes.cast(descs, P), self.n, code, self.solver, self.lim,
+                          self.tile, buf, nbytes)
+            plans.append(_Plan("nv_hydro_stage_batch_plan_size", build, self.n, self.device))
+        self._plans_stage = plans
+        self._stage_key = key
+        self._graphs.clear()
+
+    def _ion_plans(self):
+        """ray-cast jobs of every switched-on star of every member, and the fused ionize pass per member"""
+        key = tuple((tuple(self._on(m)), tuple(round(m.q_eff(i), 12) for i in self._on(m))) for m in self.members)
+        if key == self._ion_key:
+            return
+        P = _lib.P
+        jobs = []
+        ions = (_lib.NvIonDesc * self.n)()
+        vel = 1 if self.precision == "fp16-vel" else 0
+        for s, m in enumerate(self.members):
+            on = self._on(m)
+            nu, nphi = m.rfronts.shape[1:]
+            Lx, Ly, Lz = m.bmax
+            for slot, i in enumerate(on):
+                j = _lib.NvRayDesc()
+                j.rho = m.U[0].data_ptr()
+                j.nx, j.ny, j.nz = m.shape
+                j.sx, j.sy, j.sz = m.stars[i][:3]
+                j.Q = m.q_eff(i)
+                j.dr = m.dr
+                j.nsteps = m.nsteps
+                j.dx = m.dx
+                j.rcap = m.diag
+                j.bx, j.by, j.bz = Lx, Ly, Lz
+                j.rfront = m.rfronts[slot].data_ptr()
+                j.nu, j.nphi = nu, nphi
+                j.wide = 0
+                jobs.append(j)
+            xyz = np.array([m.stars[i][:3] for i in on], dtype=np.float32).reshape(-1, 3)
+            if len(on):
+                m.star_xyz[:len(on)].copy_(torch.from_numpy(xyz))
+            d = ions[s]
+            half = m.dfront.dtype == torch.float16
+            d.rfronts = m.rfronts.data_ptr()
+            d.nu, d.nphi = nu, nphi
+            d.star_xyz = m.star_xyz.data_ptr()
+            d.nstars = len(on)
+            d.dx = m.dx
+            d.width = 1.2 * m.dx
+            d.xi = m.xi.data_ptr()
+            d.dfront_h = m.dfront.data_ptr() if half else None
+            d.dfront_f = None if half else m.dfront.data_ptr()
+            d.nx, d.ny, d.nz = m.shape
+            for c in range(4):
+                d.state[c] = m.U[c].data_ptr()
+            d.cn2, d.ci2 = m.cn2, m.ci2
+            d.smax_out = self.smax.data_ptr() + 4 * s
+        self._plan_rays = None
+        if jobs:
+            arr = (_lib.NvRayDesc * len(jobs))(*jobs)
+
+            def build_rays(buf, nbytes, arr=arr, n=len(jobs)):
+                _lib.call("nv_cast_rays_batch_plan", ctypes.cast(arr, P), n, buf, nbytes)
+            self._plan_rays = _Plan("nv_cast_rays_batch_plan_size", build_rays, len(jobs), self.device)
+
+        def build_ion(buf, nbytes):
+            _lib.call("nv_ionize_apply_batch_plan", ctypes.cast(ions, P), self.n, vel, buf, nbytes)
+        self._plan_ion = _Plan("nv_ionize_apply_batch_plan_size", build_ion, self.n, self.device)
+        self._ion_key = key
+        self._graphs.clear()
+
+    # ------------------------------------------------------------------ the step
+    def _launch_ionize(self):
+        strm = self._stream
+        _lib.call("nv_fill_f32", self.smax.data_ptr(), _f(0.0), self.n, strm)
+        if self._plan_rays is not None:
+            tk = self._t0("cast_rays")
+            _lib.call("nv_cast_rays_batch", *self._plan_rays.args, strm)
+            self._t1(tk)
+        tk = self._t0("ionize_apply")
+        _lib.call("nv_ionize_apply_batch", *self._plan_ion.args, strm)
+        self._t1(tk)
+
+    def _launch_hydro(self):
+        strm = self._stream
+        tk = self._t0("hydro_stage")
+        _lib.call("nv_hydro_stage_batch", *self._plans_stage[0].args, strm)
+        self._t1(tk)
+        tk = self._t0("hydro_stage")
+        _lib.call("nv_hydro_stage_batch", *self._plans_stage[1].args, strm)
+        self._t1(tk)
+
+    def _device_step(self, ion):
+        strm = self._stream
+        if ion:
+            self._launch_ionize()
+        _lib.call("nv_cfl_dt", self.smax.data_ptr(), self.cfl_dx.data_ptr(), self.dt_cap.data_ptr(),
+                  self.dt_f.data_ptr(), self.dt_d.data_ptr(), self.t_dev.data_ptr(), self.n, strm)
+        self._launch_hydro()
+
+    def _replay(self, ion):
+        g = self._graphs.get(ion)
+        if g is None:
+            # warm the launch path once outside capture, then record the step
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                self._device_step(ion)
+            self._graphs[ion] = g
+            g.replay()
+            return
+        g.replay()
+
+    def step(self, k=1):
+        """advance every member k steps (Sim._main_step order, sim.py:1099-1114)"""
+        for m in self.members:
+            m._set_star_weights()
+        if self.device_clock_ok():
+            self._stage_plans([self._pulse(m) for m in self.members])
+            self._ion_plans()
+            for _ in range(k):
+                ion = self.it % self.ion_every == 0
+                if self.graph and self.timer is None:
+                    self._replay(ion)
+                else:
+                    self._device_step(ion)
+                self.it += 1
+            self._sync_clocks(k)
+            return
+        for _ in range(k):
+            self._host_step()
+
+    def _host_step(self):
+        ion = self.it % self.ion_every == 0
+        if ion:
+            self._ion_plans()
+            self._launch_ionize()
+        for m in self.members:
+            m._set_star_weights()
+        smax = self.smax.cpu().numpy()             # the one readback per step (sim.py:1012 per member)
+        dts, pulses = [], []
+        for s, m in enumerate(self.members):
+            dt = m.P["cfl"] * m.dx / float(smax[s])
+            if m.dt_cap is not None:
+                dt = min(dt, m.dt_cap)
+            dts.append(dt)
+            pulses.append(self._pulse(m))
+        self.dt_f.copy_(torch.tensor(np.array(dts, dtype=np.float32)))
+        self.smax.zero_()
+        self._stage_plans(pulses)
+        self._launch_hydro()
+        for s, m in enumerate(self.members):
+            m.t += dts[s]
+            m.dt = dts[s]
+            m.it += 1
+            m.version += 1
+        self.it += 1
+        self.t_dev.copy_(torch.tensor([m.t for m in self.members], dtype=torch.float64))
+
+    def _sync_clocks(self, k):
+        t = self.t_dev.cpu().numpy()
+        dt = self.dt_d.cpu().numpy()
+        for s, m in enumerate(self.members):
+            m.t = float(t[s])
+            m.dt = float(dt[s])
+            m.it += k
+            m.version += k
+            m._smax_ok = False

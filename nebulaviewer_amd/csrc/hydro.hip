@@ -32,18 +32,29 @@
 //               directly (no division) and the conserved state is rho * v. Density stays float32: rounding it
 //               to binary16 every stage stalls updates below half an fp16 ulp (measured: rel-L1 density error
 //               4e-3 after 100 steps of the 64^3 torus against 1.4e-4 with float32 density).
+#include <string.h>
 #include "nv_common.h"
 #include "nv_divide.h"
 
 namespace nv {
 
-constexpr int TK = 64;                      // z extent of a tile: one wavefront along the contiguous axis
-constexpr int TJ = 8;                       // y rows per tile, one wavefront each
-constexpr int NT = TK * TJ;                 // threads per workgroup
-constexpr int PK = TK + 4, PJ = TJ + 4;     // primitive plane with a 2-cell halo
-constexpr int CK = TK + 2, CJ = TJ + 2;     // c^2 / c plane with a 1-cell halo
-constexpr int NHALO = 4 * PK + 4 * TJ;      // halo cells of the primitive plane
-constexpr int NCHALO = 2 * CK + 2 * TJ;     // halo cells of the c plane
+constexpr int NT = 512;                     // threads per workgroup, either tile shape
+
+// Tile shapes (z extent TK, y rows TJ, TK * TJ = NT): 64 x 8 (one wavefront per y row) and 32 x 16 (one
+// wavefront per two rows), the latter for z extents that are multiples of 32 but not of 64 and where its
+// smaller halo wins.
+constexpr int TILE_AUTO = 0, TILE_64x8 = 1, TILE_32x16 = 2;
+template <int TK_, int TJ_>
+struct Tile {
+  static constexpr int TK = TK_, TJ = TJ_;
+  static_assert(TK * TJ == NT, "a tile has NT threads");
+  static constexpr int PK = TK + 4, PJ = TJ + 4;     // primitive plane with a 2-cell halo
+  static constexpr int CK = TK + 2, CJ = TJ + 2;     // c^2 / c plane with a 1-cell halo
+  static constexpr int NHALO = 4 * PK + 4 * TJ;      // halo cells of the primitive plane
+  static constexpr int NCHALO = 2 * CK + 2 * TJ;     // halo cells of the c plane
+};
+using T64 = Tile<64, 8>;
+using T32 = Tile<32, 16>;
 
 struct HydroArgs {
   const void *f0, *f1, *f2, *f3;            // U: rho and momentum (F32) or rho and velocity (F16V)
@@ -57,6 +68,7 @@ struct HydroArgs {
   float wind, jet_cos, jet_r, pulse, dt, dx, a, b, floor_, cn2, ci2, vmax;
   float vmax2_lo;                           // largest float <= vmax^2: |v|^2 <= vmax2_lo means no cap
   float inv_dx;                             // 1/dx when dx is a power of two (exact), else 0
+  const float* dt_dev;                      // if non-null, dt is read here when the stage runs
 };
 
 // ------------------------------------------------------------------------------------------ arithmetic
@@ -231,9 +243,12 @@ __device__ __forceinline__ float4 face_flux(const float4& Lq, const float4& Rq, 
   return make_float4(F[0], F[1], F[2], F[3]);
 }
 
+// One tile (64 z x 8 y columns, marching along x over ci planes) of one RK stage.
 // STAGE1: a = 0, b = 1 and U0 = U (sim.py:1025), so out = 0*U0 + 1*Un is Un (up to the sign of zero).
-template <class ST, class AR, int SOLVER, int LIM, bool SMAX, bool STAGE1>
-__global__ __launch_bounds__(NT, 4) void hydro_stage_kernel(HydroArgs A) {
+template <class TL, class ST, class AR, int SOLVER, int LIM, bool SMAX, bool STAGE1>
+__device__ __forceinline__ void stage_tile(const HydroArgs& A, int tile) {
+  constexpr int TK = TL::TK, TJ = TL::TJ, PK = TL::PK, PJ = TL::PJ, CK = TL::CK, CJ = TL::CJ;
+  constexpr int NHALO = TL::NHALO, NCHALO = TL::NCHALO;
   constexpr int LM = SOLVER == NV_SOLVER_HLL ? LIM : NV_LIM_MINMOD;   // Rusanov always uses minmod
   using Rcp = typename AR::Rcp;
   __shared__ float4 sq[PJ * PK];            // primitives of plane i, 2-cell halo
@@ -248,10 +263,6 @@ __global__ __launch_bounds__(NT, 4) void hydro_stage_kernel(HydroArgs A) {
 
   const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TK + tx;
   const int nx = A.nx, ny = A.ny, nz = A.nz;
-  // Workgroups are dealt to the 8 XCDs round robin (id % 8); give each XCD a contiguous run of tiles (z
-  // fastest, then y, then the x chunk) so the tiles whose halos overlap run on one XCD and share its L2.
-  const int tile = (blockIdx.x & 7) * A.per_xcd + (blockIdx.x >> 3);
-  if (tile >= A.tiles) return;
   const int bk = tile % A.tiles_k, bj = (tile / A.tiles_k) % A.tiles_j, bi = tile / (A.tiles_k * A.tiles_j);
   const int k0 = bk * TK, j0 = bj * TJ, i0 = bi * A.ci;
   const int i1 = min(i0 + A.ci, nx);
@@ -261,6 +272,7 @@ __global__ __launch_bounds__(NT, 4) void hydro_stage_kernel(HydroArgs A) {
   // own column, clamped (threads past the domain edge still supply the clamped ghost values of ldc)
   const long long col = (long long)min(j, ny - 1) * nz + min(k, nz - 1);
   const float floor_ = A.floor_, cn2 = A.cn2, dci = A.ci2 - A.cn2;
+  const float dt = A.dt_dev ? *A.dt_dev : A.dt;
 
   auto load_col = [&](int p, float4& raw, float& x) {
     const long long o = (long long)clampi(p, 0, nx - 1) * plane + col;
@@ -475,13 +487,13 @@ __global__ __launch_bounds__(NT, 4) void hydro_stage_kernel(HydroArgs A) {
       float out[4];
       if constexpr (STAGE1) {
 #pragma unroll
-        for (int c = 0; c < 4; ++c) out[c] = AR::mad(A.dt, dU[c], U[c]);
+        for (int c = 0; c < 4; ++c) out[c] = AR::mad(dt, dU[c], U[c]);
       } else {
         const float4 U0c = conserved<ST>(U0raw);
         const float U0v[4] = {A.u0_is_u ? U[0] : U0c.x, A.u0_is_u ? U[1] : U0c.y, A.u0_is_u ? U[2] : U0c.z,
                               A.u0_is_u ? U[3] : U0c.w};
 #pragma unroll
-        for (int c = 0; c < 4; ++c) out[c] = AR::mad(A.a, U0v[c], A.b * AR::mad(A.dt, dU[c], U[c]));
+        for (int c = 0; c < 4; ++c) out[c] = AR::mad(A.a, U0v[c], A.b * AR::mad(dt, dU[c], U[c]));
       }
       const float r = fmaxf(out[0], floor_);
       const Rcp rr = AR::rcp(r);
@@ -527,6 +539,50 @@ __global__ __launch_bounds__(NT, 4) void hydro_stage_kernel(HydroArgs A) {
   if constexpr (SMAX) block_max_atomic(tmax, red, A.smax);
 }
 
+// Workgroups are dealt to the 8 XCDs round robin (id % 8); give each XCD a contiguous run of tiles (z fastest,
+// then y, then the x chunk, then the next member of a batch) so the tiles whose halos overlap run on one XCD and
+// share its L2.
+__device__ __forceinline__ int xcd_tile(int per_xcd) { return (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3); }
+
+template <class TL, class ST, class AR, int SOLVER, int LIM, bool SMAX, bool STAGE1>
+__global__ __launch_bounds__(NT, 4) void hydro_stage_kernel(HydroArgs A) {
+  const int tile = xcd_tile(A.per_xcd);
+  if (tile >= A.tiles) return;
+  stage_tile<TL, ST, AR, SOLVER, LIM, SMAX, STAGE1>(A, tile);
+}
+
+// A batch of independent grids in one launch: tile t of the flat list belongs to the member s with
+// start[s] <= t < start[s+1] (start[0] = 0, start[n] = total), whose arguments are args[s].
+template <class TL, class ST, class AR, int SOLVER, int LIM, bool SMAX, bool STAGE1>
+__global__ __launch_bounds__(NT, 4) void hydro_stage_batch_kernel(const HydroArgs* __restrict__ args,
+                                                                  const int* __restrict__ start, int n, int total,
+                                                                  int per_xcd) {
+  const int t = xcd_tile(per_xcd);
+  if (t >= total) return;
+  int lo = 0, hi = n;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (start[mid] <= t) lo = mid;
+    else hi = mid;
+  }
+  stage_tile<TL, ST, AR, SOLVER, LIM, SMAX, STAGE1>(args[lo], t - start[lo]);
+}
+
+// Device-side timestep of n grids (Sim._compute_dt, sim.py:1009-1012, and the clock of sim.py:1099-1114), in
+// the host's double arithmetic: dt = (cfl dx) / smax, capped, rounded to float for the stages; t += dt; and smax
+// is cleared for the next reduction.
+__global__ void cfl_dt_kernel(float* smax, const double* __restrict__ cfl_dx, const double* __restrict__ dt_cap,
+                              float* dt_f, double* dt_d, double* t, int n) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  double dt = cfl_dx[s] / (double)smax[s];
+  if (dt_cap && dt_cap[s] > 0.0 && dt_cap[s] < dt) dt = dt_cap[s];
+  dt_f[s] = (float)dt;
+  if (dt_d) dt_d[s] = dt;
+  if (t) t[s] = t[s] + dt;
+  smax[s] = 0.0f;
+}
+
 // max_speed (nebula3d.py:154-162) of float32 (rho, m) fields
 __global__ __launch_bounds__(256) void max_speed_kernel(const float* rho, const float* mx, const float* my,
                                                         const float* mz, const float* xi, long long n, float cn2,
@@ -558,21 +614,105 @@ __global__ __launch_bounds__(256) void max_speed_h_kernel(const uint16_t* vx, co
 
 using HydroKernel = void (*)(HydroArgs);
 
-template <class ST, class AR, bool SMAX, bool STAGE1>
+template <class TL, class ST, class AR, bool SMAX, bool STAGE1>
 static HydroKernel pick_lim(int solver, int lim) {
-  if (solver == NV_SOLVER_RUSANOV) return hydro_stage_kernel<ST, AR, NV_SOLVER_RUSANOV, NV_LIM_MINMOD, SMAX, STAGE1>;
-  if (lim == NV_LIM_VANLEER) return hydro_stage_kernel<ST, AR, NV_SOLVER_HLL, NV_LIM_VANLEER, SMAX, STAGE1>;
-  if (lim == NV_LIM_MC) return hydro_stage_kernel<ST, AR, NV_SOLVER_HLL, NV_LIM_MC, SMAX, STAGE1>;
-  return hydro_stage_kernel<ST, AR, NV_SOLVER_HLL, NV_LIM_MINMOD, SMAX, STAGE1>;
+  if (solver == NV_SOLVER_RUSANOV) return hydro_stage_kernel<TL, ST, AR, NV_SOLVER_RUSANOV, NV_LIM_MINMOD, SMAX, STAGE1>;
+  if (lim == NV_LIM_VANLEER) return hydro_stage_kernel<TL, ST, AR, NV_SOLVER_HLL, NV_LIM_VANLEER, SMAX, STAGE1>;
+  if (lim == NV_LIM_MC) return hydro_stage_kernel<TL, ST, AR, NV_SOLVER_HLL, NV_LIM_MC, SMAX, STAGE1>;
+  return hydro_stage_kernel<TL, ST, AR, NV_SOLVER_HLL, NV_LIM_MINMOD, SMAX, STAGE1>;
 }
 
-template <class ST, class AR>
+template <class TL, class ST, class AR>
 static HydroKernel pick(int solver, int lim, bool smax, bool stage1) {
-  if (smax) return stage1 ? pick_lim<ST, AR, true, true>(solver, lim) : pick_lim<ST, AR, true, false>(solver, lim);
-  return stage1 ? pick_lim<ST, AR, false, true>(solver, lim) : pick_lim<ST, AR, false, false>(solver, lim);
+  if (smax) return stage1 ? pick_lim<TL, ST, AR, true, true>(solver, lim) : pick_lim<TL, ST, AR, true, false>(solver, lim);
+  return stage1 ? pick_lim<TL, ST, AR, false, true>(solver, lim) : pick_lim<TL, ST, AR, false, false>(solver, lim);
 }
 
 enum class Mode { F32_EXACT, F32_FAST, F16V_FAST };
+
+template <class TL>
+static HydroKernel pick_mode_tl(Mode mode, int solver, int lim, bool sm, bool stage1) {
+  return mode == Mode::F32_EXACT  ? pick<TL, F32, Exact>(solver, lim, sm, stage1)
+         : mode == Mode::F32_FAST ? pick<TL, F32, Fast>(solver, lim, sm, stage1)
+                                  : pick<TL, F16V, Fast>(solver, lim, sm, stage1);
+}
+
+static HydroKernel pick_mode(int tile, Mode mode, int solver, int lim, bool sm, bool stage1) {
+  return tile == TILE_32x16 ? pick_mode_tl<T32>(mode, solver, lim, sm, stage1)
+                            : pick_mode_tl<T64>(mode, solver, lim, sm, stage1);
+}
+
+static int tile_tk(int tile) { return tile == TILE_32x16 ? T32::TK : T64::TK; }
+static int tile_tj(int tile) { return tile == TILE_32x16 ? T32::TJ : T64::TJ; }
+
+// The shape for a z extent when the caller leaves it to the library: 64 x 8 when nz is a multiple of 64 (no idle
+// lanes), 32 x 16 when it is a multiple of 32 only, otherwise the one with fewer idle lanes.
+static int auto_tile(int nz) {
+  if (nz % 64 == 0) return TILE_64x8;
+  const int idle64 = ((nz + 63) / 64) * 64 - nz, idle32 = ((nz + 31) / 32) * 32 - nz;
+  return idle32 < idle64 ? TILE_32x16 : TILE_64x8;
+}
+
+// Validates one grid's stage arguments and fills its kernel arguments. jk_total: the number of (y, z) tile
+// columns the launch covers (this grid's own, or a whole batch's), which sets the march depth ci so that the
+// launch has about 2048 x-chunks of tiles in all.
+static int fill_stage(const char* name, const NvStageDesc& d, int solver, int lim, int tile, long long jk_total,
+                      HydroArgs& A) {
+  NV_REQUIRE(d.u[0] && d.u[1] && d.u[2] && d.u[3] && d.xi && d.u0[0] && d.u0[1] && d.u0[2] && d.u0[3] && d.out[0] &&
+                 d.out[1] && d.out[2] && d.out[3],
+             "%s: null field pointer", name);
+  NV_REQUIRE(d.nx > 0 && d.ny > 0 && d.nz > 0, "%s: bad grid %dx%dx%d", name, d.nx, d.ny, d.nz);
+  NV_REQUIRE(d.nstars >= 0 && (d.nstars == 0 || (d.stars && d.jets)), "%s: nstars=%d needs star/jet tables", name,
+             d.nstars);
+  NV_REQUIRE(solver == NV_SOLVER_RUSANOV || solver == NV_SOLVER_HLL, "%s: bad solver %d", name, solver);
+  NV_REQUIRE(lim >= NV_LIM_MINMOD && lim <= NV_LIM_MC, "%s: bad limiter %d", name, lim);
+  NV_REQUIRE(d.out[0] != d.u[0] && d.out[1] != d.u[1] && d.out[2] != d.u[2] && d.out[3] != d.u[3],
+             "%s: output aliases the input U", name);
+  NV_REQUIRE(d.floor_ > 0.0f && d.dx > 0.0f, "%s: floor and dx must be > 0", name);
+  A.f0 = d.u[0]; A.f1 = d.u[1]; A.f2 = d.u[2]; A.f3 = d.u[3]; A.xi = d.xi;
+  A.g0 = d.u0[0]; A.g1 = d.u0[1]; A.g2 = d.u0[2]; A.g3 = d.u0[3];
+  A.o0 = d.out[0]; A.o1 = d.out[1]; A.o2 = d.out[2]; A.o3 = d.out[3];
+  A.stars = reinterpret_cast<const float4*>(d.stars);
+  A.jets = reinterpret_cast<const float4*>(d.jets);
+  A.smax = d.smax_out;
+  A.dt_dev = d.dt_dev;
+  A.nx = d.nx; A.ny = d.ny; A.nz = d.nz; A.nstars = d.nstars;
+  A.u0_is_u = (d.u0[0] == d.u[0] && d.u0[1] == d.u[1] && d.u0[2] == d.u[2] && d.u0[3] == d.u[3]) ? 1 : 0;
+  A.wind = d.wind; A.jet_cos = d.jet_cos; A.jet_r = d.jet_r; A.pulse = d.pulse; A.dt = d.dt; A.dx = d.dx;
+  A.a = d.a; A.b = d.b; A.floor_ = d.floor_; A.cn2 = d.cn2; A.ci2 = d.ci2; A.vmax = d.vmax;
+  {
+    const double v2 = (double)d.vmax * (double)d.vmax;
+    float lo = (float)v2;
+    if ((double)lo > v2) lo = nextafterf(lo, 0.0f);
+    A.vmax2_lo = lo;
+  }
+  A.inv_dx = pow2_reciprocal(d.dx);
+  // march depth: long enough to amortise the 4-plane prologue, short enough to fill the chip
+  A.ci = (int)std::min<long long>(64, std::max<long long>(4, (long long)d.nx * jk_total / 2048));
+  A.tiles_k = (d.nz + tile_tk(tile) - 1) / tile_tk(tile);
+  A.tiles_j = (d.ny + tile_tj(tile) - 1) / tile_tj(tile);
+  A.tiles = A.tiles_k * A.tiles_j * ((d.nx + A.ci - 1) / A.ci);
+  A.per_xcd = (A.tiles + 7) / 8;
+  return NV_OK;
+}
+
+static long long tiles_jk(int tile, int ny, int nz) {
+  return (long long)((nz + tile_tk(tile) - 1) / tile_tk(tile)) * ((ny + tile_tj(tile) - 1) / tile_tj(tile));
+}
+
+static bool is_stage1(const HydroArgs& A) { return A.u0_is_u && A.a == 0.0f && A.b == 1.0f; }
+
+static int launch_desc(Mode mode, const char* name, const NvStageDesc& d, int solver, int lim, int tile,
+                       void* stream) {
+  NV_REQUIRE(tile >= TILE_AUTO && tile <= TILE_32x16, "%s: bad tile shape %d", name, tile);
+  if (tile == TILE_AUTO) tile = auto_tile(d.nz);
+  HydroArgs A;
+  const int rc = fill_stage(name, d, solver, lim, tile, tiles_jk(tile, d.ny, d.nz), A);
+  if (rc != NV_OK) return rc;
+  HydroKernel kern = pick_mode(tile, mode, solver, lim, d.smax_out != nullptr, is_stage1(A));
+  hipLaunchKernelGGL(kern, dim3(8 * A.per_xcd), dim3(tile_tk(tile), tile_tj(tile), 1), 0, as_stream(stream), A);
+  return check_launch(name);
+}
 
 static int launch_stage(Mode mode, const char* name, const void* f0, const void* f1, const void* f2, const void* f3,
                         const float* xi, const void* g0, const void* g1, const void* g2, const void* g3, void* o0,
@@ -581,47 +721,68 @@ static int launch_stage(Mode mode, const char* name, const void* f0, const void*
                         float a, float b, float floor_, float cn2, float ci2, float vmax, int solver, int lim,
                         float* smax_out, void* stream) {
   clear_error();
-  NV_REQUIRE(f0 && f1 && f2 && f3 && xi && g0 && g1 && g2 && g3 && o0 && o1 && o2 && o3, "%s: null field pointer",
-             name);
-  NV_REQUIRE(nx > 0 && ny > 0 && nz > 0, "%s: bad grid %dx%dx%d", name, nx, ny, nz);
-  NV_REQUIRE(nstars >= 0 && (nstars == 0 || (stars && jets)), "%s: nstars=%d needs star/jet tables", name, nstars);
-  NV_REQUIRE(solver == NV_SOLVER_RUSANOV || solver == NV_SOLVER_HLL, "%s: bad solver %d", name, solver);
-  NV_REQUIRE(lim >= NV_LIM_MINMOD && lim <= NV_LIM_MC, "%s: bad limiter %d", name, lim);
-  NV_REQUIRE(o0 != f0 && o1 != f1 && o2 != f2 && o3 != f3, "%s: output aliases the input U", name);
-  NV_REQUIRE(floor_ > 0.0f && dx > 0.0f, "%s: floor and dx must be > 0", name);
-  HydroArgs A;
-  A.f0 = f0; A.f1 = f1; A.f2 = f2; A.f3 = f3; A.xi = xi;
-  A.g0 = g0; A.g1 = g1; A.g2 = g2; A.g3 = g3;
-  A.o0 = o0; A.o1 = o1; A.o2 = o2; A.o3 = o3;
-  A.stars = reinterpret_cast<const float4*>(stars);
-  A.jets = reinterpret_cast<const float4*>(jets);
-  A.smax = smax_out;
-  A.nx = nx; A.ny = ny; A.nz = nz; A.nstars = nstars;
-  A.u0_is_u = (g0 == f0 && g1 == f1 && g2 == f2 && g3 == f3) ? 1 : 0;
-  A.wind = wind; A.jet_cos = jet_cos; A.jet_r = jet_r; A.pulse = pulse; A.dt = dt; A.dx = dx;
-  A.a = a; A.b = b; A.floor_ = floor_; A.cn2 = cn2; A.ci2 = ci2; A.vmax = vmax;
-  {
-    const double v2 = (double)vmax * (double)vmax;
-    float lo = (float)v2;
-    if ((double)lo > v2) lo = nextafterf(lo, 0.0f);
-    A.vmax2_lo = lo;
+  NvStageDesc d = {};
+  d.u[0] = f0; d.u[1] = f1; d.u[2] = f2; d.u[3] = f3; d.xi = xi;
+  d.u0[0] = g0; d.u0[1] = g1; d.u0[2] = g2; d.u0[3] = g3;
+  d.out[0] = o0; d.out[1] = o1; d.out[2] = o2; d.out[3] = o3;
+  d.stars = stars; d.jets = jets; d.smax_out = smax_out; d.dt_dev = nullptr;
+  d.nx = nx; d.ny = ny; d.nz = nz; d.nstars = nstars;
+  d.wind = wind; d.jet_cos = jet_cos; d.jet_r = jet_r; d.pulse = pulse; d.dt = dt; d.dx = dx;
+  d.a = a; d.b = b; d.floor_ = floor_; d.cn2 = cn2; d.ci2 = ci2; d.vmax = vmax;
+  return launch_desc(mode, name, d, solver, lim, TILE_AUTO, stream);
+}
+
+// ------------------------------------------------------------------------------------------ batch plans
+// A plan is one host-built blob, uploaded once and used by every launch: header, kernel arguments of each
+// member, then the tile prefix start[0..n] (start[n] = total tiles).
+struct BatchHeader {
+  uint32_t magic;
+  int n, mode, solver, lim, smax, stage1, total, per_xcd, tile;
+  int pad[6];
+};
+static_assert(sizeof(BatchHeader) == 64, "plan header layout");
+constexpr uint32_t kBatchMagic = 0x4e564231u;   // "NVB1"
+
+static long long plan_bytes(int n) {
+  return (long long)sizeof(BatchHeader) + (long long)n * (long long)sizeof(HydroArgs) + 4LL * (n + 1);
+}
+
+typedef void (*BatchKernel)(const HydroArgs*, const int*, int, int, int);
+
+template <class TL, class ST, class AR, bool SMAX, bool STAGE1>
+static BatchKernel pick_batch_lim(int solver, int lim) {
+  if (solver == NV_SOLVER_RUSANOV)
+    return hydro_stage_batch_kernel<TL, ST, AR, NV_SOLVER_RUSANOV, NV_LIM_MINMOD, SMAX, STAGE1>;
+  if (lim == NV_LIM_VANLEER) return hydro_stage_batch_kernel<TL, ST, AR, NV_SOLVER_HLL, NV_LIM_VANLEER, SMAX, STAGE1>;
+  if (lim == NV_LIM_MC) return hydro_stage_batch_kernel<TL, ST, AR, NV_SOLVER_HLL, NV_LIM_MC, SMAX, STAGE1>;
+  return hydro_stage_batch_kernel<TL, ST, AR, NV_SOLVER_HLL, NV_LIM_MINMOD, SMAX, STAGE1>;
+}
+
+template <class TL, class ST, class AR>
+static BatchKernel pick_batch_st(int solver, int lim, bool sm, bool stage1) {
+  if (sm)
+    return stage1 ? pick_batch_lim<TL, ST, AR, true, true>(solver, lim)
+                  : pick_batch_lim<TL, ST, AR, true, false>(solver, lim);
+  return stage1 ? pick_batch_lim<TL, ST, AR, false, true>(solver, lim)
+                : pick_batch_lim<TL, ST, AR, false, false>(solver, lim);
+}
+
+template <class TL>
+static BatchKernel pick_batch_tl(int mode, int solver, int lim, bool sm, bool stage1) {
+  return mode == NV_PREC_F32        ? pick_batch_st<TL, F32, Exact>(solver, lim, sm, stage1)
+         : mode == NV_PREC_F32_FAST ? pick_batch_st<TL, F32, Fast>(solver, lim, sm, stage1)
+                                    : pick_batch_st<TL, F16V, Fast>(solver, lim, sm, stage1);
+}
+
+// The shape for a batch: the one with fewer idle lanes over all members (cells of the padded z extents).
+static int auto_tile_batch(const NvStageDesc* m, int n) {
+  long long pad64 = 0, pad32 = 0;
+  for (int s = 0; s < n; ++s) {
+    const long long xy = (long long)m[s].nx * m[s].ny;
+    pad64 += xy * (((m[s].nz + 63) / 64) * 64);
+    pad32 += xy * (((m[s].nz + 31) / 32) * 32);
   }
-  A.inv_dx = pow2_reciprocal(dx);
-  // march depth: long enough to amortise the 4-plane prologue, short enough to fill the chip
-  const long long tiles_jk = (long long)((nz + TK - 1) / TK) * ((ny + TJ - 1) / TJ);
-  const int ci = (int)std::min<long long>(64, std::max<long long>(4, (long long)nx * tiles_jk / 2048));
-  A.ci = ci;
-  A.tiles_k = (nz + TK - 1) / TK;
-  A.tiles_j = (ny + TJ - 1) / TJ;
-  A.tiles = A.tiles_k * A.tiles_j * ((nx + ci - 1) / ci);
-  A.per_xcd = (A.tiles + 7) / 8;
-  const bool stage1 = A.u0_is_u && a == 0.0f && b == 1.0f;
-  const bool sm = smax_out != nullptr;
-  HydroKernel kern = mode == Mode::F32_EXACT  ? pick<F32, Exact>(solver, lim, sm, stage1)
-                     : mode == Mode::F32_FAST ? pick<F32, Fast>(solver, lim, sm, stage1)
-                                              : pick<F16V, Fast>(solver, lim, sm, stage1);
-  hipLaunchKernelGGL(kern, dim3(8 * A.per_xcd), dim3(TK, TJ, 1), 0, as_stream(stream), A);
-  return check_launch(name);
+  return pad32 < pad64 ? TILE_32x16 : TILE_64x8;
 }
 
 }  // namespace nv
@@ -660,6 +821,96 @@ extern "C" int nv_hydro_stage_h(const float* rho, const uint16_t* vx, const uint
                                 void* stream) {
   return nv::launch_stage(nv::Mode::F16V_FAST, "nv_hydro_stage_h", rho, vx, vy, vz, xi, r0, vx0, vy0, vz0, o_r, o_vx,
                           o_vy, o_vz, NV_STAGE_TAIL);
+}
+
+extern "C" int nv_hydro_stage_ex(const NvStageDesc* d, int precision, int solver, int lim, int tile, void* stream) {
+  using namespace nv;
+  clear_error();
+  NV_REQUIRE(d, "nv_hydro_stage_ex: null descriptor");
+  NV_REQUIRE(precision >= NV_PREC_F32 && precision <= NV_PREC_F16V, "nv_hydro_stage_ex: bad precision %d", precision);
+  const Mode mode = precision == NV_PREC_F32 ? Mode::F32_EXACT : precision == NV_PREC_F32_FAST ? Mode::F32_FAST
+                                                                                               : Mode::F16V_FAST;
+  return launch_desc(mode, "nv_hydro_stage_ex", *d, solver, lim, tile, stream);
+}
+
+extern "C" long long nv_hydro_stage_batch_plan_size(int n) { return n > 0 ? nv::plan_bytes(n) : -1; }
+
+extern "C" int nv_hydro_stage_batch_plan(const NvStageDesc* members, int n, int precision, int solver, int lim,
+                                         int tile, void* plan, long long plan_bytes) {
+  using namespace nv;
+  clear_error();
+  NV_REQUIRE(members && plan && n > 0, "nv_hydro_stage_batch_plan: need n > 0 members and a plan buffer");
+  NV_REQUIRE(plan_bytes >= nv::plan_bytes(n), "nv_hydro_stage_batch_plan: plan needs %lld bytes, got %lld",
+             nv::plan_bytes(n), plan_bytes);
+  NV_REQUIRE(precision >= NV_PREC_F32 && precision <= NV_PREC_F16V, "nv_hydro_stage_batch_plan: bad precision %d",
+             precision);
+  NV_REQUIRE(tile >= TILE_AUTO && tile <= TILE_32x16, "nv_hydro_stage_batch_plan: bad tile shape %d", tile);
+  long long jk_total = 0;
+  for (int s = 0; s < n; ++s) {
+    NV_REQUIRE(members[s].nx > 0 && members[s].ny > 0 && members[s].nz > 0,
+               "nv_hydro_stage_batch_plan: member %d: bad grid", s);
+  }
+  if (tile == TILE_AUTO) tile = auto_tile_batch(members, n);
+  for (int s = 0; s < n; ++s) jk_total += tiles_jk(tile, members[s].ny, members[s].nz);
+  BatchHeader H = {};
+  HydroArgs* args = reinterpret_cast<HydroArgs*>(static_cast<char*>(plan) + sizeof(BatchHeader));
+  int* start = reinterpret_cast<int*>(args + n);
+  const bool sm = members[0].smax_out != nullptr;
+  bool stage1 = true;
+  long long total = 0;
+  char name[64];
+  for (int s = 0; s < n; ++s) {
+    snprintf(name, sizeof(name), "nv_hydro_stage_batch_plan: member %d", s);
+    HydroArgs A;
+    const int rc = fill_stage(name, members[s], solver, lim, tile, jk_total, A);
+    if (rc != NV_OK) return rc;
+    NV_REQUIRE((members[s].smax_out != nullptr) == sm, "%s: smax_out must be set for all members or none", name);
+    stage1 = stage1 && is_stage1(A);
+    start[s] = (int)total;
+    total += A.tiles;
+    NV_REQUIRE(total < (1LL << 30), "nv_hydro_stage_batch_plan: more than 2^30 tiles");
+    memcpy(&args[s], &A, sizeof(A));
+  }
+  start[n] = (int)total;
+  H.magic = kBatchMagic;
+  H.n = n;
+  H.mode = precision;
+  H.solver = solver;
+  H.lim = lim;
+  H.smax = sm ? 1 : 0;
+  H.stage1 = stage1 ? 1 : 0;
+  H.total = (int)total;
+  H.per_xcd = (int)((total + 7) / 8);
+  H.tile = tile;
+  memcpy(plan, &H, sizeof(H));
+  return NV_OK;
+}
+
+extern "C" int nv_hydro_stage_batch(const void* plan_host, const void* plan_dev, void* stream) {
+  using namespace nv;
+  clear_error();
+  NV_REQUIRE(plan_host && plan_dev, "nv_hydro_stage_batch: null plan");
+  BatchHeader H;
+  memcpy(&H, plan_host, sizeof(H));
+  NV_REQUIRE(H.magic == kBatchMagic && H.n > 0, "nv_hydro_stage_batch: not a stage batch plan");
+  const HydroArgs* args = reinterpret_cast<const HydroArgs*>(static_cast<const char*>(plan_dev) + sizeof(BatchHeader));
+  const int* start = reinterpret_cast<const int*>(args + H.n);
+  const bool sm = H.smax != 0, st1 = H.stage1 != 0;
+  BatchKernel kern = H.tile == TILE_32x16 ? pick_batch_tl<T32>(H.mode, H.solver, H.lim, sm, st1)
+                                          : pick_batch_tl<T64>(H.mode, H.solver, H.lim, sm, st1);
+  hipLaunchKernelGGL(kern, dim3(8 * H.per_xcd), dim3(tile_tk(H.tile), tile_tj(H.tile), 1), 0, as_stream(stream), args,
+                     start, H.n, H.total, H.per_xcd);
+  return check_launch("nv_hydro_stage_batch");
+}
+
+extern "C" int nv_cfl_dt(float* smax, const double* cfl_dx, const double* dt_cap, float* dt_f, double* dt_d,
+                         double* t, int n, void* stream) {
+  using namespace nv;
+  clear_error();
+  NV_REQUIRE(smax && cfl_dx && dt_f && n > 0, "nv_cfl_dt: null pointer or n <= 0");
+  hipLaunchKernelGGL(cfl_dt_kernel, dim3((n + 255) / 256), dim3(256), 0, as_stream(stream), smax, cfl_dx, dt_cap, dt_f,
+                     dt_d, t, n);
+  return check_launch("nv_cfl_dt");
 }
 
 extern "C" int nv_max_speed(const float* rho, const float* mx, const float* my, const float* mz, const float* xi,

@@ -10,8 +10,8 @@
 //
 // Methods (textbook, no library code reproduced):
 //   sin / cos : Cody-Waite reduction by pi/2 (two-part constant), Taylor series to degree 17/18 on |r|<=pi/4.
-//   atan2     : quadrant folding, atan(t) = pi/6 + atan((sqrt3 t - 1)/(t + sqrt3)) for t > tan(pi/12),
-//               Taylor series of atan to degree 27 on |t| <= tan(pi/12).
+//   atan2     : quadrant folding, atan(t) = pi/6 + atan((sqrt3 t - 1)/(t + sqrt3)) for t > tan(pi/12) folded
+//               into one division, Taylor series of atan to degree 19 on |t| <= tan(pi/12).
 //   exp       : Cody-Waite reduction by ln2, Taylor series to degree 13 on |r| <= ln2/2, exact 2^k scaling.
 // Valid domain: finite arguments; sincos is used for |x| < 2^20 (ray angles lie in [0, 2pi)).
 #pragma once
@@ -86,33 +86,34 @@ NV_HD inline void sincosf_det(float xf, float* s, float* c) {
 NV_HD inline float sinf_det(float x) { float s, c; sincosf_det(x, &s, &c); return s; }
 NV_HD inline float cosf_det(float x) { float s, c; sincosf_det(x, &s, &c); return c; }
 
-// atan(t) for 0 <= t <= 1
-NV_HD inline double atan_unit(double t) {
-  double base = 0.0;
-  if (t > TAN_PI_12) {
-    t = (t * SQRT3_D - 1.0) / (t + SQRT3_D);
-    base = PIO6_D;
-  }
-  const double t2 = t * t;
-  // sum_{k=0}^{13} (-1)^k t^(2k+1) / (2k+1), Horner from the top
-  double p = -1.0 / 27.0;
-  p = p * t2 + 1.0 / 25.0;
-  p = p * t2 - 1.0 / 23.0;
-  p = p * t2 + 1.0 / 21.0;
-  p = p * t2 - 1.0 / 19.0;
-  p = p * t2 + 1.0 / 17.0;
-  p = p * t2 - 1.0 / 15.0;
-  p = p * t2 + 1.0 / 13.0;
-  p = p * t2 - 1.0 / 11.0;
-  p = p * t2 + 1.0 / 9.0;
-  p = p * t2 - 1.0 / 7.0;
-  p = p * t2 + 1.0 / 5.0;
-  p = p * t2 - 1.0 / 3.0;
-  p = p * t2 + 1.0;
-  return base + t * p;
+// atan(a / b) for 0 <= a <= b, b > 0, with one division: for a/b > tan(pi/12) the identity
+// atan(t) = pi/6 + atan((sqrt3 t - 1)/(t + sqrt3)) is applied to t = a/b as (sqrt3 a - b)/(a + sqrt3 b), so the
+// series argument s satisfies |s| <= tan(pi/12) = 0.268 either way; the Taylor series of atan to degree 19
+// leaves a relative error below s^20/21 < 2e-13. The two cases select their numerator and denominator and
+// share the division and the series (one code path for every lane of a wavefront).
+NV_HD inline double atan_ratio(double a, double b) {
+  const bool big = a > TAN_PI_12 * b;
+  const double num = big ? SQRT3_D * a - b : a;
+  const double den = big ? a + SQRT3_D * b : b;
+  const double s = num / den;
+  const double s2 = s * s;
+  // sum_{k=0}^{9} (-1)^k s^(2k+1) / (2k+1), Horner from the top
+  double p = -1.0 / 19.0;
+  p = p * s2 + 1.0 / 17.0;
+  p = p * s2 - 1.0 / 15.0;
+  p = p * s2 + 1.0 / 13.0;
+  p = p * s2 - 1.0 / 11.0;
+  p = p * s2 + 1.0 / 9.0;
+  p = p * s2 - 1.0 / 7.0;
+  p = p * s2 + 1.0 / 5.0;
+  p = p * s2 - 1.0 / 3.0;
+  p = p * s2 + 1.0;
+  return (big ? PIO6_D : 0.0) + s * p;
 }
 
-// atan2 with the C99 conventions for signed zeros; finite arguments only.
+// atan2 with the C99 conventions for signed zeros; finite arguments only. The octant is folded first so that
+// one atan_ratio serves every case: atan2 = atan(min/max), or pi/2 - atan(min/max) above the diagonal,
+// reflected to pi - a for x < 0 and negated for y < 0.
 NV_HD inline float atan2f_det(float yf, float xf) {
   const double y = (double)yf, x = (double)xf;
   const double ay = fabs(y), ax = fabs(x);
@@ -121,11 +122,10 @@ NV_HD inline float atan2f_det(float yf, float xf) {
   double a;
   if (ay == 0.0 && ax == 0.0) {
     a = xneg ? PI_D : 0.0;
-  } else if (ay > ax) {
-    a = PIO2_D - atan_unit(ax / ay);
-    if (xneg) a = PI_D - a;
   } else {
-    a = atan_unit(ay / ax);
+    const bool steep = ay > ax;
+    const double t = atan_ratio(steep ? ax : ay, steep ? ay : ax);
+    a = steep ? PIO2_D - t : t;
     if (xneg) a = PI_D - a;
   }
   float r = (float)a;

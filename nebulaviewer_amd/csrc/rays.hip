@@ -10,6 +10,8 @@
 // pass (a 1-cell halo of the unsmoothed xi is recomputed in LDS instead of being written and re-read), and
 // nv_fuv_apply accumulates every star into a register before the single store.
 // Transcendentals come from nv_math.h so the CPU oracle reproduces them bit for bit.
+#include <string.h>
+#include <type_traits>
 #include "nv_common.h"
 #include "nv_math.h"
 #include "nv_divide.h"
@@ -54,30 +56,40 @@ __device__ __forceinline__ float ldf(const uint16_t* __restrict__ f, unsigned i)
   return (float)__builtin_bit_cast(_Float16, f[i]);
 }
 
-// Two z-neighbours (i, i+1) in one 8-byte load (4-byte aligned, which global loads allow).
-__device__ __forceinline__ float2 ld_pair(const float* __restrict__ f, unsigned i) {
-  typedef float f2u __attribute__((ext_vector_type(2), aligned(4)));
-  const f2u v = *reinterpret_cast<const f2u*>(f + i);
-  return make_float2(v.x, v.y);
-}
-
 // How a sampler turns a world coordinate into cell units: p / dx through the shared double reciprocal (the
 // identical correctly rounded quotient, nv_divide.h), or p * (1/dx) when dx is a power of two, which rounds the
-// same real number and so gives the same float.
-struct CellScale {
+// same real number and so gives the same float. A compile-time choice, so only one form is issued.
+struct ScaleDiv {
   double rdx;
-  float inv;   // 1/dx if dx = 2^e, else 0
-  __device__ __forceinline__ float operator()(float p) const { return inv != 0.0f ? p * inv : div_r(p, rdx); }
+  __device__ __forceinline__ float operator()(float p) const { return div_r(p, rdx); }
 };
+struct ScalePow2 {
+  float inv;
+  __device__ __forceinline__ float operator()(float p) const { return p * inv; }
+};
+
+// Loads at a cell index: OFF = uint32_t forms 32-bit byte offsets from the field's base (grids up to 2^30
+// cells, so that 4 * index < 2^32), uint64_t 64-bit ones.
+template <class OFF>
+__device__ __forceinline__ float ld1(const float* __restrict__ f, unsigned i) {
+  return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(f) + (OFF)i * 4u);
+}
+// two z-neighbours (i, i+1) in one 8-byte load (4-byte aligned, which global loads allow)
+template <class OFF>
+__device__ __forceinline__ float2 ld2(const float* __restrict__ f, unsigned i) {
+  typedef float f2u __attribute__((ext_vector_type(2), aligned(4)));
+  const f2u v = *reinterpret_cast<const f2u*>(reinterpret_cast<const char*>(f) + (OFF)i * 4u);
+  return make_float2(v.x, v.y);
+}
 
 // trilinear sample at world p, 0 outside [-1/2, n-1/2] in cell units (sample_n, nebula3d.py:195-226), with
 // 32-bit cell indices (grids up to 2^32 cells). PAIR (nz >= 2): the corners at ka = clamp(k0) and
 // kb = clamp(k0+1) come from one load of (kp, kp+1), kp = clamp(k0, 0, nz-2): k0 = -1 gives ka = kb = 0 (the
 // pair's first value), k0 = nz-1 gives ka = kb = nz-1 (its second), so every corner is the value the separate
 // clamped loads read.
-template <bool PAIR>
+template <bool PAIR, class OFF, class SC>
 __device__ __forceinline__ float sample_r(const float* __restrict__ f, int nx, int ny, int nz, float px, float py,
-                                          float pz, const CellScale& cs) {
+                                          float pz, const SC& cs) {
   const float gx = cs(px) - 0.5f, gy = cs(py) - 0.5f, gz = cs(pz) - 0.5f;
   if (gx < -0.5f || gy < -0.5f || gz < -0.5f) return 0.0f;
   if (gx > (float)nx - 0.5f || gy > (float)ny - 0.5f || gz > (float)nz - 0.5f) return 0.0f;
@@ -91,8 +103,8 @@ __device__ __forceinline__ float sample_r(const float* __restrict__ f, int nx, i
   if constexpr (PAIR) {
     const unsigned kp = clampi(k0, 0, nz - 2);
     const bool a_second = k0 > nz - 2, b_first = k0 < 0;
-    const float2 p00 = ld_pair(f, r00 + kp), p10 = ld_pair(f, r10 + kp);
-    const float2 p01 = ld_pair(f, r01 + kp), p11 = ld_pair(f, r11 + kp);
+    const float2 p00 = ld2<OFF>(f, r00 + kp), p10 = ld2<OFF>(f, r10 + kp);
+    const float2 p01 = ld2<OFF>(f, r01 + kp), p11 = ld2<OFF>(f, r11 + kp);
     c000 = a_second ? p00.y : p00.x;
     c100 = a_second ? p10.y : p10.x;
     c010 = a_second ? p01.y : p01.x;
@@ -103,8 +115,9 @@ __device__ __forceinline__ float sample_r(const float* __restrict__ f, int nx, i
     c111 = b_first ? p11.x : p11.y;
   } else {
     const unsigned ka = clampi(k0, 0, nz - 1), kb = clampi(k0 + 1, 0, nz - 1);
-    c000 = f[r00 + ka]; c100 = f[r10 + ka]; c010 = f[r01 + ka]; c110 = f[r11 + ka];
-    c001 = f[r00 + kb]; c101 = f[r10 + kb]; c011 = f[r01 + kb]; c111 = f[r11 + kb];
+    c000 = ld1<OFF>(f, r00 + ka); c100 = ld1<OFF>(f, r10 + ka); c010 = ld1<OFF>(f, r01 + ka);
+    c110 = ld1<OFF>(f, r11 + ka); c001 = ld1<OFF>(f, r00 + kb); c101 = ld1<OFF>(f, r10 + kb);
+    c011 = ld1<OFF>(f, r01 + kb); c111 = ld1<OFF>(f, r11 + kb);
   }
   const float c00 = c000 + (c100 - c000) * fx;
   const float c10 = c010 + (c110 - c010) * fx;
@@ -126,27 +139,35 @@ __device__ __forceinline__ bool ray_of_thread(long long t, int nu, int nphi, int
   return *iu < nu && *ip < nphi;
 }
 
-// cast_rays (nebula3d.py:247-270). Two samples per iteration: the corner loads of sample n+1 are in flight
-// while sample n is tested, and the column is still accumulated sample by sample in the reference order.
-// WIDE: a ray whose marched length (n1 - n0) dr exceeds 512 cells keeps its column in FP64 (the test
-// col + inc > Q and the interpolation of rf in double); shorter rays are unchanged.
-template <bool PAIR, bool WIDE>
-__global__ __launch_bounds__(256) void cast_rays_kernel(const float* __restrict__ rho, int nx, int ny, int nz,
-                                                        float sx, float sy, float sz, float Q, float dr,
-                                                        int nsteps, CellScale cs, float dx, float rcap, float bx,
-                                                        float by, float bz, float* __restrict__ rfront, int nu,
-                                                        int nphi) {
+// One ray-cast job: a star, its direction grid and the density grid it marches through.
+struct RayJob {
+  const float* rho;
+  float* rfront;
+  double rdx;        // 1/dx, correctly rounded (the shared reciprocal of the general cell scaling)
+  int nx, ny, nz, nsteps, nu, nphi, wide;
+  float sx, sy, sz, Q, dr, dx, rcap, bx, by, bz;
+  float inv;         // 1/dx when dx is a power of two, else 0
+};
+
+// cast_rays (nebula3d.py:247-270) for the ray of thread t of a job. Two samples per iteration: the corner loads
+// of sample n+1 are in flight while sample n is tested, and the column is still accumulated sample by sample in
+// the reference order. WIDE: a ray whose marched length (n1 - n0) dr exceeds 512 cells keeps its column in FP64
+// (the test col + inc > Q and the interpolation of rf in double); shorter rays are unchanged.
+template <bool PAIR, bool WIDE, class OFF, class SC>
+__device__ __forceinline__ void cast_one(const RayJob& J, const SC& cs, long long t) {
   int iu, ip;
-  if (!ray_of_thread((long long)blockIdx.x * blockDim.x + threadIdx.x, nu, nphi, &iu, &ip)) return;
+  if (!ray_of_thread(t, J.nu, J.nphi, &iu, &ip)) return;
+  const int nx = J.nx, ny = J.ny, nz = J.nz;
+  const float sx = J.sx, sy = J.sy, sz = J.sz, Q = J.Q, dr = J.dr;
   float d[3];
-  ray_dir(iu, ip, nu, nphi, d);
-  const float st[3] = {sx, sy, sz}, bm[3] = {bx, by, bz};
+  ray_dir(iu, ip, J.nu, J.nphi, d);
+  const float st[3] = {sx, sy, sz}, bm[3] = {J.bx, J.by, J.bz};
   float t0, t1;
   box_range(st, d, bm, &t0, &t1);
   const int n0 = (int)(t0 / dr);
-  const int n1 = min(nsteps, (int)(t1 / dr) + 2);
-  const bool wide = WIDE && (float)(n1 - n0) * dr > 512.0f * dx;
-  float col = 0.0f, rf = rcap;
+  const int n1 = min(J.nsteps, (int)(t1 / dr) + 2);
+  const bool wide = WIDE && (float)(n1 - n0) * dr > 512.0f * J.dx;
+  float col = 0.0f, rf = J.rcap;
   double col64 = 0.0;
   // one sample's threshold test; true when the front is found (rf set)
   auto step = [&](float r, float inc) -> bool {
@@ -168,13 +189,51 @@ __global__ __launch_bounds__(256) void cast_rays_kernel(const float* __restrict_
   for (int n = n0; n < n1; n += 2) {
     const float ra = ((float)n + 0.5f) * dr;
     const float rb = ((float)(n + 1) + 0.5f) * dr;
-    const float na = sample_r<PAIR>(rho, nx, ny, nz, sx + d[0] * ra, sy + d[1] * ra, sz + d[2] * ra, cs);
-    const float nb = sample_r<PAIR>(rho, nx, ny, nz, sx + d[0] * rb, sy + d[1] * rb, sz + d[2] * rb, cs);
+    const float na = sample_r<PAIR, OFF>(J.rho, nx, ny, nz, sx + d[0] * ra, sy + d[1] * ra, sz + d[2] * ra, cs);
+    const float nb = sample_r<PAIR, OFF>(J.rho, nx, ny, nz, sx + d[0] * rb, sy + d[1] * rb, sz + d[2] * rb, cs);
     if (step(ra, na * na * ra * ra * dr)) break;
     if (n + 1 >= n1) break;
     if (step(rb, nb * nb * rb * rb * dr)) break;
   }
-  rfront[(long long)iu * nphi + ip] = rf;
+  J.rfront[(long long)iu * J.nphi + ip] = rf;
+}
+
+template <bool PAIR, bool WIDE, class OFF, class SC>
+__global__ __launch_bounds__(256) void cast_rays_kernel(RayJob J, SC cs) {
+  cast_one<PAIR, WIDE, OFF, SC>(J, cs, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// blocks of 256 rays each job needs (64-ray direction patches, nebula ray_of_thread)
+__host__ __device__ inline long long ray_blocks(int nu, int nphi) {
+  return ((long long)((nu + 7) / 8) * ((nphi + 7) / 8) * 64 + 255) / 256;
+}
+
+// Many ray-cast jobs in one launch: block b belongs to the job s with start[s] <= b < start[s+1]. The cell
+// scaling and the FP64 column are per job (block-uniform branches); paired z loads and 32-bit offsets are shared.
+template <bool PAIR, class OFF>
+__global__ __launch_bounds__(256) void cast_rays_batch_kernel(const RayJob* __restrict__ jobs,
+                                                              const int* __restrict__ start, int n, int total) {
+  const int b = blockIdx.x;
+  if (b >= total) return;
+  int lo = 0, hi = n;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (start[mid] <= b) lo = mid;
+    else hi = mid;
+  }
+  const RayJob& J = jobs[lo];
+  const long long t = (long long)(b - start[lo]) * 256 + threadIdx.x;
+  if (J.inv != 0.0f) {
+    ScalePow2 sc;
+    sc.inv = J.inv;
+    if (J.wide) cast_one<PAIR, true, OFF>(J, sc, t);
+    else cast_one<PAIR, false, OFF>(J, sc, t);
+  } else {
+    ScaleDiv sc;
+    sc.rdx = J.rdx;
+    if (J.wide) cast_one<PAIR, true, OFF>(J, sc, t);
+    else cast_one<PAIR, false, OFF>(J, sc, t);
+  }
 }
 
 // cell centre seen from a star: r, u = cos(theta), phi in [0, 2pi) (nebula3d.py:284-290)
@@ -189,24 +248,72 @@ __device__ __forceinline__ void cell_sph(int i, int j, int k, float sx, float sy
   *phi_o = phi;
 }
 
+// phi index wrapped into [0, nphi): ((ip % nphi) + nphi) % nphi, with the common cases as compares (front_at
+// and fuv_at only produce ip in [-1, nphi])
+__device__ __forceinline__ int wrap_phi(int ip, int nphi) {
+  if (ip >= 0 && ip < nphi) return ip;
+  if (ip < 0 && ip >= -nphi) return ip + nphi;
+  if (ip >= nphi && ip < 2 * nphi) return ip - nphi;
+  return ((ip % nphi) + nphi) % nphi;
+}
+
+// 1 / (2 pi) as the shared double reciprocal of the float 2*PI_F (correctly rounded, so within the 2^-52 that
+// div_r needs): div_r(phi, R2PI) is phi / (2.0f * PI_F) bit for bit
+constexpr double R2PI = 1.0 / (double)(2.0f * PI_F);
+constexpr double R6 = 1.0 / 6.0;
+
+// row[pa] and row[pb] for pb = pa + 1 or the wrap pb = 0: one 8-byte load of (q, q+1), q = min(pa, nphi-2), when
+// nphi >= 2 (row[0] loaded on its own only at the wrap)
+__device__ __forceinline__ void phi_pair(const float* __restrict__ row, int pa, int pb, int nphi, float* va,
+                                         float* vb) {
+  if (nphi < 2) {
+    *va = row[pa];
+    *vb = row[pb];
+    return;
+  }
+  const int q = min(pa, nphi - 2);
+  typedef float f2u __attribute__((ext_vector_type(2), aligned(4)));
+  const f2u v = *reinterpret_cast<const f2u*>(row + q);
+  *va = pa == q ? v.x : v.y;
+  *vb = pb == q + 1 ? v.y : row[pb];
+}
+
 // bilinear front radius at direction (u, phi) (nebula3d.py:291-301, rf_at 273-278)
 __device__ __forceinline__ float front_at(const float* __restrict__ rfront, int nu, int nphi, float u, float phi) {
   const float fu = (u + 1.0f) * 0.5f * (float)nu - 0.5f;
-  const float fp = phi / (2.0f * PI_F) * (float)nphi - 0.5f;
+  const float fp = div_r(phi, R2PI) * (float)nphi - 0.5f;
   const int iu = (int)floorf(fu), ip = (int)floorf(fp);
   const float tu = fu - (float)iu, tp = fp - (float)ip;
   const int ua = clampi(iu, 0, nu - 1), ub = clampi(iu + 1, 0, nu - 1);
-  const int pa = ((ip % nphi) + nphi) % nphi, pb = (((ip + 1) % nphi) + nphi) % nphi;
-  const float* ra = rfront + (long long)ua * nphi;
-  const float* rb = rfront + (long long)ub * nphi;
-  const float av = ra[pa] * (1.0f - tp) + ra[pb] * tp;
-  const float bv = rb[pa] * (1.0f - tp) + rb[pb] * tp;
+  const int pa = wrap_phi(ip, nphi), pb = wrap_phi(ip + 1, nphi);
+  const float* ra = rfront + ua * nphi;
+  const float* rb = rfront + ub * nphi;
+  float ra0, ra1, rb0, rb1;
+  phi_pair(ra, pa, pb, nphi, &ra0, &ra1);
+  phi_pair(rb, pa, pb, nphi, &rb0, &rb1);
+  const float av = ra0 * (1.0f - tp) + ra1 * tp;
+  const float bv = rb0 * (1.0f - tp) + rb1 * tp;
   return av * (1.0f - tu) + bv * tu;
+}
+
+// x / d for one divisor and three numerators: the shared reciprocal where d is a positive normal float (the
+// identical IEEE quotients, nv_divide.h), IEEE division otherwise (a raw density of 0 or below the float range)
+__device__ __forceinline__ void div3(float a, float b, float c, float d, float* qa, float* qb, float* qc) {
+  if (d >= 1.17549435e-38f && d <= 3.40282347e+38f) {
+    const double r = recip_d(d);
+    *qa = div_r(a, r);
+    *qb = div_r(b, r);
+    *qc = div_r(c, r);
+  } else {
+    *qa = a / d;
+    *qb = b / d;
+    *qc = c / d;
+  }
 }
 
 // apply_front (nebula3d.py:281-307)
 __global__ __launch_bounds__(256) void apply_front_kernel(const float* __restrict__ rfront, int nu, int nphi,
-                                                          float sx, float sy, float sz, float dx, float width,
+                                                          float sx, float sy, float sz, float dx, double rwidth,
                                                           float* xi, float* dfront, int nx, int ny, int nz) {
   const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= (long long)nx * ny * nz) return;
@@ -216,7 +323,7 @@ __global__ __launch_bounds__(256) void apply_front_kernel(const float* __restric
   float r, u, phi;
   cell_sph(i, j, k, sx, sy, sz, dx, &r, &u, &phi);
   const float rf = front_at(rfront, nu, nphi, u, phi);
-  const float val = fminf(fmaxf((rf - r) / width + 0.5f, 0.0f), 1.0f);
+  const float val = fminf(fmaxf(div_r(rf - r, rwidth) + 0.5f, 0.0f), 1.0f);
   if (val > xi[q]) xi[q] = val;
   dfront[q] = fminf(dfront[q], r - rf);
 }
@@ -229,10 +336,10 @@ __global__ __launch_bounds__(256) void smooth_xi_kernel(const float* __restrict_
   const int k = (int)(q % nz);
   const long long ij = q / nz;
   const int j = (int)(ij % ny), i = (int)(ij / ny);
-  const float m = (src[cell(clampi(i - 1, 0, nx - 1), j, k, ny, nz)] + src[cell(clampi(i + 1, 0, nx - 1), j, k, ny, nz)] +
+  const float m = div_r(src[cell(clampi(i - 1, 0, nx - 1), j, k, ny, nz)] + src[cell(clampi(i + 1, 0, nx - 1), j, k, ny, nz)] +
                    src[cell(i, clampi(j - 1, 0, ny - 1), k, ny, nz)] + src[cell(i, clampi(j + 1, 0, ny - 1), k, ny, nz)] +
-                   src[cell(i, j, clampi(k - 1, 0, nz - 1), ny, nz)] + src[cell(i, j, clampi(k + 1, 0, nz - 1), ny, nz)]) /
-                  6.0f;
+                   src[cell(i, j, clampi(k - 1, 0, nz - 1), ny, nz)] + src[cell(i, j, clampi(k + 1, 0, nz - 1), ny, nz)],
+                  R6);
   dst[q] = 0.5f * src[q] + 0.5f * m;
 }
 
@@ -261,17 +368,20 @@ struct IonArgs {
   const void *f0, *f1, *f2, *f3;   // state for the CFL max: (rho, m) float32, or (rho, v) binary16
   float* smax;
   int nu, nphi, nstars, nx, ny, nz, ci;
-  float dx, width, cn2, ci2;
+  int tiles_k, tiles_j, tiles;     // tile grid: z tiles, y tiles, all tiles (x chunks of ci planes slowest)
+  float dx, cn2, ci2;
+  double rwidth;                   // 1 / width, correctly rounded (the shared reciprocal of the ramp width)
 };
 
+// One 64 z x 8 y tile of the fused ionize pass, marching over x chunk bi.
 template <bool SMAX, bool VEL>
-__global__ __launch_bounds__(INT_) void ionize_apply_kernel(IonArgs A) {
+__device__ __forceinline__ void ionize_tile(const IonArgs& A, int bk, int bj, int bi) {
   __shared__ float sx_[XR][XN];   // unsmoothed xi (max over stars of the front ramp)
   __shared__ float sd_[XR][XN];   // fp32 dfront (min over stars of r - rf)
   __shared__ float red[INT_ / 64];
   const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * IK + tx;
   const int nx = A.nx, ny = A.ny, nz = A.nz;
-  const int k0 = blockIdx.x * IK, j0 = blockIdx.y * IJ, i0 = blockIdx.z * A.ci;
+  const int k0 = bk * IK, j0 = bj * IJ, i0 = bi * A.ci;
   const int i1 = min(i0 + A.ci, nx);
   const int k = k0 + tx, j = j0 + ty;
   const bool active = k < nz && j < ny;
@@ -290,7 +400,7 @@ __global__ __launch_bounds__(INT_) void ionize_apply_kernel(IonArgs A) {
         float r, u, phi;
         cell_sph(ic, jc, kc, ssx, ssy, ssz, A.dx, &r, &u, &phi);
         const float rf = front_at(A.rfronts + st * tab, A.nu, A.nphi, u, phi);
-        const float val = fminf(fmaxf((rf - r) / A.width + 0.5f, 0.0f), 1.0f);
+        const float val = fminf(fmaxf(div_r(rf - r, A.rwidth) + 0.5f, 0.0f), 1.0f);
         if (val > x) x = val;
         df = fminf(df, r - rf);
       }
@@ -311,7 +421,7 @@ __global__ __launch_bounds__(INT_) void ionize_apply_kernel(IonArgs A) {
       const float* pm = sx_[(((i - 1) % XR) + XR) % XR];
       const float* pp = sx_[(((i + 1) % XR) + XR) % XR];
       // neighbours in the order of nebula3d.py:313-314
-      const float m = (pm[c] + pp[c] + pc[c - XK] + pc[c + XK] + pc[c - 1] + pc[c + 1]) / 6.0f;
+      const float m = div_r(pm[c] + pp[c] + pc[c - XK] + pc[c + XK] + pc[c - 1] + pc[c + 1], R6);
       const float xs = 0.5f * pc[c] + 0.5f * m;
       const long long o = cell(i, j, k, ny, nz);
       A.xi[o] = xs;
@@ -324,9 +434,10 @@ __global__ __launch_bounds__(INT_) void ionize_apply_kernel(IonArgs A) {
           s = ((fabsf(ldf((const uint16_t*)A.f1, o)) + fabsf(ldf((const uint16_t*)A.f2, o))) +
                fabsf(ldf((const uint16_t*)A.f3, o))) + sqrtf(A.cn2 + xs * (A.ci2 - A.cn2));
         } else {
-          const float rr = ((const float*)A.f0)[o];
-          s = ((fabsf(((const float*)A.f1)[o] / rr) + fabsf(((const float*)A.f2)[o] / rr)) +
-               fabsf(((const float*)A.f3)[o] / rr)) + sqrtf(A.cn2 + xs * (A.ci2 - A.cn2));
+          float v0, v1, v2;
+          div3(((const float*)A.f1)[o], ((const float*)A.f2)[o], ((const float*)A.f3)[o], ((const float*)A.f0)[o],
+               &v0, &v1, &v2);
+          s = ((fabsf(v0) + fabsf(v1)) + fabsf(v2)) + sqrtf(A.cn2 + xs * (A.ci2 - A.cn2));
         }
         tmax = fmaxf(tmax, s);
       }
@@ -335,12 +446,35 @@ __global__ __launch_bounds__(INT_) void ionize_apply_kernel(IonArgs A) {
   if constexpr (SMAX) block_max_atomic(tmax, red, A.smax);
 }
 
+template <bool SMAX, bool VEL>
+__global__ __launch_bounds__(INT_) void ionize_apply_kernel(IonArgs A) {
+  ionize_tile<SMAX, VEL>(A, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+// Many grids in one launch: tile t of the flat list belongs to member s with start[s] <= t < start[s+1].
+template <bool SMAX, bool VEL>
+__global__ __launch_bounds__(INT_) void ionize_apply_batch_kernel(const IonArgs* __restrict__ args,
+                                                                  const int* __restrict__ start, int n, int total) {
+  const int t = blockIdx.x;
+  if (t >= total) return;
+  int lo = 0, hi = n;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (start[mid] <= t) lo = mid;
+    else hi = mid;
+  }
+  const IonArgs& A = args[lo];
+  const int l = t - start[lo];
+  const int bk = l % A.tiles_k, bj = (l / A.tiles_k) % A.tiles_j, bi = l / (A.tiles_k * A.tiles_j);
+  ionize_tile<SMAX, VEL>(A, bk, bj, bi);
+}
+
 // ---------------------------------------------------------------------------------------- FUV
 // cast_fuv (nebula3d.py:320-352)
-template <bool PAIR>
+template <bool PAIR, class OFF, class SC>
 __global__ __launch_bounds__(256) void cast_fuv_kernel(const float* __restrict__ rho, const float* __restrict__ xi,
                                                        int nx, int ny, int nz, float sx, float sy, float sz,
-                                                       float kappa, float dr, int nsteps, CellScale cs, float binr,
+                                                       float kappa, float dr, int nsteps, SC cs, float binr,
                                                        float bx, float by, float bz, float* __restrict__ att, int nr,
                                                        int nu, int nphi) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -359,8 +493,8 @@ __global__ __launch_bounds__(256) void cast_fuv_kernel(const float* __restrict__
   for (int n = n0; n < n1; ++n) {
     const float r = ((float)n + 0.5f) * dr;
     const float qx = sx + d[0] * r, qy = sy + d[1] * r, qz = sz + d[2] * r;
-    const float n_tot = sample_r<PAIR>(rho, nx, ny, nz, qx, qy, qz, cs);
-    const float x_i = sample_r<PAIR>(xi, nx, ny, nz, qx, qy, qz, cs);
+    const float n_tot = sample_r<PAIR, OFF>(rho, nx, ny, nz, qx, qy, qz, cs);
+    const float x_i = sample_r<PAIR, OFF>(xi, nx, ny, nz, qx, qy, qz, cs);
     tau = tau + kappa * n_tot * (1.0f - 0.92f * x_i) * dr;
     const int nb = (int)(r / binr);
     if (nb > ib) {
@@ -374,33 +508,36 @@ __global__ __launch_bounds__(256) void cast_fuv_kernel(const float* __restrict__
   for (int b = ib; b < nr; ++b) att[(long long)b * nang + t] = e;
 }
 
-// trilinear att at (r/binr - 1/2, u, phi) divided by max(r^2, 4e-4) (nebula3d.py:362-387)
+// trilinear att at (r/binr - 1/2, u, phi) (nebula3d.py:362-387); rbinr = 1/binr as the shared reciprocal
 __device__ __forceinline__ float fuv_at(const float* __restrict__ att, int nr, int nu, int nphi, float r, float u,
-                                        float phi, float binr) {
-  const float fr = r / binr - 0.5f;
+                                        float phi, double rbinr) {
+  const float fr = div_r(r, rbinr) - 0.5f;
   const float fu = (u + 1.0f) * 0.5f * (float)nu - 0.5f;
-  const float fp = phi / (2.0f * PI_F) * (float)nphi - 0.5f;
+  const float fp = div_r(phi, R2PI) * (float)nphi - 0.5f;
   const int ir = (int)floorf(fr), iu = (int)floorf(fu), ip = (int)floorf(fp);
   const float tr = fr - (float)ir, tu = fu - (float)iu, tp = fp - (float)ip;
+  const int pa = wrap_phi(ip, nphi), pb = wrap_phi(ip + 1, nphi);
   float acc = 0.0f;
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
-    for (int b = 0; b < 2; ++b)
+    for (int b = 0; b < 2; ++b) {
+      const int rr = clampi(ir + a, 0, nr - 1), uu = clampi(iu + b, 0, nu - 1);
+      float v0, v1;
+      phi_pair(att + ((long long)rr * nu + uu) * nphi, pa, pb, nphi, &v0, &v1);
 #pragma unroll
       for (int c = 0; c < 2; ++c) {
         const float w = (a == 0 ? 1.0f - tr : tr) * (b == 0 ? 1.0f - tu : tu) * (c == 0 ? 1.0f - tp : tp);
-        const int rr = clampi(ir + a, 0, nr - 1), uu = clampi(iu + b, 0, nu - 1);
-        const int pp = (((ip + c) % nphi) + nphi) % nphi;
-        acc = acc + w * att[((long long)rr * nu + uu) * nphi + pp];
+        acc = acc + w * (c == 0 ? v0 : v1);
       }
+    }
   return acc;
 }
 
 // apply_fuv (nebula3d.py:355-388)
 __global__ __launch_bounds__(256) void apply_fuv_kernel(const float* __restrict__ att, int nr, int nu, int nphi,
                                                         float sx, float sy, float sz, float lum, float dx,
-                                                        float binr, float* g0, int nx, int ny, int nz) {
+                                                        double rbinr, float* g0, int nx, int ny, int nz) {
   const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= (long long)nx * ny * nz) return;
   const int k = (int)(q % nz);
@@ -408,7 +545,7 @@ __global__ __launch_bounds__(256) void apply_fuv_kernel(const float* __restrict_
   const int j = (int)(ij % ny), i = (int)(ij / ny);
   float r, u, phi;
   cell_sph(i, j, k, sx, sy, sz, dx, &r, &u, &phi);
-  const float acc = fuv_at(att, nr, nu, nphi, r, u, phi, binr);
+  const float acc = fuv_at(att, nr, nu, nphi, r, u, phi, rbinr);
   const float r2 = fmaxf(r * r, 0.0004f);
   g0[q] = g0[q] + lum * acc / r2;
 }
@@ -417,7 +554,7 @@ __global__ __launch_bounds__(256) void apply_fuv_kernel(const float* __restrict_
 __global__ __launch_bounds__(256) void fuv_apply_kernel(const float* __restrict__ atts, int nr, int nu, int nphi,
                                                         const float* __restrict__ star_xyz,
                                                         const float* __restrict__ lums, int nstars, float dx,
-                                                        float binr, uint16_t* g0_h, float* g0_f, int nx, int ny,
+                                                        double rbinr, uint16_t* g0_h, float* g0_f, int nx, int ny,
                                                         int nz) {
   const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= (long long)nx * ny * nz) return;
@@ -429,7 +566,7 @@ __global__ __launch_bounds__(256) void fuv_apply_kernel(const float* __restrict_
   for (int s = 0; s < nstars; ++s) {
     float r, u, phi;
     cell_sph(i, j, k, star_xyz[3 * s], star_xyz[3 * s + 1], star_xyz[3 * s + 2], dx, &r, &u, &phi);
-    const float acc = fuv_at(atts + s * tab, nr, nu, nphi, r, u, phi, binr);
+    const float acc = fuv_at(atts + s * tab, nr, nu, nphi, r, u, phi, rbinr);
     const float r2 = fmaxf(r * r, 0.0004f);
     g = g + lums[s] * acc / r2;
   }
@@ -443,11 +580,35 @@ inline int blocks_for(long long n, int bs) { return (int)((n + bs - 1) / bs); }
 
 using namespace nv;
 
-static CellScale cell_scale(float dx) {
-  CellScale cs;
-  cs.rdx = 1.0 / (double)dx;   // correctly rounded: |rdx dx - 1| <= 2^-53, inside the bound div_r needs
-  cs.inv = pow2_reciprocal(dx);
-  return cs;
+// Calls f(pair, offset, scale) with the sampler's compile-time forms for this grid: paired z loads when nz >= 2,
+// 32-bit byte offsets up to 2^30 cells, and the power-of-two or shared-reciprocal cell scaling.
+template <class F>
+static void dispatch_sampler(int nz, long long cells, float dx, F&& f) {
+  const float inv = pow2_reciprocal(dx);
+  ScaleDiv sd;
+  sd.rdx = 1.0 / (double)dx;   // correctly rounded: |rdx dx - 1| <= 2^-53, inside the bound div_r needs
+  ScalePow2 sp;
+  sp.inv = inv;
+  auto by_scale = [&](auto pair, auto off) {
+    if (inv != 0.0f) f(pair, off, sp);
+    else f(pair, off, sd);
+  };
+  auto by_off = [&](auto pair) {
+    if (cells <= (1LL << 30)) by_scale(pair, uint32_t{});
+    else by_scale(pair, uint64_t{});
+  };
+  if (nz >= 2) by_off(std::true_type{});
+  else by_off(std::false_type{});
+}
+
+static void fill_ray_job(const float* rho, int nx, int ny, int nz, float sx, float sy, float sz, float Q, float dr,
+                         int nsteps, float dx, float rcap, float bx, float by, float bz, float* rfront, int nu,
+                         int nphi, bool wide, RayJob& J) {
+  J.rho = rho; J.rfront = rfront;
+  J.rdx = 1.0 / (double)dx;
+  J.nx = nx; J.ny = ny; J.nz = nz; J.nsteps = nsteps; J.nu = nu; J.nphi = nphi; J.wide = wide ? 1 : 0;
+  J.sx = sx; J.sy = sy; J.sz = sz; J.Q = Q; J.dr = dr; J.dx = dx; J.rcap = rcap; J.bx = bx; J.by = by; J.bz = bz;
+  J.inv = pow2_reciprocal(dx);
 }
 
 static int cast_rays_launch(const char* name, bool wide, const float* rho, int nx, int ny, int nz, float sx,
@@ -458,13 +619,16 @@ static int cast_rays_launch(const char* name, bool wide, const float* rho, int n
   NV_REQUIRE(nx > 0 && ny > 0 && nz > 0 && nu > 0 && nphi > 0, "%s: bad sizes", name);
   NV_REQUIRE(dr > 0.0f && dx > 0.0f, "%s: dr and dx must be > 0", name);
   NV_REQUIRE((long long)nx * ny * nz < (1LL << 32), "%s: grids above 2^32 cells are not supported", name);
-  const long long n = (long long)((nu + 7) / 8) * ((nphi + 7) / 8) * 64;
-  const CellScale cs = cell_scale(dx);
-  const bool pair = nz >= 2;
-  auto kern = pair ? (wide ? cast_rays_kernel<true, true> : cast_rays_kernel<true, false>)
-                   : (wide ? cast_rays_kernel<false, true> : cast_rays_kernel<false, false>);
-  hipLaunchKernelGGL(kern, dim3(blocks_for(n, 256)), dim3(256), 0, as_stream(stream), rho, nx, ny, nz, sx, sy, sz, Q,
-                     dr, nsteps, cs, dx, rcap, bx, by, bz, rfront, nu, nphi);
+  RayJob J;
+  fill_ray_job(rho, nx, ny, nz, sx, sy, sz, Q, dr, nsteps, dx, rcap, bx, by, bz, rfront, nu, nphi, wide, J);
+  const long long blocks = ray_blocks(nu, nphi);
+  dispatch_sampler(nz, (long long)nx * ny * nz, dx, [&](auto pair, auto off, auto sc) {
+    constexpr bool P = decltype(pair)::value;
+    using O = decltype(off);
+    using SC = decltype(sc);
+    auto kern = wide ? cast_rays_kernel<P, true, O, SC> : cast_rays_kernel<P, false, O, SC>;
+    hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(256), 0, as_stream(stream), J, sc);
+  });
   return check_launch(name);
 }
 
@@ -487,9 +651,10 @@ extern "C" int nv_apply_front(const float* rfront, int nu, int nphi, float sx, f
   clear_error();
   NV_REQUIRE(rfront && xi && dfront, "nv_apply_front: null pointer");
   NV_REQUIRE(nx > 0 && ny > 0 && nz > 0 && nu > 0 && nphi > 0, "nv_apply_front: bad sizes");
+  NV_REQUIRE(width >= 1.17549435e-38f && width <= 3.40282347e+38f, "nv_apply_front: width must be a positive normal float");
   const long long n = (long long)nx * ny * nz;
   hipLaunchKernelGGL(apply_front_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, as_stream(stream), rfront, nu, nphi,
-                     sx, sy, sz, dx, width, xi, dfront, nx, ny, nz);
+                     sx, sy, sz, dx, 1.0 / (double)width, xi, dfront, nx, ny, nz);
   return check_launch("nv_apply_front");
 }
 
@@ -521,23 +686,39 @@ extern "C" int nv_fill_f32(float* dst, float v, long long n, void* stream) {
   return check_launch("nv_fill_f32");
 }
 
+static int fill_ionize(const char* name, const float* rfronts, int nu, int nphi, const float* star_xyz, int nstars,
+                       float dx, float width, float* xi, uint16_t* dfront_h, float* dfront_f, int nx, int ny, int nz,
+                       const void* f0, const void* f1, const void* f2, const void* f3, float cn2, float ci2,
+                       float* smax_out, long long jk_total, IonArgs& A) {
+  NV_REQUIRE(xi && (nstars == 0 || (rfronts && star_xyz)), "%s: null pointer", name);
+  NV_REQUIRE((dfront_h != nullptr) != (dfront_f != nullptr), "%s: pass exactly one dfront buffer", name);
+  NV_REQUIRE(nx > 0 && ny > 0 && nz > 0 && nu > 0 && nphi > 0 && nstars >= 0, "%s: bad sizes", name);
+  NV_REQUIRE(!smax_out || (f0 && f1 && f2 && f3), "%s: smax needs the density and momentum fields", name);
+  NV_REQUIRE(width >= 1.17549435e-38f && width <= 3.40282347e+38f, "%s: width must be a positive normal float", name);
+  A.rfronts = rfronts; A.star_xyz = star_xyz; A.xi = xi; A.df_h = dfront_h; A.df_f = dfront_f;
+  A.f0 = f0; A.f1 = f1; A.f2 = f2; A.f3 = f3; A.smax = smax_out;
+  A.nu = nu; A.nphi = nphi; A.nstars = nstars; A.nx = nx; A.ny = ny; A.nz = nz;
+  A.dx = dx; A.cn2 = cn2; A.ci2 = ci2;
+  A.rwidth = 1.0 / (double)width;
+  A.ci = (int)std::min<long long>(64, std::max<long long>(4, (long long)nx * jk_total / 2048));
+  A.tiles_k = (nz + IK - 1) / IK;
+  A.tiles_j = (ny + IJ - 1) / IJ;
+  A.tiles = A.tiles_k * A.tiles_j * ((nx + A.ci - 1) / A.ci);
+  return NV_OK;
+}
+
+static long long ion_jk(int ny, int nz) { return (long long)((nz + IK - 1) / IK) * ((ny + IJ - 1) / IJ); }
+
 static int ionize_apply_launch(const char* name, bool vel, const float* rfronts, int nu, int nphi,
                                const float* star_xyz, int nstars, float dx, float width, float* xi, uint16_t* dfront_h,
                                float* dfront_f, int nx, int ny, int nz, const void* f0, const void* f1,
                                const void* f2, const void* f3, float cn2, float ci2, float* smax_out, void* stream) {
   clear_error();
-  NV_REQUIRE(xi && (nstars == 0 || (rfronts && star_xyz)), "%s: null pointer", name);
-  NV_REQUIRE((dfront_h != nullptr) != (dfront_f != nullptr), "%s: pass exactly one dfront buffer", name);
-  NV_REQUIRE(nx > 0 && ny > 0 && nz > 0 && nu > 0 && nphi > 0 && nstars >= 0, "%s: bad sizes", name);
-  NV_REQUIRE(!smax_out || (f0 && f1 && f2 && f3), "%s: smax needs the density and momentum fields", name);
   IonArgs A;
-  A.rfronts = rfronts; A.star_xyz = star_xyz; A.xi = xi; A.df_h = dfront_h; A.df_f = dfront_f;
-  A.f0 = f0; A.f1 = f1; A.f2 = f2; A.f3 = f3; A.smax = smax_out;
-  A.nu = nu; A.nphi = nphi; A.nstars = nstars; A.nx = nx; A.ny = ny; A.nz = nz;
-  A.dx = dx; A.width = width; A.cn2 = cn2; A.ci2 = ci2;
-  const long long tiles_jk = (long long)((nz + IK - 1) / IK) * ((ny + IJ - 1) / IJ);
-  A.ci = (int)std::min<long long>(64, std::max<long long>(4, (long long)nx * tiles_jk / 2048));
-  dim3 grid((nz + IK - 1) / IK, (ny + IJ - 1) / IJ, (nx + A.ci - 1) / A.ci);
+  const int rc = fill_ionize(name, rfronts, nu, nphi, star_xyz, nstars, dx, width, xi, dfront_h, dfront_f, nx, ny, nz,
+                             f0, f1, f2, f3, cn2, ci2, smax_out, ion_jk(ny, nz), A);
+  if (rc != NV_OK) return rc;
+  dim3 grid(A.tiles_k, A.tiles_j, (nx + A.ci - 1) / A.ci);
   auto kern = !smax_out ? ionize_apply_kernel<false, false>
                         : (vel ? ionize_apply_kernel<true, true> : ionize_apply_kernel<true, false>);
   hipLaunchKernelGGL(kern, grid, dim3(IK, IJ), 0, as_stream(stream), A);
@@ -569,10 +750,12 @@ extern "C" int nv_cast_fuv(const float* rho, const float* xi, int nx, int ny, in
   NV_REQUIRE(dr > 0.0f && dx > 0.0f && binr > 0.0f, "nv_cast_fuv: dr, dx, binr must be > 0");
   NV_REQUIRE((long long)nx * ny * nz < (1LL << 32), "nv_cast_fuv: grids above 2^32 cells are not supported");
   const long long n = (long long)nu * nphi;
-  const CellScale cs = cell_scale(dx);
-  auto kern = nz >= 2 ? cast_fuv_kernel<true> : cast_fuv_kernel<false>;
-  hipLaunchKernelGGL(kern, dim3(blocks_for(n, 256)), dim3(256), 0, as_stream(stream), rho, xi, nx, ny, nz, sx, sy, sz,
-                     kappa, dr, nsteps, cs, binr, bx, by, bz, att, nr, nu, nphi);
+  dispatch_sampler(nz, (long long)nx * ny * nz, dx, [&](auto pair, auto off, auto sc) {
+    constexpr bool P = decltype(pair)::value;
+    hipLaunchKernelGGL((cast_fuv_kernel<P, decltype(off), decltype(sc)>), dim3(blocks_for(n, 256)), dim3(256), 0,
+                       as_stream(stream), rho, xi, nx, ny, nz, sx, sy, sz, kappa, dr, nsteps, sc, binr, bx, by, bz,
+                       att, nr, nu, nphi);
+  });
   return check_launch("nv_cast_fuv");
 }
 
@@ -581,9 +764,10 @@ extern "C" int nv_apply_fuv(const float* att, int nr, int nu, int nphi, float sx
   clear_error();
   NV_REQUIRE(att && g0, "nv_apply_fuv: null pointer");
   NV_REQUIRE(nx > 0 && ny > 0 && nz > 0 && nr > 0 && nu > 0 && nphi > 0, "nv_apply_fuv: bad sizes");
+  NV_REQUIRE(binr >= 1.17549435e-38f && binr <= 3.40282347e+38f, "nv_apply_fuv: binr must be a positive normal float");
   const long long n = (long long)nx * ny * nz;
   hipLaunchKernelGGL(apply_fuv_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, as_stream(stream), att, nr, nu, nphi,
-                     sx, sy, sz, lum, dx, binr, g0, nx, ny, nz);
+                     sx, sy, sz, lum, dx, 1.0 / (double)binr, g0, nx, ny, nz);
   return check_launch("nv_apply_fuv");
 }
 
@@ -594,8 +778,137 @@ extern "C" int nv_fuv_apply(const float* atts, int nr, int nu, int nphi, const f
   NV_REQUIRE(nstars == 0 || (atts && star_xyz && lums), "nv_fuv_apply: null pointer");
   NV_REQUIRE((g0_h != nullptr) != (g0_f != nullptr), "nv_fuv_apply: pass exactly one g0 buffer");
   NV_REQUIRE(nx > 0 && ny > 0 && nz > 0 && nr > 0 && nu > 0 && nphi > 0 && nstars >= 0, "nv_fuv_apply: bad sizes");
+  NV_REQUIRE(binr >= 1.17549435e-38f && binr <= 3.40282347e+38f, "nv_fuv_apply: binr must be a positive normal float");
   const long long n = (long long)nx * ny * nz;
   hipLaunchKernelGGL(fuv_apply_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, as_stream(stream), atts, nr, nu, nphi,
-                     star_xyz, lums, nstars, dx, binr, g0_h, g0_f, nx, ny, nz);
+                     star_xyz, lums, nstars, dx, 1.0 / (double)binr, g0_h, g0_f, nx, ny, nz);
   return check_launch("nv_fuv_apply");
+}
+
+// ---------------------------------------------------------------------------------------- batch plans
+// Plan layout (host blob, uploaded once): a 64-byte header, the jobs' kernel arguments, then start[0..n].
+namespace nv {
+struct RayBatchHeader {
+  uint32_t magic;
+  int n, total, pair, off32;
+  int pad[11];
+};
+static_assert(sizeof(RayBatchHeader) == 64, "plan header layout");
+constexpr uint32_t kRayMagic = 0x4e565231u;   // "NVR1"
+constexpr uint32_t kIonMagic = 0x4e564931u;   // "NVI1"
+
+template <class T>
+static long long batch_bytes(int n) {
+  return (long long)sizeof(RayBatchHeader) + (long long)n * (long long)sizeof(T) + 4LL * (n + 1);
+}
+}  // namespace nv
+
+extern "C" long long nv_cast_rays_batch_plan_size(int n) { return n > 0 ? batch_bytes<RayJob>(n) : -1; }
+
+extern "C" int nv_cast_rays_batch_plan(const NvRayDesc* jobs, int n, void* plan, long long plan_bytes) {
+  clear_error();
+  NV_REQUIRE(jobs && plan && n > 0, "nv_cast_rays_batch_plan: need n > 0 jobs and a plan buffer");
+  NV_REQUIRE(plan_bytes >= batch_bytes<RayJob>(n), "nv_cast_rays_batch_plan: plan needs %lld bytes",
+             batch_bytes<RayJob>(n));
+  RayBatchHeader H = {};
+  RayJob* out = reinterpret_cast<RayJob*>(static_cast<char*>(plan) + sizeof(H));
+  int* start = reinterpret_cast<int*>(out + n);
+  long long total = 0;
+  bool pair = true, off32 = true;
+  for (int s = 0; s < n; ++s) {
+    const NvRayDesc& d = jobs[s];
+    NV_REQUIRE(d.rho && d.rfront, "nv_cast_rays_batch_plan: job %d: null pointer", s);
+    NV_REQUIRE(d.nx > 0 && d.ny > 0 && d.nz > 0 && d.nu > 0 && d.nphi > 0, "nv_cast_rays_batch_plan: job %d: bad sizes",
+               s);
+    NV_REQUIRE(d.dr > 0.0f && d.dx > 0.0f, "nv_cast_rays_batch_plan: job %d: dr and dx must be > 0", s);
+    const long long cells = (long long)d.nx * d.ny * d.nz;
+    NV_REQUIRE(cells < (1LL << 32), "nv_cast_rays_batch_plan: job %d: grids above 2^32 cells are not supported", s);
+    pair = pair && d.nz >= 2;
+    off32 = off32 && cells <= (1LL << 30);
+    RayJob J;
+    fill_ray_job(d.rho, d.nx, d.ny, d.nz, d.sx, d.sy, d.sz, d.Q, d.dr, d.nsteps, d.dx, d.rcap, d.bx, d.by, d.bz,
+                 d.rfront, d.nu, d.nphi, d.wide != 0, J);
+    memcpy(&out[s], &J, sizeof(J));
+    start[s] = (int)total;
+    total += ray_blocks(d.nu, d.nphi);
+    NV_REQUIRE(total < (1LL << 31) - 1, "nv_cast_rays_batch_plan: too many rays");
+  }
+  start[n] = (int)total;
+  H.magic = kRayMagic;
+  H.n = n;
+  H.total = (int)total;
+  H.pair = pair ? 1 : 0;
+  H.off32 = off32 ? 1 : 0;
+  memcpy(plan, &H, sizeof(H));
+  return NV_OK;
+}
+
+extern "C" int nv_cast_rays_batch(const void* plan_host, const void* plan_dev, void* stream) {
+  clear_error();
+  NV_REQUIRE(plan_host && plan_dev, "nv_cast_rays_batch: null plan");
+  RayBatchHeader H;
+  memcpy(&H, plan_host, sizeof(H));
+  NV_REQUIRE(H.magic == kRayMagic && H.n > 0, "nv_cast_rays_batch: not a ray batch plan");
+  const RayJob* jobs = reinterpret_cast<const RayJob*>(static_cast<const char*>(plan_dev) + sizeof(H));
+  const int* start = reinterpret_cast<const int*>(jobs + H.n);
+  auto kern = H.pair ? (H.off32 ? cast_rays_batch_kernel<true, uint32_t> : cast_rays_batch_kernel<true, uint64_t>)
+                     : (H.off32 ? cast_rays_batch_kernel<false, uint32_t> : cast_rays_batch_kernel<false, uint64_t>);
+  hipLaunchKernelGGL(kern, dim3(H.total), dim3(256), 0, as_stream(stream), jobs, start, H.n, H.total);
+  return check_launch("nv_cast_rays_batch");
+}
+
+extern "C" long long nv_ionize_apply_batch_plan_size(int n) { return n > 0 ? batch_bytes<IonArgs>(n) : -1; }
+
+extern "C" int nv_ionize_apply_batch_plan(const NvIonDesc* members, int n, int vel, void* plan, long long plan_bytes) {
+  clear_error();
+  NV_REQUIRE(members && plan && n > 0, "nv_ionize_apply_batch_plan: need n > 0 members and a plan buffer");
+  NV_REQUIRE(plan_bytes >= batch_bytes<IonArgs>(n), "nv_ionize_apply_batch_plan: plan needs %lld bytes",
+             batch_bytes<IonArgs>(n));
+  long long jk_total = 0;
+  for (int s = 0; s < n; ++s) {
+    NV_REQUIRE(members[s].ny > 0 && members[s].nz > 0, "nv_ionize_apply_batch_plan: member %d: bad grid", s);
+    jk_total += ion_jk(members[s].ny, members[s].nz);
+  }
+  RayBatchHeader H = {};
+  IonArgs* out = reinterpret_cast<IonArgs*>(static_cast<char*>(plan) + sizeof(H));
+  int* start = reinterpret_cast<int*>(out + n);
+  const bool sm = members[0].smax_out != nullptr;
+  long long total = 0;
+  char name[64];
+  for (int s = 0; s < n; ++s) {
+    const NvIonDesc& d = members[s];
+    snprintf(name, sizeof(name), "nv_ionize_apply_batch_plan: member %d", s);
+    NV_REQUIRE((d.smax_out != nullptr) == sm, "%s: smax_out must be set for all members or none", name);
+    IonArgs A;
+    const int rc = fill_ionize(name, d.rfronts, d.nu, d.nphi, d.star_xyz, d.nstars, d.dx, d.width, d.xi, d.dfront_h,
+                               d.dfront_f, d.nx, d.ny, d.nz, d.state[0], d.state[1], d.state[2], d.state[3], d.cn2,
+                               d.ci2, d.smax_out, jk_total, A);
+    if (rc != NV_OK) return rc;
+    memcpy(&out[s], &A, sizeof(A));
+    start[s] = (int)total;
+    total += A.tiles;
+    NV_REQUIRE(total < (1LL << 30), "nv_ionize_apply_batch_plan: more than 2^30 tiles");
+  }
+  start[n] = (int)total;
+  H.magic = kIonMagic;
+  H.n = n;
+  H.total = (int)total;
+  H.pair = sm ? 1 : 0;     // reused: the CFL reduction is on
+  H.off32 = vel ? 1 : 0;   // reused: binary16-velocity state
+  memcpy(plan, &H, sizeof(H));
+  return NV_OK;
+}
+
+extern "C" int nv_ionize_apply_batch(const void* plan_host, const void* plan_dev, void* stream) {
+  clear_error();
+  NV_REQUIRE(plan_host && plan_dev, "nv_ionize_apply_batch: null plan");
+  RayBatchHeader H;
+  memcpy(&H, plan_host, sizeof(H));
+  NV_REQUIRE(H.magic == kIonMagic && H.n > 0, "nv_ionize_apply_batch: not an ionize batch plan");
+  const IonArgs* args = reinterpret_cast<const IonArgs*>(static_cast<const char*>(plan_dev) + sizeof(H));
+  const int* start = reinterpret_cast<const int*>(args + H.n);
+  auto kern = !H.pair ? ionize_apply_batch_kernel<false, false>
+                      : (H.off32 ? ionize_apply_batch_kernel<true, true> : ionize_apply_batch_kernel<true, false>);
+  hipLaunchKernelGGL(kern, dim3(H.total), dim3(IK, IJ), 0, as_stream(stream), args, start, H.n, H.total);
+  return check_launch("nv_ionize_apply_batch");
 }

@@ -17,6 +17,8 @@ The workloads are synthetic seeded scenes, one per benchmark configuration:
   wall   - 512^3 half-space wall n=100 (tanh edge 4 cells) times lognormal sigma 0.4 (seed 2) plus 30 globules
            n=500, radius U[4, 12] cells; one star 0.4 above the face with Q = 0.4^3/3. 300 steps.
   torus_jet - the torus scene at 1024^3 (seed 3) with a +-z jet on the star. 200 steps.
+  batch  - 64 independent members of uneven size (batch_setups): torus or sheet, N in {48..160}, swept Q and
+           wind. 200 steps.
 The random fields use the Gaussian-random-field recipe of nebula3d.grf3 (nebula3d.py:391-401): amplitude
 k^(-slope/2) times complex normals from numpy.random.default_rng(seed), inverse real FFT, standardised; and
 the lognormal map exp(sigma g - sigma^2/2) (nebula3d.py:427).
@@ -210,6 +212,25 @@ CONFIGS = {
     2: ("wall", 512, 2),
     3: ("torus_jet", 1024, 3),
 }
+
+
+# the uneven batch: many independent nebulae (sweeps over luminosity, wind and seed), member i from seed
+# base + i: grid N in {48, 64, 96, 128, 160} (E[N^3] = 1.49 M cells), torus for even seeds and sheet for odd,
+# Q_main ~ U[0.3, 3] Q0, wind ~ U[0, 1]; 200 steps
+BATCH_SIZES = (48, 64, 96, 128, 160)
+
+
+def batch_setups(count=64, base_seed=100, sizes=BATCH_SIZES):
+    out = []
+    for i in range(count):
+        seed = base_seed + i
+        rng = np.random.default_rng(seed)
+        n = int(rng.choice(sizes))
+        params = dict(q_main=float(Q0 * rng.uniform(0.3, 3.0)), wind=float(rng.uniform(0.0, 1.0)))
+        st = make_setup("torus" if seed % 2 == 0 else "sheet", n, seed, params)
+        st.steps = 200
+        out.append(st)
+    return out
 
 
 def make_setup(scenario, n=None, seed=None, params=None):

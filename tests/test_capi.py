@@ -16,7 +16,7 @@ def declared():
     src = open(HEADER).read()
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
     out = {}
-    for m in re.finditer(r"\b(?:int|const char\*)\s+(nv_\w+)\s*\(([^)]*)\)\s*;", src):
+    for m in re.finditer(r"\b(?:int|long long|const char\*)\s+(nv_\w+)\s*\(([^)]*)\)\s*;", src):
         args = m.group(2).strip()
         out[m.group(1)] = 0 if args in ("", "void") else args.count(",") + 1
     return out
@@ -41,6 +41,82 @@ def test_every_declared_symbol_is_exported_and_bound(lib):
 
 def test_abi_version(lib):
     assert lib.nv_abi_version() == 1
+
+
+def struct_fields(name):
+    """(type, name) of each member of a typedef struct in the header, arrays expanded as name[i]"""
+    src = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    body = re.search(r"typedef struct %s \{(.*?)\} %s;" % (name, name), src, re.S).group(1)
+    out = []
+    for decl in body.split(";"):
+        decl = decl.strip()
+        if not decl:
+            continue
+        m = re.match(r"(const\s+)?(\w+)(\s*\*)?\s*(.*)", decl)
+        base = m.group(2) + ("*" if m.group(3) or "*" in m.group(4) else "")
+        for item in m.group(4).split(","):
+            item = item.strip().lstrip("*").strip()
+            arr = re.match(r"(\w+)\[(\d+)\]", item)
+            if arr:
+                out += [(base, f"{arr.group(1)}[{i}]") for i in range(int(arr.group(2)))]
+            else:
+                out.append((base, item))
+    return out
+
+
+@pytest.mark.parametrize("name", ["NvStageDesc", "NvRayDesc", "NvIonDesc"])
+def test_descriptor_structs_match_header(name):
+    """the ctypes mirrors have the header's members in order, with pointer/int/float kinds and C sizes"""
+    cls = getattr(_lib, name)
+    hdr = struct_fields(name)
+    py = []
+    for fname, ftype in cls._fields_:
+        if issubclass(ftype, ctypes.Array):
+            py += [(ftype._type_, f"{fname}[{i}]") for i in range(ftype._length_)]
+        else:
+            py.append((ftype, fname))
+    assert [n for _, n in py] == [n for _, n in hdr]
+    kinds = {"float": ctypes.c_float, "int": ctypes.c_int}
+    for (pt, pn), (ht, hn) in zip(py, hdr):
+        if ht.endswith("*"):
+            assert pt in (ctypes.c_void_p,), (pn, pt)
+        else:
+            assert pt is kinds[ht], (pn, ht, pt)
+
+
+def test_descriptor_layout_matches_c(tmp_path):
+    """sizeof and every offsetof of the descriptor structs as gcc lays out the header, against the ctypes mirrors"""
+    import subprocess
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-o", str(exe), os.path.join(ROOT, "tests", "csrc", "struct_layout.c")],
+                   check=True)
+    c = dict(line.split() for line in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                     check=True).stdout.splitlines())
+    for name in ("NvStageDesc", "NvRayDesc", "NvIonDesc"):
+        cls = getattr(_lib, name)
+        assert ctypes.sizeof(cls) == int(c[name]), name
+        for fname, _ in cls._fields_:
+            assert getattr(cls, fname).offset == int(c[f"{name}.{fname}"]), (name, fname)
+
+
+def test_batch_plans_validate(lib):
+    """plans refuse empty batches, short buffers and bad members before any launch"""
+    assert lib.nv_hydro_stage_batch_plan_size(0) == -1
+    need = lib.nv_hydro_stage_batch_plan_size(2)
+    assert need > 0
+    members = (_lib.NvStageDesc * 2)()
+    buf = ctypes.create_string_buffer(int(need))
+    assert lib.nv_hydro_stage_batch_plan(members, 2, 0, 1, 0, 0, buf, 8) == 1001
+    assert b"plan needs" in lib.nv_last_error()
+    assert lib.nv_hydro_stage_batch_plan(members, 2, 0, 1, 0, 0, buf, need) == 1001
+    assert b"member 0" in lib.nv_last_error()
+    rays = (_lib.NvRayDesc * 1)()
+    rbuf = ctypes.create_string_buffer(int(lib.nv_cast_rays_batch_plan_size(1)))
+    assert lib.nv_cast_rays_batch_plan(rays, 1, rbuf, len(rbuf)) == 1001
+    ions = (_lib.NvIonDesc * 1)()
+    ibuf = ctypes.create_string_buffer(int(lib.nv_ionize_apply_batch_plan_size(1)))
+    assert lib.nv_ionize_apply_batch_plan(ions, 1, 0, ibuf, len(ibuf)) == 1001
+    assert lib.nv_hydro_stage_batch(buf, buf, None) == 1001     # a refused plan never got its header
 
 
 def test_rejects_null_fields(lib):

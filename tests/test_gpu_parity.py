@@ -109,6 +109,51 @@ def test_hydro_stage_bitwise(env, okern32, shape, solver, lim, mode):
     assert float(smax.item()) == ref_smax
 
 
+@pytest.mark.parametrize("shape", [(12, 9, 14), (33, 40, 96), (9, 35, 160), (17, 24, 64)])
+@pytest.mark.parametrize("tile", [1, 2])
+@pytest.mark.parametrize("precision", [0, 2])
+def test_hydro_stage_tile_shapes(env, okern32, shape, tile, precision):
+    """nv_hydro_stage_ex with each workgroup tile shape: bitwise the oracle (FP32 exact), bitwise the automatic
+    shape's result (binary16-velocity form) - the shape changes only the work decomposition"""
+    say(f"[hydro tile] shape={shape} tile={tile} precision={precision}")
+    torch, dev, L = env
+    dx = F32(1.0 / max(shape))
+    U, xi = random_state(shape, seed=5)
+    stars, jets = tables(shape, float(dx))
+    U0n = random_state(shape, seed=6)[0]
+    gst, gjt = T(env, stars), T(env, jets)
+    gxi = T(env, xi)
+    if precision == 0:
+        gU, gU0 = [T(env, u) for u in U], [T(env, u) for u in U0n]
+    else:
+        def enc(V):
+            r = np.maximum(V[0], F32(2e-3))
+            return [T(env, V[0])] + [T(env, (V[c] / r).astype(np.float16)) for c in (1, 2, 3)]
+        gU, gU0 = enc(U), enc(U0n)
+    outs = {}
+    for t in (tile, 0):
+        out = [torch.zeros_like(g) for g in gU]
+        smax = torch.zeros(1, dtype=torch.float32, device=dev)
+        d = L.NvStageDesc()
+        for c in range(4):
+            d.u[c], d.u0[c], d.out[c] = gU[c].data_ptr(), gU0[c].data_ptr(), out[c].data_ptr()
+        d.xi, d.stars, d.jets, d.smax_out, d.dt_dev = gxi.data_ptr(), gst.data_ptr(), gjt.data_ptr(), smax.data_ptr(), None
+        d.nx, d.ny, d.nz = shape
+        d.nstars, d.wind, d.jet_cos, d.jet_r, d.pulse, d.dt, d.dx = 3, 0.7, math.cos(math.radians(30)), 0.6, 1.3, 0.002, float(dx)
+        d.a, d.b, d.floor_, d.cn2, d.ci2, d.vmax = 0.5, 0.5, 2e-3, (1 / 12) ** 2, 1.0, 6.0
+        L.call("nv_hydro_stage_ex", ctypes.byref(d), precision, 1, 0, t, stream(env))
+        torch.cuda.synchronize()
+        outs[t] = [N(o) for o in out] + [float(smax.item())]
+    for c in range(5):
+        np.testing.assert_array_equal(outs[tile][c], outs[0][c], err_msg=f"field {c}")
+    if precision == 0:
+        ref_out = [np.zeros(shape, F32) for _ in range(4)]
+        okern32.hydro_stage(U, xi, U0n, ref_out, shape, stars, 3, 0.7, jets, math.cos(math.radians(30)), 0.6, 1.3,
+                            0.002, float(dx), 0.5, 0.5, 2e-3, (1 / 12) ** 2, 1.0, 6.0, 1, 0)
+        for c in range(4):
+            np.testing.assert_array_equal(outs[tile][c], ref_out[c], err_msg=f"field {c}")
+
+
 def test_hydro_stage_no_stars(env, okern32):
     say("[hydro] no stars, null tables")
     torch, dev, L = env
