@@ -82,6 +82,14 @@ typedef struct NvIonDesc {
   float* smax_out;
 } NvIonDesc;
 
+/* Render packing parameters: the reference's PackParams struct (render.py:77-88), same member order. */
+typedef struct NvPackParams {
+  float c_ha[3], c_o3[3], c_s2[3], c_dg[3], c_sh[3];
+  float gs, ambient, edge;
+  float inv_L[3];
+  float dx;
+} NvPackParams;
+
 /* Message of the last failed call on this thread ("" if none). */
 const char* nv_last_error(void);
 /* ABI version of this library (bumped on any signature change). */
@@ -235,6 +243,45 @@ int nv_apply_fuv(const float* att, int nr, int nu, int nphi, float sx, float sy,
 int nv_fuv_apply(const float* atts, int nr, int nu, int nphi, const float* star_xyz, const float* lums,
                  int nstars, float dx, float binr, uint16_t* g0_h, float* g0_f, int nx, int ny, int nz,
                  void* stream);
+
+/* ---- consumers of the step's fields ---- */
+
+/* Timeline snapshot: the 13-byte-per-cell stored form of a state (rho, mx, my, mz, dfront, g0 binary16 saturated at
+ * +-65000; xi as uint8 round(255 clamp(xi, 0, 1))). fields_half = 1: dfront and g0 are binary16 (the sim's own
+ * fields), else float32. Replaces timeline.k_pack_lm / k_pack (timeline.py:19-34, 57-72). */
+int nv_timeline_pack(const float* rho, const float* mx, const float* my, const float* mz, const float* xi,
+                     const void* dfront, const void* g0, int fields_half, uint16_t* o_r, uint16_t* o_mx,
+                     uint16_t* o_my, uint16_t* o_mz, uint8_t* o_xi, uint16_t* o_df, uint16_t* o_g0, long long n,
+                     void* stream);
+
+/* Restores a state from a snapshot (rho = max(stored rho, floor), xi = stored / 255). Replaces
+ * timeline.k_unpack_lm / k_unpack (timeline.py:37-52, 75-89). */
+int nv_timeline_unpack(const uint16_t* s_r, const uint16_t* s_mx, const uint16_t* s_my, const uint16_t* s_mz,
+                       const uint8_t* s_xi, const uint16_t* s_df, const uint16_t* s_g0, float floor_, float* rho,
+                       float* mx, float* my, float* mz, float* xi, void* dfront, void* g0, int fields_half,
+                       long long n, void* stream);
+
+/* Compression-heating proxy max(0, -div v) rho, central differences, clamped neighbours. Replaces
+ * nebula3d.shock_field (nebula3d.py:182-191), behind Sim.shock. */
+int nv_shock_field(const float* rho, const float* mx, const float* my, const float* mz, int nx, int ny, int nz,
+                   float dx, float* out, void* stream);
+
+/* The stored forms of one bake frame (bake.py:94-99): g0 -> min(g0, 6e4) binary16, rho -> binary16,
+ * xi -> uint8 round-half-even(255 clip(xi, 0, 1)), dfront -> clip(df, -4, 4) binary16, shock -> min(sh, 6e4)
+ * binary16 (shock and o_sh may both be null). fields_half as for nv_timeline_pack. */
+int nv_frame_pack(const float* rho, const float* xi, const void* dfront, const void* g0, const float* shock,
+                  int fields_half, uint16_t* o_rho, uint8_t* o_xi, uint16_t* o_df, uint16_t* o_g0, uint16_t* o_sh,
+                  long long n, void* stream);
+
+/* The renderer's RGBA16F volume: emission (H-alpha, [O III], [S II], dust x G0, shock) and dust per cell, faded
+ * at the box faces, binary16 x 4 per cell (out = uint16[4 * cells]). Replaces render.pack_volume_lm
+ * (render.py:176-204). */
+int nv_pack_volume(const float* rho, const float* mx, const float* my, const float* mz, const float* xi,
+                   const void* dfront, const void* g0, int fields_half, const NvPackParams* pp, int shock_on,
+                   uint16_t* out, int nx, int ny, int nz, void* stream);
+
+/* [1 4 1]/6 along one axis of a binary16 x 4 grid. Replaces render.smooth_axis (render.py:281-298). */
+int nv_smooth_axis(const uint16_t* src, uint16_t* dst, int axis, int nx, int ny, int nz, void* stream);
 
 #ifdef __cplusplus
 }

@@ -45,6 +45,12 @@ SIGNATURES = {
     "nv_ionize_apply_batch_plan_size": [I],
     "nv_ionize_apply_batch_plan": [P, I, I, P, LL],
     "nv_ionize_apply_batch": [P, P, P],
+    "nv_timeline_pack": [P] * 7 + [I] + [P] * 7 + [LL, P],
+    "nv_timeline_unpack": [P] * 7 + [F] + [P] * 7 + [I, LL, P],
+    "nv_shock_field": [P] * 4 + [I, I, I, F, P, P],
+    "nv_frame_pack": [P] * 5 + [I] + [P] * 5 + [LL, P],
+    "nv_pack_volume": [P] * 7 + [I, P, I, P, I, I, I, P],
+    "nv_smooth_axis": [P, P, I, I, I, I, P],
 }
 
 RESTYPES = {"nv_last_error": ctypes.c_char_p, "nv_hydro_stage_batch_plan_size": LL,
@@ -71,6 +77,12 @@ class NvIonDesc(ctypes.Structure):
     _fields_ = [("rfronts", P), ("nu", I), ("nphi", I), ("star_xyz", P), ("nstars", I), ("dx", F), ("width", F),
                 ("xi", P), ("dfront_h", P), ("dfront_f", P), ("nx", I), ("ny", I), ("nz", I), ("state", P * 4),
                 ("cn2", F), ("ci2", F), ("smax_out", P)]
+
+
+class NvPackParams(ctypes.Structure):
+    """render packing parameters (PackParams, render.py:77-88)"""
+    _fields_ = [("c_ha", F * 3), ("c_o3", F * 3), ("c_s2", F * 3), ("c_dg", F * 3), ("c_sh", F * 3), ("gs", F),
+                ("ambient", F), ("edge", F), ("inv_L", F * 3), ("dx", F)]
 
 
 _lib = None

@@ -464,8 +464,20 @@ class Sim:
         self._wkey = None
         self._set_star_weights()
         self.fields_modified()
+        # xi, dfront and g0 come with the restored fields: no ray casting needed to show this state
         self.rf_version = self.version
         self.fuv_version = self.version
+        self.fuv_nstars = sum(1 for s in self.stars if self.t >= s[4])
+        self.fuv_phase = self.phase
+
+    @property
+    def shock(self):
+        """compression heating proxy max(0, -div v) rho in a new float32 grid (nv_shock_field; sim.py:827-833)"""
+        out = torch.empty(self.shape, dtype=torch.float32, device=self.device)
+        rho, mx, my, mz = self.rho, self.mx, self.my, self.mz
+        _lib.call("nv_shock_field", rho.data_ptr(), mx.data_ptr(), my.data_ptr(), mz.data_ptr(), *self.shape,
+                  _f(self.dx), out.data_ptr(), self._stream)
+        return out
 
     def render_scratch(self):
         """two (nx, ny, nz, 4) float16 views of the stage block, free between steps (sim.py:821-825)"""

@@ -64,6 +64,53 @@ def load():
     return _lib
 
 
+class Codec:
+    """The oracle's restatement of the consumers' kernels (oracle/codec_oracle.cpp)."""
+
+    def __init__(self):
+        self.lib = load()
+        P, I, F, LL = ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_longlong
+        L = self.lib
+        L.or_timeline_pack.argtypes = [P] * 14 + [LL]
+        L.or_timeline_unpack.argtypes = [P] * 7 + [F] + [P] * 7 + [LL]
+        L.or_shock_field.argtypes = [P] * 4 + [I, I, I, F, P]
+        L.or_pack_volume.argtypes = [P] * 7 + [I, I, I, P, I, P]
+        L.or_smooth_axis.argtypes = [P, P, I, I, I, I]
+
+    def timeline_pack(self, rho, mx, my, mz, xi, df, g0):
+        n = rho.size
+        out = [np.empty(rho.shape, np.uint16) for _ in range(4)] + [np.empty(rho.shape, np.uint8)] + \
+              [np.empty(rho.shape, np.uint16) for _ in range(2)]
+        ins = [np.ascontiguousarray(a, np.float32) for a in (rho, mx, my, mz, xi, df, g0)]
+        self.lib.or_timeline_pack(*[_ptr(a) for a in ins], *[_ptr(a) for a in out], n)
+        return out
+
+    def timeline_unpack(self, arrs, floor):
+        shape = arrs[0].shape
+        out = [np.empty(shape, np.float32) for _ in range(7)]
+        self.lib.or_timeline_unpack(*[_ptr(a) for a in arrs], ctypes.c_float(floor), *[_ptr(a) for a in out],
+                                    arrs[0].size)
+        return out
+
+    def shock_field(self, rho, mx, my, mz, dx):
+        out = np.empty(rho.shape, np.float32)
+        self.lib.or_shock_field(_ptr(rho), _ptr(mx), _ptr(my), _ptr(mz), *rho.shape, ctypes.c_float(dx), _ptr(out))
+        return out
+
+    def pack_volume(self, rho, mx, my, mz, xi, df, g0, pp, shock_on):
+        """pp: 22 floats in the PackParams order; df, g0 as float32 values"""
+        out = np.empty(rho.shape + (4,), np.uint16)
+        ppa = np.ascontiguousarray(pp, np.float32)
+        ins = [np.ascontiguousarray(a, np.float32) for a in (rho, mx, my, mz, xi, df, g0)]
+        self.lib.or_pack_volume(*[_ptr(a) for a in ins], *rho.shape, _ptr(ppa), int(shock_on), _ptr(out))
+        return out
+
+    def smooth_axis(self, src, axis):
+        dst = np.empty_like(src)
+        self.lib.or_smooth_axis(_ptr(src), _ptr(dst), axis, *src.shape[:3])
+        return dst
+
+
 def _ptr(a):
     assert a.flags["C_CONTIGUOUS"]
     return a.ctypes.data

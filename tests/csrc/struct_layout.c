@@ -24,5 +24,9 @@ int main(void) {
   OFF(NvIonDesc, nstars); OFF(NvIonDesc, dx); OFF(NvIonDesc, width); OFF(NvIonDesc, xi); OFF(NvIonDesc, dfront_h);
   OFF(NvIonDesc, dfront_f); OFF(NvIonDesc, nx); OFF(NvIonDesc, ny); OFF(NvIonDesc, nz); OFF(NvIonDesc, state);
   OFF(NvIonDesc, cn2); OFF(NvIonDesc, ci2); OFF(NvIonDesc, smax_out);
+  printf("NvPackParams %zu\n", sizeof(NvPackParams));
+  OFF(NvPackParams, c_ha); OFF(NvPackParams, c_o3); OFF(NvPackParams, c_s2); OFF(NvPackParams, c_dg);
+  OFF(NvPackParams, c_sh); OFF(NvPackParams, gs); OFF(NvPackParams, ambient); OFF(NvPackParams, edge);
+  OFF(NvPackParams, inv_L); OFF(NvPackParams, dx);
   return 0;
 }

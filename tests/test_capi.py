@@ -64,7 +64,7 @@ def struct_fields(name):
     return out
 
 
-@pytest.mark.parametrize("name", ["NvStageDesc", "NvRayDesc", "NvIonDesc"])
+@pytest.mark.parametrize("name", ["NvStageDesc", "NvRayDesc", "NvIonDesc", "NvPackParams"])
 def test_descriptor_structs_match_header(name):
     """the ctypes mirrors have the header's members in order, with pointer/int/float kinds and C sizes"""
     cls = getattr(_lib, name)
@@ -92,7 +92,7 @@ def test_descriptor_layout_matches_c(tmp_path):
                    check=True)
     c = dict(line.split() for line in subprocess.run([str(exe)], capture_output=True, text=True,
                                                      check=True).stdout.splitlines())
-    for name in ("NvStageDesc", "NvRayDesc", "NvIonDesc"):
+    for name in ("NvStageDesc", "NvRayDesc", "NvIonDesc", "NvPackParams"):
         cls = getattr(_lib, name)
         assert ctypes.sizeof(cls) == int(c[name]), name
         for fname, _ in cls._fields_:
