@@ -283,6 +283,29 @@ int nv_pack_volume(const float* rho, const float* mx, const float* my, const flo
 /* [1 4 1]/6 along one axis of a binary16 x 4 grid. Replaces render.smooth_axis (render.py:281-298). */
 int nv_smooth_axis(const uint16_t* src, uint16_t* dst, int axis, int nx, int ny, int nz, void* stream);
 
+/* ---- the pre-turbulence phase ---- */
+
+/* Turbulent stirring: m += rho amp dt ((1 - blend) F + blend G), F and G trilinear samples of two stirring vector
+ * fields (fx..fz, gx..gz: float[sx][sy][sz] each, spacing dxs) at the cell centres. Replaces nebula3d.drive
+ * (nebula3d.py:165-179), launched by Sim._stir (sim.py:1030-1042). */
+int nv_drive(float* rho, float* mx, float* my, float* mz, int nx, int ny, int nz, const float* fx, const float* fy,
+             const float* fz, const float* gx, const float* gy, const float* gz, int sx, int sy, int sz, float blend,
+             float amp, float dt, float dx, float dxs, void* stream);
+
+/* The torus added at the end of the pre-turbulence phase: rho += torus, m *= 1 - torus / (torus + rho). Replaces
+ * sim.add_torus (sim.py:353-363). */
+int nv_add_torus(float* rho, float* mx, float* my, float* mz, const float* torus, long long n, void* stream);
+
+/* out[0] += sum |m|^2 / rho, out[1] += sum rho over n cells, each cell's term in float as sim.mach_sums
+ * (sim.py:502-508), the sums in FP64 (the reference's float atomics depend on their order; these do only in the
+ * last bits of a double). The caller zeroes out[0..1]. */
+int nv_mach_sums(const float* rho, const float* mx, const float* my, const float* mz, long long n, double* out,
+                 void* stream);
+
+/* dst = src[::2, ::2, ::2] as float32 (src binary16 when src_half). Replaces sim.every2 / every2_half
+ * (sim.py:517-527), the subsampling behind the star-formation trigger. */
+int nv_every2(const void* src, int src_half, float* dst, int nx, int ny, int nz, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -51,6 +51,10 @@ SIGNATURES = {
     "nv_frame_pack": [P] * 5 + [I] + [P] * 5 + [LL, P],
     "nv_pack_volume": [P] * 7 + [I, P, I, P, I, I, I, P],
     "nv_smooth_axis": [P, P, I, I, I, I, P],
+    "nv_drive": [P] * 4 + [I, I, I] + [P] * 6 + [I, I, I, F, F, F, F, F, P],
+    "nv_add_torus": [P] * 5 + [LL, P],
+    "nv_mach_sums": [P] * 4 + [LL, P, P],
+    "nv_every2": [P, I, P, I, I, I, P],
 }
 
 RESTYPES = {"nv_last_error": ctypes.c_char_p, "nv_hydro_stage_batch_plan_size": LL,

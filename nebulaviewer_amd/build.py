@@ -9,7 +9,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libnebula_hip.so")
-SOURCES = ["capi.hip", "hydro.hip", "rays.hip", "codec.hip"]
+SOURCES = ["capi.hip", "hydro.hip", "rays.hip", "codec.hip", "pre.hip"]
 ARCH = os.environ.get("NV_OFFLOAD_ARCH", "gfx950")
 
 # -ffp-contract=off + IEEE division/sqrt: the kernels reproduce the CPU oracle bit for bit.

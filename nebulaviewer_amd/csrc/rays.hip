@@ -15,6 +15,7 @@
 #include "nv_common.h"
 #include "nv_math.h"
 #include "nv_divide.h"
+#include "nv_sample.h"
 
 namespace nv {
 
@@ -54,78 +55,6 @@ __device__ __forceinline__ void ray_dir(int iu, int ip, int nu, int nphi, float 
 __device__ __forceinline__ float ldf(const float* __restrict__ f, unsigned i) { return f[i]; }
 __device__ __forceinline__ float ldf(const uint16_t* __restrict__ f, unsigned i) {
   return (float)__builtin_bit_cast(_Float16, f[i]);
-}
-
-// How a sampler turns a world coordinate into cell units: p / dx through the shared double reciprocal (the
-// identical correctly rounded quotient, nv_divide.h), or p * (1/dx) when dx is a power of two, which rounds the
-// same real number and so gives the same float. A compile-time choice, so only one form is issued.
-struct ScaleDiv {
-  double rdx;
-  __device__ __forceinline__ float operator()(float p) const { return div_r(p, rdx); }
-};
-struct ScalePow2 {
-  float inv;
-  __device__ __forceinline__ float operator()(float p) const { return p * inv; }
-};
-
-// Loads at a cell index: OFF = uint32_t forms 32-bit byte offsets from the field's base (grids up to 2^30
-// cells, so that 4 * index < 2^32), uint64_t 64-bit ones.
-template <class OFF>
-__device__ __forceinline__ float ld1(const float* __restrict__ f, unsigned i) {
-  return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(f) + (OFF)i * 4u);
-}
-// two z-neighbours (i, i+1) in one 8-byte load (4-byte aligned, which global loads allow)
-template <class OFF>
-__device__ __forceinline__ float2 ld2(const float* __restrict__ f, unsigned i) {
-  typedef float f2u __attribute__((ext_vector_type(2), aligned(4)));
-  const f2u v = *reinterpret_cast<const f2u*>(reinterpret_cast<const char*>(f) + (OFF)i * 4u);
-  return make_float2(v.x, v.y);
-}
-
-// trilinear sample at world p, 0 outside [-1/2, n-1/2] in cell units (sample_n, nebula3d.py:195-226), with
-// 32-bit cell indices (grids up to 2^32 cells). PAIR (nz >= 2): the corners at ka = clamp(k0) and
-// kb = clamp(k0+1) come from one load of (kp, kp+1), kp = clamp(k0, 0, nz-2): k0 = -1 gives ka = kb = 0 (the
-// pair's first value), k0 = nz-1 gives ka = kb = nz-1 (its second), so every corner is the value the separate
-// clamped loads read.
-template <bool PAIR, class OFF, class SC>
-__device__ __forceinline__ float sample_r(const float* __restrict__ f, int nx, int ny, int nz, float px, float py,
-                                          float pz, const SC& cs) {
-  const float gx = cs(px) - 0.5f, gy = cs(py) - 0.5f, gz = cs(pz) - 0.5f;
-  if (gx < -0.5f || gy < -0.5f || gz < -0.5f) return 0.0f;
-  if (gx > (float)nx - 0.5f || gy > (float)ny - 0.5f || gz > (float)nz - 0.5f) return 0.0f;
-  const int i0 = (int)floorf(gx), j0 = (int)floorf(gy), k0 = (int)floorf(gz);
-  const float fx = gx - (float)i0, fy = gy - (float)j0, fz = gz - (float)k0;
-  const unsigned ia = clampi(i0, 0, nx - 1), ib = clampi(i0 + 1, 0, nx - 1);
-  const unsigned ja = clampi(j0, 0, ny - 1), jb = clampi(j0 + 1, 0, ny - 1);
-  const unsigned r00 = (ia * ny + ja) * nz, r10 = (ib * ny + ja) * nz;
-  const unsigned r01 = (ia * ny + jb) * nz, r11 = (ib * ny + jb) * nz;
-  float c000, c100, c010, c110, c001, c101, c011, c111;
-  if constexpr (PAIR) {
-    const unsigned kp = clampi(k0, 0, nz - 2);
-    const bool a_second = k0 > nz - 2, b_first = k0 < 0;
-    const float2 p00 = ld2<OFF>(f, r00 + kp), p10 = ld2<OFF>(f, r10 + kp);
-    const float2 p01 = ld2<OFF>(f, r01 + kp), p11 = ld2<OFF>(f, r11 + kp);
-    c000 = a_second ? p00.y : p00.x;
-    c100 = a_second ? p10.y : p10.x;
-    c010 = a_second ? p01.y : p01.x;
-    c110 = a_second ? p11.y : p11.x;
-    c001 = b_first ? p00.x : p00.y;
-    c101 = b_first ? p10.x : p10.y;
-    c011 = b_first ? p01.x : p01.y;
-    c111 = b_first ? p11.x : p11.y;
-  } else {
-    const unsigned ka = clampi(k0, 0, nz - 1), kb = clampi(k0 + 1, 0, nz - 1);
-    c000 = ld1<OFF>(f, r00 + ka); c100 = ld1<OFF>(f, r10 + ka); c010 = ld1<OFF>(f, r01 + ka);
-    c110 = ld1<OFF>(f, r11 + ka); c001 = ld1<OFF>(f, r00 + kb); c101 = ld1<OFF>(f, r10 + kb);
-    c011 = ld1<OFF>(f, r01 + kb); c111 = ld1<OFF>(f, r11 + kb);
-  }
-  const float c00 = c000 + (c100 - c000) * fx;
-  const float c10 = c010 + (c110 - c010) * fx;
-  const float c01 = c001 + (c101 - c001) * fx;
-  const float c11 = c011 + (c111 - c011) * fx;
-  const float c0 = c00 + (c10 - c00) * fy;
-  const float c1 = c01 + (c11 - c01) * fy;
-  return c0 + (c1 - c0) * fz;
 }
 
 // Direction cell of thread t: each wavefront takes an 8 x 8 patch of (u, phi) cells, so its 64 rays stay

@@ -35,6 +35,9 @@ STEP_DEFAULTS = dict(
     cfl=0.3, ci_over_cn=12.0, floor=2e-3, wind=0.0, ion_every=2, kappa_fuv=25.0,
     jet=0.0, jet_proto=0.0, jet_angle=10.0, jet_r=0.04, jet_pulse=0.6, jet_period=0.02,
     max_stars=32, q_main=Q0, q_ramp=0.0, solver="hll", limiter="minmod", fp32_fields=False, seed=0,
+    # pre-turbulence phase (nebula3d.py:619-625): duration, stirring amplitude, band, correlation time; drive_main
+    # keeps stirring in the main phase at drive_main x pre_amp
+    pre_t=0.0, pre_amp=7.0, pre_kmax=4.0, pre_corr=0.06, drive_main=0.0,
 )
 # parameters that may change while the sim runs (sim.py:82-83); the others need a reset
 LIVE = ("q_main", "wind", "jet", "jet_proto", "jet_pulse", "jet_period", "kappa_fuv", "drive_main", "pre_amp",
@@ -98,6 +101,7 @@ class Setup:
     mx: np.ndarray = None
     my: np.ndarray = None
     mz: np.ndarray = None
+    torus: np.ndarray = None     # density added at rest when the pre-turbulence phase ends (sim.py:806-812)
     stars: list = field(default_factory=list)
     kinds: list = field(default_factory=list)
     jets: list = field(default_factory=list)
@@ -120,6 +124,40 @@ def make_params(overrides=None):
     return p
 
 
+def stirring_field(shape, rng, kmin=1.0, kmax=6.0):
+    """A unit-RMS divergence-free random vector field on a small grid, band-limited to kmin <= |k| <= kmax (k in
+    grid wavenumbers), amplitude 1/|k|: the large-scale stirring of the pre-turbulence phase. The steps are those
+    of nebula3d.solenoidal (nebula3d.py:514-524): complex normal modes, the divergence projected out, the real
+    part of the inverse FFT, normalised to unit RMS speed."""
+    k = np.meshgrid(*[np.fft.fftfreq(n) * n for n in shape], indexing="ij")
+    kk = np.sqrt(k[0] ** 2 + k[1] ** 2 + k[2] ** 2)
+    band = (kk >= kmin) & (kk <= kmax)
+    amp = np.zeros(shape)
+    amp[band] = 1.0 / kk[band]
+    modes = [amp * (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)) for _ in range(3)]
+    k2 = np.maximum(kk * kk, 1e-12)
+    div = (k[0] * modes[0] + k[1] * modes[1] + k[2] * modes[2]) / k2
+    v = [np.fft.ifftn(modes[c] - k[c] * div).real for c in range(3)]
+    rms = math.sqrt(sum(float((c * c).mean()) for c in v))
+    return [np.ascontiguousarray(c / rms, dtype=np.float32) for c in v]
+
+
+class Stirring:
+    """The sequence of stirring fields of one run, from its own seeded stream, so that every implementation of
+    the step draws the same fields. Grid: 24 cells per domain unit plus a guard cell each side (the reference's
+    fixed-resolution forcing, sim.py:761-764)."""
+
+    DXS = 1.0 / 24.0
+
+    def __init__(self, setup):
+        self.shape = tuple(int(math.ceil(L_ / self.DXS)) + 2 for L_ in setup.L)
+        self.kmax = setup.P["pre_kmax"]
+        self.rng = np.random.default_rng(int(setup.P.get("seed", 0)) + 7919)
+
+    def next(self):
+        return stirring_field(self.shape, self.rng, 1.0, self.kmax)
+
+
 def _axes(n, dx):
     x = ((np.arange(n) + 0.5) * dx).astype(np.float32)
     return x[:, None, None], x[None, :, None], x[None, None, :]
@@ -138,14 +176,21 @@ def torus(n=64, seed=0, jet=False, params=None):
     X, Y, Z = _axes(n, dx)
     rc = np.sqrt((X - 0.5) ** 2 + (Y - 0.5) ** 2)
     h = Z - np.float32(0.5)
-    base = 1.0 + 50.0 * np.exp(-((rc - 0.2) ** 2 + h * h) / np.float32(2 * 0.06 ** 2))
+    tor = 50.0 * np.exp(-((rc - 0.2) ** 2 + h * h) / np.float32(2 * 0.06 ** 2))
     rng = np.random.default_rng(seed)
-    rho = base.astype(np.float32) * lognormal((n, n, n), 0.3, 4.0, rng)
-    rho = np.maximum(rho, np.float32(P["floor"])).astype(np.float32)
+    ln = lognormal((n, n, n), 0.3, 4.0, rng)
+    torus_rho = None
+    if P["pre_t"] > 0:
+        # with a turbulent phase, the torus forms when it ends, at rest (the reference's bipolar scene)
+        rho = np.maximum(ln, np.float32(P["floor"])).astype(np.float32)
+        torus_rho = (tor.astype(np.float32) * ln).astype(np.float32)
+    else:
+        rho = (1.0 + tor).astype(np.float32) * ln
+        rho = np.maximum(rho, np.float32(P["floor"])).astype(np.float32)
     stars = [(0.5, 0.5, 0.5, 1.0, 0.0)]
     jets = [(0.0, 0.0, 1.0)]
-    return Setup("torus_jet" if jet else "torus", (n, n, n), n, P, rho, stars=stars, kinds=["main"], jets=jets,
-                 jkinds=["jet" if jet else "none"], steps=200 if jet else 100)
+    return Setup("torus_jet" if jet else "torus", (n, n, n), n, P, rho, torus=torus_rho, stars=stars, kinds=["main"],
+                 jets=jets, jkinds=["jet" if jet else "none"], steps=200 if jet else 100)
 
 
 def sheet(n=256, seed=1, params=None):

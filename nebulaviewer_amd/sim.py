@@ -19,6 +19,7 @@ precision: "fp32" reproduces the reference's float32 step bit for bit; "fp32-fas
 hardware reciprocals / square roots in the RK stage; "fp16-vel" stores float32 density with binary16 velocity
 (58 instead of 88 bytes of state traffic per cell and step) and then mx, my, mz are computed on access.
 """
+import copy
 import math
 import time
 
@@ -189,16 +190,79 @@ class Sim:
         self.rf_version = -1
         self.fuv_version = -10 ** 9
         self.events = []
-        self.phase = "main"       # the pre-turbulence phase is not part of this step (config pre_t = 0)
-        self._start_main()
+        self.mach = None
+        self.tp, self.ip = 0.0, 0
+        self.torus = setup.torus
+        self.stirring = S.Stirring(setup)
+        self.sshape, self.dxs = self.stirring.shape, S.Stirring.DXS
+        self.drv = None
+        if P.get("pre_t", 0.0) > 0:
+            # pre-turbulence phase (sim.py:786-789): stirred hydro, no stars, until tp reaches pre_t
+            self.phase = "pre"
+            self.drv = {"A": self._mkF(), "B": self._mkF(), "clock": 0.0}
+        else:
+            self.phase = "main"
+            self._start_main()
         torch.cuda.synchronize(dev)
         self.setup_s = time.perf_counter() - t_setup
         self.log(f"{self.scenario} grid {nx}x{ny}x{nz} ({nx * ny * nz / 1e6:.1f}M cells) rays {nu}x{nphi} "
                  f"fuv {fg} solver {P['solver']}/{P['limiter']} setup {self.setup_s:.2f}s, {len(self.stars)} stars")
 
     def _start_main(self):
+        """end of the turbulent phase: the torus added at rest, the first ionization (sim.py:806-818)"""
+        if self.torus is not None:
+            t = torch.from_numpy(np.ascontiguousarray(self.torus, dtype=np.float32)).to(self.device)
+            U = self.U if not self.vel_state else None
+            if U is None:
+                raise ValueError("the torus of a pre-turbulence run needs the float32 state")
+            _lib.call("nv_add_torus", *[u.data_ptr() for u in U], t.data_ptr(), t.numel(), self._stream)
+            torch.cuda.synchronize(self.device)
+            self.torus = None
+            self.fields_modified()
+        self.phase = "main"
         self._set_star_weights()
         self.ionize()
+
+    # ------------------------------------------------------------------ pre-turbulence (sim.py:1030-1060)
+    def _mkF(self):
+        """the next stirring field (3 device arrays on the stirring grid)"""
+        return [torch.from_numpy(c).to(self.device) for c in self.stirring.next()]
+
+    def _stir(self, dt, amp):
+        d = self.drv
+        blend = min(d["clock"] / self.P["pre_corr"], 1.0)
+        U = self.U
+        if self.vel_state:
+            raise ValueError("stirring needs the float32 state")
+        _lib.call("nv_drive", *[u.data_ptr() for u in U], *self.shape, *[f.data_ptr() for f in d["A"]],
+                  *[f.data_ptr() for f in d["B"]], *self.sshape, _f(blend), _f(amp), _f(dt), _f(self.dx),
+                  _f(self.dxs), self._stream)
+        d["clock"] += dt
+        if d["clock"] >= self.P["pre_corr"]:
+            d["A"], d["B"], d["clock"] = d["B"], self._mkF(), 0.0
+        self._smax_ok = False          # the stage's fused CFL max predates the stirring
+
+    def mach_number(self):
+        """mass-weighted rms Mach number sqrt(sum m^2/rho / sum rho) / c_n (sim.py:1052-1055), sums in FP64"""
+        s = torch.zeros(2, dtype=torch.float64, device=self.device)
+        _lib.call("nv_mach_sums", *[t.data_ptr() for t in (self.rho, self.mx, self.my, self.mz)], self.rho.numel(),
+                  s.data_ptr(), self._stream)
+        a, b = s.tolist()
+        return math.sqrt(a / b) * self.P["ci_over_cn"]
+
+    def _pre_step(self):
+        dt = self._compute_dt()
+        self._hydro(dt, 0, 0.0, 0.0)
+        self._stir(dt, self.P["pre_amp"])
+        self.tp += dt
+        self.ip += 1
+        self.dt = dt
+        if self.tp >= self.P["pre_t"]:
+            self.mach = self.mach_number()
+            self.log(f"pre-turbulence done: {self.ip} steps t={self.tp:.3f} mass-weighted Mach ~{self.mach:.1f}")
+            if self.P["drive_main"] <= 0:
+                self.drv = None
+            self._start_main()
 
     # ------------------------------------------------------------------ stars (sim.py:879-925)
     def q_of(self, i):
@@ -417,6 +481,8 @@ class Sim:
             dt = min(dt, self.dt_cap)
         pulse = max(0.0, 1.0 + P["jet_pulse"] * math.sin(2 * math.pi * self.t / P["jet_period"]))
         self._hydro(dt, len(self.stars), P["wind"], pulse)
+        if P["drive_main"] > 0 and self.drv is not None:
+            self._stir(dt, P["pre_amp"] * P["drive_main"])
         self.t += dt
         self.it += 1
         self.dt = dt
@@ -426,7 +492,10 @@ class Sim:
         """advance k steps. Returns wall seconds."""
         t0 = time.perf_counter()
         for _ in range(k):
-            self._main_step()
+            if self.phase == "pre":
+                self._pre_step()
+            else:
+                self._main_step()
             self.version += 1
         self._tick("step", t0)
         self.prof["n_steps"] = self.prof.get("n_steps", 0) + k
@@ -437,32 +506,50 @@ class Sim:
         if isinstance(old, int) and not isinstance(old, bool):
             value = int(round(value))
         self.P[name] = value
+        if name == "drive_main" and value > 0 and self.drv is None and self.phase == "main":
+            self.drv = {"A": self._mkF(), "B": self._mkF(), "clock": 0.0}
         if name in S.LIVE:
-            self.events.append((float(self.t), name, value))
+            self.events.append((float(self.t if self.phase == "main" else 0.0), name, value))
             if name in ("q_main", "jet", "jet_proto"):
                 self._wkey = None
         return value
 
     def tau(self):
-        return self.t
+        """timeline time: tp - pre_t (< 0) during the pre-turbulence phase, t afterwards (sim.py:1169-1171)"""
+        return self.t if self.phase == "main" else self.tp - self.P["pre_t"]
 
     def time_label(self):
         return f"t={self.t:.4f}"
 
     def get_state_meta(self):
-        return dict(t=self.t, it=self.it, dt=self.dt, phase=self.phase, stars=list(self.stars),
-                    kinds=list(self.kinds), jets=list(self.jets), jkinds=list(self.jkinds), P=dict(self.P),
-                    n_events=len(self.events))
+        """everything but the big fields needed to resume from this moment (sim.py:1173-1184): clocks, phase, stars,
+        parameters, the stirring fields in use (device arrays, replaced - never modified - by the stirring) and
+        the stirring stream's state"""
+        d = self.drv
+        return dict(t=self.t, it=self.it, tp=self.tp, ip=self.ip, dt=self.dt, phase=self.phase, mach=self.mach,
+                    stars=list(self.stars), kinds=list(self.kinds), jets=list(self.jets), jkinds=list(self.jkinds),
+                    P=dict(self.P), n_events=len(self.events),
+                    drv=None if d is None else {"A": list(d["A"]), "B": list(d["B"]), "clock": d["clock"]},
+                    rng=copy.deepcopy(self.stirring.rng.bit_generator.state),
+                    torus=self.torus)
 
     def set_state_meta(self, m):
+        """after the fields were restored (Timeline.restore; sim.py:1186-1211)"""
         for k in ("t", "it", "dt", "phase"):
             setattr(self, k, m[k])
+        self.tp, self.ip, self.mach = m.get("tp", 0.0), m.get("ip", 0), m.get("mach")
         self.stars, self.kinds = list(m["stars"]), list(m["kinds"])
         self.jets, self.jkinds = list(m["jets"]), list(m["jkinds"])
         self.P = dict(m["P"])
         self.events = self.events[:m["n_events"]]
+        d = m.get("drv")
+        self.drv = None if d is None else {"A": list(d["A"]), "B": list(d["B"]), "clock": d["clock"]}
+        if "rng" in m:
+            self.stirring.rng.bit_generator.state = copy.deepcopy(m["rng"])
+        self.torus = m.get("torus")
         self._wkey = None
-        self._set_star_weights()
+        if self.phase == "main":
+            self._set_star_weights()
         self.fields_modified()
         # xi, dfront and g0 come with the restored fields: no ray casting needed to show this state
         self.rf_version = self.version

@@ -598,6 +598,55 @@ void or_cast_rays_wide_f32(const float* rho, int nx, int ny, int nz, float sx, f
   cast_rays_wide(rho, nx, ny, nz, st, Q, dr, nsteps, dx, rcap, bm, rfront, nu, nphi, nthreads);
 }
 
+// nebula3d.drive (nebula3d.py:165-179): m += rho amp dt ((1 - blend) F + blend G) at each cell centre, F and G
+// trilinear on the stirring grid of spacing dxs
+void or_drive(const float* rho, float* mx, float* my, float* mz, int nx, int ny, int nz, const float* fx,
+              const float* fy, const float* fz, const float* gx, const float* gy, const float* gz, int sx, int sy,
+              int sz, float blend, float amp, float dt, float dx, float dxs) {
+  const Grid s{sx, sy, sz};
+  const Grid g{nx, ny, nz};
+  const float* F[3] = {fx, fy, fz};
+  const float* Gf[3] = {gx, gy, gz};
+  float* M[3] = {mx, my, mz};
+  for (int i = 0; i < nx; ++i)
+    for (int j = 0; j < ny; ++j)
+      for (int k = 0; k < nz; ++k) {
+        const float p[3] = {((float)i + 0.5f) * dx, ((float)j + 0.5f) * dx, ((float)k + 0.5f) * dx};
+        const size_t o = g.at(i, j, k);
+        const float r = rho[o] * amp * dt;
+        for (int c = 0; c < 3; ++c) {
+          const float fa = sample_n(F[c], s, p[0], p[1], p[2], dxs);
+          const float fb = sample_n(Gf[c], s, p[0], p[1], p[2], dxs);
+          const float f = (1.0f - blend) * fa + blend * fb;
+          M[c][o] = M[c][o] + r * f;
+        }
+      }
+}
+
+// sim.add_torus (sim.py:353-363)
+void or_add_torus(float* rho, float* mx, float* my, float* mz, const float* torus, long long n) {
+  for (long long q = 0; q < n; ++q) {
+    const float r = rho[q], tq = torus[q];
+    const float f = 1.0f - tq / (tq + r);
+    rho[q] = r + tq;
+    mx[q] = mx[q] * f;
+    my[q] = my[q] * f;
+    mz[q] = mz[q] * f;
+  }
+}
+
+// sim.mach_sums (sim.py:502-508): the per-cell float terms summed in double
+void or_mach_sums(const float* rho, const float* mx, const float* my, const float* mz, long long n, double* out) {
+  double a = 0.0, b = 0.0;
+  for (long long q = 0; q < n; ++q) {
+    const float m2 = (mx[q] * mx[q] + my[q] * my[q]) + mz[q] * mz[q];
+    a += (double)(m2 / rho[q]);
+    b += (double)rho[q];
+  }
+  out[0] = a;
+  out[1] = b;
+}
+
 void or_to_half(const float* src, uint16_t* dst, long long n, int nthreads) {
 #pragma omp parallel for num_threads(nthreads) schedule(static)
   for (long long q = 0; q < n; ++q) dst[q] = f32_to_f16(src[q]);

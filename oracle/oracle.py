@@ -105,6 +105,23 @@ class Codec:
         self.lib.or_pack_volume(*[_ptr(a) for a in ins], *rho.shape, _ptr(ppa), int(shock_on), _ptr(out))
         return out
 
+    def drive(self, rho, m, F, G, blend, amp, dt, dx, dxs):
+        """m (3 arrays) updated in place"""
+        P, I, R = ctypes.c_void_p, ctypes.c_int, ctypes.c_float
+        self.lib.or_drive.argtypes = [P] * 4 + [I, I, I] + [P] * 6 + [I, I, I] + [R] * 5
+        self.lib.or_drive(_ptr(rho), *[_ptr(a) for a in m], *rho.shape, *[_ptr(a) for a in F], *[_ptr(a) for a in G],
+                          *F[0].shape, R(blend), R(amp), R(dt), R(dx), R(dxs))
+
+    def add_torus(self, rho, m, torus):
+        self.lib.or_add_torus.argtypes = [ctypes.c_void_p] * 5 + [ctypes.c_longlong]
+        self.lib.or_add_torus(_ptr(rho), *[_ptr(a) for a in m], _ptr(torus), rho.size)
+
+    def mach_sums(self, rho, m):
+        out = np.zeros(2)
+        self.lib.or_mach_sums.argtypes = [ctypes.c_void_p] * 4 + [ctypes.c_longlong, ctypes.c_void_p]
+        self.lib.or_mach_sums(_ptr(rho), *[_ptr(a) for a in m], rho.size, _ptr(out))
+        return out
+
     def smooth_axis(self, src, axis):
         dst = np.empty_like(src)
         self.lib.or_smooth_axis(_ptr(src), _ptr(dst), axis, *src.shape[:3])
@@ -201,8 +218,54 @@ class OracleSim:
         self.rfront = np.zeros(tuple(rays), dt_)
         self.att = np.zeros(tuple(fuv), dt_)
         self.t, self.it, self.dt, self.version = 0.0, 0, 0.0, 0
+        # pre-turbulence phase (sim.py:786-789, 1030-1060); the float oracle only
+        from nebulaviewer_amd import scenarios as S
+        self.tp, self.ip, self.mach = 0.0, 0, None
+        self.torus = None if setup.torus is None else np.ascontiguousarray(setup.torus, dtype=np.float32)
+        self.stirring = S.Stirring(setup)
+        self.drv = None
+        if self.P.get("pre_t", 0.0) > 0:
+            if dt_ != np.float32:
+                raise ValueError("the pre-turbulence phase is restated for the float oracle")
+            self.codec = Codec()
+            self._set_star_weights()          # allocated; the pre-phase stages pass nstars = 0
+            self.phase = "pre"
+            self.drv = {"A": self.stirring.next(), "B": self.stirring.next(), "clock": 0.0}
+        else:
+            self.phase = "main"
+            self._set_star_weights()
+            self.ionize()
+
+    def _start_main(self):
+        if self.torus is not None:
+            self.codec.add_torus(self.rho, [self.mx, self.my, self.mz], self.torus)
+            self.torus = None
+        self.phase = "main"
         self._set_star_weights()
         self.ionize()
+
+    def _stir(self, dt, amp):
+        d = self.drv
+        blend = min(d["clock"] / self.P["pre_corr"], 1.0)
+        self.codec.drive(self.rho, [self.mx, self.my, self.mz], d["A"], d["B"], blend, amp, dt, self.dx,
+                         self.stirring.DXS)
+        d["clock"] += dt
+        if d["clock"] >= self.P["pre_corr"]:
+            d["A"], d["B"], d["clock"] = d["B"], self.stirring.next(), 0.0
+
+    def _pre_step(self):
+        dt = self._compute_dt()
+        self._hydro(dt, 0, 0.0, 0.0)
+        self._stir(dt, self.P["pre_amp"])
+        self.tp += dt
+        self.ip += 1
+        self.dt = dt
+        if self.tp >= self.P["pre_t"]:
+            a, b = self.codec.mach_sums(self.rho, [self.mx, self.my, self.mz])
+            self.mach = math.sqrt(a / b) * self.P["ci_over_cn"]
+            if self.P["drive_main"] <= 0:
+                self.drv = None
+            self._start_main()
 
     def q_of(self, i):
         s = self.stars[i]
@@ -283,12 +346,18 @@ class OracleSim:
     def step(self, k=1):
         P = self.P
         for _ in range(k):
+            if self.phase == "pre":
+                self._pre_step()
+                self.version += 1
+                continue
             if self.it % self.C["ion_every"] == 0:
                 self.ionize()
             self._set_star_weights()
             dt = self._compute_dt()
             pulse = max(0.0, 1.0 + P["jet_pulse"] * math.sin(2 * math.pi * self.t / P["jet_period"]))
             self._hydro(dt, len(self.stars), P["wind"], pulse)
+            if P["drive_main"] > 0 and self.drv is not None:
+                self._stir(dt, P["pre_amp"] * P["drive_main"])
             self.t += dt
             self.it += 1
             self.dt = dt
