@@ -84,66 +84,8 @@ def copy_calibration(nbytes):
     print(json.dumps({"copy_bytes": n * 4, "ms": dt * 1e3, "GBps_read_plus_write": 2 * n * 4 / dt / 1e9}), flush=True)
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=4)
-    ap.add_argument("--config", type=int, default=2)
-    ap.add_argument("--n", type=int, default=None, help="override the grid size (testing only)")
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp32-fast", "fp16-vel"],
-                    help="the form whose throughput is `value` (fp32 = the bit-exact drop-in)")
-    ap.add_argument("--no-variants", action="store_true", help="do not also time the other two forms")
-    ap.add_argument("--batch", type=int, default=64,
-                    help="members of the uneven-batch variant (0: skip it; timed with the variants)")
-    ap.add_argument("--copy-bytes", type=int, default=0,
-                    help="counter calibration: run a plain device-to-device copy of this many bytes 3 times, exit")
-    a = ap.parse_args()
-    if a.copy_bytes:
-        copy_calibration(a.copy_bytes)
-        return
-    if a.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not supported (the step has no "
-              "multi-GPU path)", file=sys.stderr)
-        sys.exit(2)
-
+def make_measure_batch(a, S, KernelTimer):
     import torch
-    from nebulaviewer_amd import scenarios as S
-    from nebulaviewer_amd.sim import Sim
-    from nebulaviewer_amd.timing import KernelTimer
-
-    scenario, n, seed = S.CONFIGS[a.config]
-    n = a.n or n
-    t0 = time.perf_counter()
-    setup = S.make_setup(scenario, n, seed)
-    log(f"[bench] {scenario} {n}^3 initial conditions in {time.perf_counter() - t0:.1f} s")
-    def measure(precision):
-        sim = Sim(setup=setup, verbose=False, precision=precision)
-        log(f"[bench] {precision}: warmup {a.warmup} steps, timing {a.steps} steps")
-        sim.step(a.warmup)
-        torch.cuda.synchronize()
-        timer = KernelTimer()
-        sim.timer = timer
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        sim.step(a.steps)
-        torch.cuda.synchronize()
-        wall = time.perf_counter() - t0
-        sim.timer = None
-        ks = timer.summary()
-        cells = math.prod(sim.shape)
-        hyd_s = ks["hydro_stage"]["avg_ms"] / 1e3
-        achieved = BYTES_PER_CELL_STEP[precision] / 2 * cells / hyd_s / 1e9
-        r = dict(value=cells * a.steps / wall, ms_per_step=wall / a.steps * 1e3, hydro_launch_ms=hyd_s * 1e3,
-                 hydro_only_cell_updates_per_s=cells / (2 * hyd_s), hydro_achieved_GBps=achieved,
-                 kernels_ms={k: round(v["avg_ms"], 4) for k, v in ks.items()}, ion_every=sim.ion_every,
-                 rays=tuple(sim.rfronts.shape[1:]), nstars=len(sim.stars))
-        log(f"[bench] {precision}: {r['value'] / 1e9:.2f} G cell-updates/s, {r['ms_per_step']:.3f} ms/step, "
-            f"hydro {hyd_s * 1e3:.3f} ms/launch = {achieved:.0f} GB/s; kernels {r['kernels_ms']}")
-        del sim
-        torch.cuda.empty_cache()
-        return r
 
     def measure_batch(count):
         from nebulaviewer_amd.batch import SimBatch
@@ -176,6 +118,97 @@ def main():
         del b
         torch.cuda.empty_cache()
         return r
+
+    return measure_batch
+
+
+def batch_only(a):
+    """--config 4: the uneven batch as the measured workload (one JSON line of its own)"""
+    from nebulaviewer_amd import scenarios as S
+    from nebulaviewer_amd.timing import KernelTimer
+    r = make_measure_batch(a, S, KernelTimer)(a.batch or 64)
+    out = {
+        "metric": f"cell-updates/s, batch of {r['members']} uneven grids (FP32 step)",
+        "value": r["value"], "unit": "cell-updates/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": r["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "precision": "fp32", "data": "synthetic (seeded initial conditions generated on the host)",
+        "config": {"workload": f"batch of {r['members']} members (torus / sheet, N in {r['sizes']}), HLL+minmod "
+                               "RK2, CFL 0.3, ionization every 2 steps with 512x1024 rays, seeds 100+i",
+                   "cells": r["cells"], "stars": r["stars"]},
+        "roofline": {"bound": "hbm", "kernel": "hydro_stage_batch_kernel", "achieved": r["hydro_achieved_GBps"],
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": r["hydro_achieved_GBps"] / HBM_PEAK_GBS,
+                     "traffic": None, "algorithmic_bytes_per_launch": BYTES_PER_CELL_STEP["fp32"] / 2 * r["cells"]},
+        "hydro_only_cell_updates_per_s": r["hydro_only_cell_updates_per_s"], "kernels_ms": r["kernels_ms"],
+        "variants": {}, "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--config", type=int, default=2, help="0-3: one grid of scenarios.CONFIGS; 4: the uneven batch")
+    ap.add_argument("--n", type=int, default=None, help="override the grid size (testing only)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp32-fast", "fp16-vel"],
+                    help="the form whose throughput is `value` (fp32 = the bit-exact drop-in)")
+    ap.add_argument("--no-variants", action="store_true", help="do not also time the other two forms")
+    ap.add_argument("--batch", type=int, default=64,
+                    help="members of the uneven-batch variant (0: skip it; timed with the variants)")
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="counter calibration: run a plain device-to-device copy of this many bytes 3 times, exit")
+    a = ap.parse_args()
+    if a.copy_bytes:
+        copy_calibration(a.copy_bytes)
+        return
+    if a.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not supported (the step has no "
+              "multi-GPU path)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from nebulaviewer_amd import scenarios as S
+    from nebulaviewer_amd.sim import Sim
+    from nebulaviewer_amd.timing import KernelTimer
+
+    if a.config == 4:
+        batch_only(a)
+        return
+    scenario, n, seed = S.CONFIGS[a.config]
+    n = a.n or n
+    t0 = time.perf_counter()
+    setup = S.make_setup(scenario, n, seed)
+    log(f"[bench] {scenario} {n}^3 initial conditions in {time.perf_counter() - t0:.1f} s")
+    def measure(precision):
+        sim = Sim(setup=setup, verbose=False, precision=precision)
+        log(f"[bench] {precision}: warmup {a.warmup} steps, timing {a.steps} steps")
+        sim.step(a.warmup)
+        torch.cuda.synchronize()
+        timer = KernelTimer()
+        sim.timer = timer
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sim.step(a.steps)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        sim.timer = None
+        ks = timer.summary()
+        cells = math.prod(sim.shape)
+        hyd_s = ks["hydro_stage"]["avg_ms"] / 1e3
+        achieved = BYTES_PER_CELL_STEP[precision] / 2 * cells / hyd_s / 1e9
+        r = dict(value=cells * a.steps / wall, ms_per_step=wall / a.steps * 1e3, hydro_launch_ms=hyd_s * 1e3,
+                 hydro_only_cell_updates_per_s=cells / (2 * hyd_s), hydro_achieved_GBps=achieved,
+                 kernels_ms={k: round(v["avg_ms"], 4) for k, v in ks.items()}, ion_every=sim.ion_every,
+                 rays=tuple(sim.rfronts.shape[1:]), nstars=len(sim.stars))
+        log(f"[bench] {precision}: {r['value'] / 1e9:.2f} G cell-updates/s, {r['ms_per_step']:.3f} ms/step, "
+            f"hydro {hyd_s * 1e3:.3f} ms/launch = {achieved:.0f} GB/s; kernels {r['kernels_ms']}")
+        del sim
+        torch.cuda.empty_cache()
+        return r
+
+    measure_batch = make_measure_batch(a, S, KernelTimer)
 
     main_r = measure(a.precision)
     variants = {}
