@@ -149,6 +149,16 @@ int nv_hydro_stage_batch_plan(const NvStageDesc* members, int n, int precision, 
                               void* plan, long long plan_bytes);
 int nv_hydro_stage_batch(const void* plan_host, const void* plan_dev, void* stream);
 
+/* Mass budget in FP64 on the GPU (the north star's global conservation sums): out[0] += sum of rho over the grid,
+ * out[1] += the net outward mass flux through the six domain faces, each face's float flux (the one the exact
+ * stage adds to the boundary cell: with clamped ghosts both reconstructed states are the cell's own) summed in
+ * double. For a stage out = a U0 + b (U + dt dU) that the density floor does not touch,
+ * sum rho_out = a sum rho0 + b (sum rho - (dt / dx) out[1]) up to the rounding of each cell's update. The caller
+ * zeroes out[0..1]. Replaces the d_bflux accumulation a conservation check needs; no single reference line. */
+int nv_conservation_sums(const float* rho, const float* mx, const float* my, const float* mz, const float* xi,
+                         int nx, int ny, int nz, float floor_, float cn2, float ci2, int solver, double* out,
+                         void* stream);
+
 /* Device-side timestep of n grids, so a step needs no host round trip: for each s,
  *   dt = cfl_dx[s] / (double)smax[s]  (cfl_dx = cfl * dx computed on the host in double, as sim.py:1011 does),
  *   dt = min(dt, dt_cap[s]) if dt_cap is non-null and dt_cap[s] > 0,

@@ -568,6 +568,73 @@ __global__ __launch_bounds__(NT, 4) void hydro_stage_batch_kernel(const HydroArg
   stage_tile<TL, ST, AR, SOLVER, LIM, SMAX, STAGE1>(args[lo], t - start[lo]);
 }
 
+// ------------------------------------------------------------------------------------------ conservation sums
+// Mass budget of a stage in FP64: out[0] += sum rho, out[1] += sum over the six domain faces of the outward mass
+// flux of the stage's face flux there. At a domain face the clamped ghosts make every limiter's slope zero, so
+// both reconstructed states equal the boundary cell's primitive state and the face flux is face_flux(q, q) with
+// that cell's c^2: the value the stage adds to the cell, in the same float arithmetic. Over a stage
+// (a = 0, b = 1), sum rho_out = sum rho - (dt / dx) out[1] up to the rounding of each cell's update and the density
+// floor, since every interior face flux enters two cells with opposite signs.
+__global__ __launch_bounds__(256) void mass_sum_kernel(const float* __restrict__ rho, long long n, double* out) {
+  __shared__ double red[4];
+  double a = 0.0;
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long long)gridDim.x * blockDim.x)
+    a += (double)rho[q];
+  for (int o = 32; o > 0; o >>= 1) a += __shfl_down(a, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = a;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicAdd(&out[0], ((red[0] + red[1]) + red[2]) + red[3]);
+}
+
+template <int SOLVER>
+__device__ __forceinline__ float boundary_mass_flux(const float* rho, const float* mx, const float* my,
+                                                    const float* mz, const float* xi, long long c, int axis,
+                                                    float floor_, float cn2, float dci) {
+  const float r = fmaxf(rho[c], floor_);
+  const double rr = recip_d(r);
+  const float4 q = make_float4(r, div_r(mx[c], rr), div_r(my[c], rr), div_r(mz[c], rr));
+  const float c2 = xi[c] * dci + cn2;
+  const float cs = sqrtf(c2);
+  const float4 F = axis == 0   ? face_flux<Exact, 0, SOLVER>(q, q, c2, c2, cs, cs, floor_)
+                   : axis == 1 ? face_flux<Exact, 1, SOLVER>(q, q, c2, c2, cs, cs, floor_)
+                               : face_flux<Exact, 2, SOLVER>(q, q, c2, c2, cs, cs, floor_);
+  return F.x;
+}
+
+template <int SOLVER>
+__global__ __launch_bounds__(256) void boundary_flux_kernel(const float* __restrict__ rho, const float* __restrict__ mx,
+                                                            const float* __restrict__ my, const float* __restrict__ mz,
+                                                            const float* __restrict__ xi, int nx, int ny, int nz,
+                                                            float floor_, float cn2, float dci, double* out) {
+  __shared__ double red[4];
+  const long long fx = (long long)ny * nz, fy = (long long)nx * nz, fz = (long long)nx * ny;
+  const long long total = 2 * (fx + fy + fz);
+  double a = 0.0;
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+       t += (long long)gridDim.x * blockDim.x) {
+    long long u = t;
+    int axis, side, i, j, k;
+    if (u < 2 * fx) {
+      axis = 0; side = (int)(u / fx); u %= fx;
+      i = side ? nx - 1 : 0; j = (int)(u / nz); k = (int)(u % nz);
+    } else if ((u -= 2 * fx) < 2 * fy) {
+      axis = 1; side = (int)(u / fy); u %= fy;
+      j = side ? ny - 1 : 0; i = (int)(u / nz); k = (int)(u % nz);
+    } else {
+      u -= 2 * fy;
+      axis = 2; side = (int)(u / fz); u %= fz;
+      k = side ? nz - 1 : 0; i = (int)(u / ny); j = (int)(u % ny);
+    }
+    const long long c = cell(i, j, k, ny, nz);
+    const float f = boundary_mass_flux<SOLVER>(rho, mx, my, mz, xi, c, axis, floor_, cn2, dci);
+    a += side ? (double)f : -(double)f;       // outward: + on the high faces, - on the low faces
+  }
+  for (int o = 32; o > 0; o >>= 1) a += __shfl_down(a, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = a;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicAdd(&out[1], ((red[0] + red[1]) + red[2]) + red[3]);
+}
+
 // Device-side timestep of n grids (Sim._compute_dt, sim.py:1009-1012, and the clock of sim.py:1099-1114), in
 // the host's double arithmetic: dt = (cfl dx) / smax, capped, rounded to float for the stages; t += dt; and smax
 // is cleared for the next reduction.
@@ -901,6 +968,26 @@ extern "C" int nv_hydro_stage_batch(const void* plan_host, const void* plan_dev,
   hipLaunchKernelGGL(kern, dim3(8 * H.per_xcd), dim3(tile_tk(H.tile), tile_tj(H.tile), 1), 0, as_stream(stream), args,
                      start, H.n, H.total, H.per_xcd);
   return check_launch("nv_hydro_stage_batch");
+}
+
+extern "C" int nv_conservation_sums(const float* rho, const float* mx, const float* my, const float* mz,
+                                    const float* xi, int nx, int ny, int nz, float floor_, float cn2, float ci2,
+                                    int solver, double* out, void* stream) {
+  using namespace nv;
+  clear_error();
+  NV_REQUIRE(rho && mx && my && mz && xi && out, "nv_conservation_sums: null pointer");
+  NV_REQUIRE(nx > 0 && ny > 0 && nz > 0, "nv_conservation_sums: bad grid");
+  NV_REQUIRE(solver == NV_SOLVER_RUSANOV || solver == NV_SOLVER_HLL, "nv_conservation_sums: bad solver %d", solver);
+  NV_REQUIRE(floor_ > 0.0f, "nv_conservation_sums: floor must be > 0");
+  const long long n = (long long)nx * ny * nz;
+  hipLaunchKernelGGL(mass_sum_kernel, dim3((int)std::min<long long>((n + 255) / 256, 4096)), dim3(256), 0,
+                     as_stream(stream), rho, n, out);
+  const long long faces = 2 * ((long long)ny * nz + (long long)nx * nz + (long long)nx * ny);
+  const int blocks = (int)std::min<long long>((faces + 255) / 256, 4096);
+  auto kern = solver == NV_SOLVER_HLL ? boundary_flux_kernel<NV_SOLVER_HLL> : boundary_flux_kernel<NV_SOLVER_RUSANOV>;
+  hipLaunchKernelGGL(kern, dim3(blocks), dim3(256), 0, as_stream(stream), rho, mx, my, mz, xi, nx, ny, nz, floor_, cn2,
+                     ci2 - cn2, out);
+  return check_launch("nv_conservation_sums");
 }
 
 extern "C" int nv_cfl_dt(float* smax, const double* cfl_dx, const double* dt_cap, float* dt_f, double* dt_d,

@@ -486,3 +486,46 @@ def test_mass_change_equals_boundary_flux_512(env):
     tol = 6 * np.sqrt(np.sum((np.spacing(U[0]).astype(np.float64)) ** 2)) + 1e-9 * abs(expect)
     assert abs(dM - expect) <= tol, (dM, expect, tol)
     assert abs(expect) > 10 * tol   # the boundary flux is resolved, not lost in rounding
+    # the same budget from the GPU's FP64 sums: the float boundary fluxes of the stage against the host's FP64
+    # restatement (float vs double face arithmetic: 1e-6), and the mass change against them to FP32 rounding
+    sums_in = torch.zeros(2, dtype=torch.float64, device=dev)
+    L.call("nv_conservation_sums", *[t.data_ptr() for t in gU], gxi.data_ptr(), *shape, f(2e-3), f(cn2), f(1.0), 1,
+           sums_in.data_ptr(), stream(env))
+    sums_out = torch.zeros(2, dtype=torch.float64, device=dev)
+    L.call("nv_conservation_sums", *[t.data_ptr() for t in out], gxi.data_ptr(), *shape, f(2e-3), f(cn2), f(1.0), 1,
+           sums_out.data_ptr(), stream(env))
+    m_in, b_in = sums_in.tolist()
+    m_out = sums_out.tolist()[0]
+    assert abs(b_in - net) <= 1e-6 * abs(net), (b_in, net)
+    assert abs((m_out - m_in) - (-dt / dx * b_in)) <= tol
+    assert abs(m_in - float(gU[0].double().sum())) <= 1e-9 * m_in
+
+
+def test_rk2_step_mass_budget(env):
+    """one Sim step (both RK stages) on the 64^3 sheet (Mach-5 flow through every face; its wind adds momentum
+    only): M^{n+1} - M^n = -(dt/dx) (B(U^n) + B(U1)) / 2 from the GPU's FP64 sums, to FP32 rounding (the density
+    floor does not act)"""
+    from nebulaviewer_amd import scenarios as S
+    from nebulaviewer_amd.sim import Sim
+    torch, dev, L = env
+    say("[property] RK2 mass budget, sheet 64^3")
+    sim = Sim(setup=S.make_setup("sheet", 64, 1), verbose=False, rays=(64, 128), fuv_grid=(16, 16, 32))
+    sim.step(3)                                           # some flow through the faces
+
+    def sums(U):
+        o = torch.zeros(2, dtype=torch.float64, device=dev)
+        L.call("nv_conservation_sums", *[t.data_ptr() for t in U], sim.xi.data_ptr(), *sim.shape,
+               f(sim.P["floor"]), f(sim.cn2), f(sim.ci2), 1, o.data_ptr(), stream(env))
+        return o.tolist()
+
+    m0, b0 = sums(sim.U)
+    rho0 = sim.rho.cpu().numpy()
+    assert float(sim.rho.min()) > 10 * sim.P["floor"]
+    sim.step(1)
+    m1, b1 = sums([sim.stage[c] for c in range(4)])       # U1: the stage block after the step
+    m2 = sums(sim.U)[0]
+    dt = sim.dt
+    expect = -0.5 * float(np.float32(dt)) / sim.dx * (b0 + b1)
+    tol = 12 * np.sqrt(np.sum(np.spacing(rho0).astype(np.float64) ** 2))
+    assert abs((m2 - m0) - expect) <= tol, (m2 - m0, expect, tol)
+    assert abs(expect) > 10 * tol
