@@ -17,7 +17,8 @@ struct DriveArgs {
 };
 
 // a = amp * lerp(F, G, blend) at each cell centre, F and G sampled trilinearly on the stirring grid; m += rho amp
-// dt * lerp (nebula3d.py:170-179)
+// dt * lerp (nebula3d.py:170-179). The stirring grid is small (24 cells per unit plus guards): paired z loads and
+// 32-bit offsets.
 template <class SC>
 __global__ __launch_bounds__(256) void drive_kernel(DriveArgs A, SC cs) {
   const long long n = (long long)A.nx * A.ny * A.nz;
@@ -32,8 +33,8 @@ __global__ __launch_bounds__(256) void drive_kernel(DriveArgs A, SC cs) {
     const float* Fb[3] = {A.gx, A.gy, A.gz};
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      const float fa = sample_r<false, uint64_t>(Fa[c], A.sx, A.sy, A.sz, px, py, pz, cs);
-      const float fb = sample_r<false, uint64_t>(Fb[c], A.sx, A.sy, A.sz, px, py, pz, cs);
+      const float fa = sample_r<true, uint32_t>(Fa[c], A.sx, A.sy, A.sz, px, py, pz, cs);
+      const float fb = sample_r<true, uint32_t>(Fb[c], A.sx, A.sy, A.sz, px, py, pz, cs);
       f[c] = (1.0f - A.blend) * fa + A.blend * fb;
     }
     const float r = A.rho[q] * A.amp * A.dt;
@@ -119,7 +120,9 @@ extern "C" int nv_drive(float* rho, float* mx, float* my, float* mz, int nx, int
                         int sy, int sz, float blend, float amp, float dt, float dx, float dxs, void* stream) {
   clear_error();
   NV_REQUIRE(rho && mx && my && mz && fx && fy && fz && gx && gy && gz, "nv_drive: null pointer");
-  NV_REQUIRE(nx > 0 && ny > 0 && nz > 0 && sx > 0 && sy > 0 && sz > 0, "nv_drive: bad sizes");
+  NV_REQUIRE(nx > 0 && ny > 0 && nz > 0 && sx > 0 && sy > 0 && sz >= 2,
+             "nv_drive: bad sizes (the stirring grid needs sz >= 2)");
+  NV_REQUIRE((long long)sx * sy * sz <= (1LL << 30), "nv_drive: stirring grid above 2^30 cells");
   NV_REQUIRE(dx > 0.0f && dxs >= 1.17549435e-38f && dxs <= 3.40282347e+38f, "nv_drive: dx, dxs must be > 0");
   DriveArgs A{rho, mx, my, mz, fx, fy, fz, gx, gy, gz, nx, ny, nz, sx, sy, sz, blend, amp, dt, dx};
   const long long n = (long long)nx * ny * nz;
