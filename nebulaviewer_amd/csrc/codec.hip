@@ -71,7 +71,7 @@ struct UnpackArgs {
   long long n;
 };
 
-constexpr double R255 = 1.0 / 255.0;   // shared reciprocal: div_r(x, R255) is x / 255.0f
+constexpr Recip R255 = {1.0 / 255.0, 255.0f};   // div_r(x, R255) is x / 255.0f
 
 // k_unpack / k_unpack_lm (timeline.py:46-52, 84-89)
 template <bool HALF>
@@ -94,7 +94,7 @@ __global__ __launch_bounds__(256) void timeline_unpack_kernel(UnpackArgs A) {
 // r2dx = 1 / (2 dx) as the shared reciprocal of the float 2 dx
 __device__ __forceinline__ float shock_cell(const float* __restrict__ rho, const float* __restrict__ mx,
                                             const float* __restrict__ my, const float* __restrict__ mz, int i, int j,
-                                            int k, int nx, int ny, int nz, double r2dx) {
+                                            int k, int nx, int ny, int nz, const Recip& r2dx) {
   const int ip = min(i + 1, nx - 1), im = max(i - 1, 0), jp = min(j + 1, ny - 1), jm = max(j - 1, 0);
   const int kp = min(k + 1, nz - 1), km = max(k - 1, 0);
   const long long a = cell(ip, j, k, ny, nz), b = cell(im, j, k, ny, nz);
@@ -110,7 +110,7 @@ __device__ __forceinline__ float shock_cell(const float* __restrict__ rho, const
 
 __global__ __launch_bounds__(256) void shock_kernel(const float* __restrict__ rho, const float* __restrict__ mx,
                                                     const float* __restrict__ my, const float* __restrict__ mz,
-                                                    int nx, int ny, int nz, double r2dx, float* __restrict__ out) {
+                                                    int nx, int ny, int nz, Recip r2dx, float* __restrict__ out) {
   const long long n = (long long)nx * ny * nz;
   const long long stride = (long long)gridDim.x * blockDim.x;
   for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride) {
@@ -149,23 +149,23 @@ struct VolArgs {
   const float *rho, *mx, *my, *mz, *xi;
   const void *df, *g0;
   NvPackParams pp;
-  double r_gs, r_edge, r2dx;       // shared reciprocals of gs, edge (0 if edge <= 0) and 2 dx
+  Recip r_gs, r_edge, r2dx;        // gs, edge (r = 0 if edge <= 0) and 2 dx with their shared reciprocals
   int nx, ny, nz, shock_on;
   uint16_t* out;                   // binary16 x 4 per cell
 };
 
 // sstep(e0, e1, x) of render.py:121-124 with e1 - e0 = d and its shared reciprocal rd
-__device__ __forceinline__ float sstep_r(float e0, double rd, float x) {
+__device__ __forceinline__ float sstep_r(float e0, const Recip& rd, float x) {
   const float t = wclamp(div_r(x - e0, rd), 0.0f, 1.0f);
   return t * t * (3.0f - 2.0f * t);
 }
 
 // fade_axis(u, e) of render.py:127-130 (rde = 1/e, e > 0)
-__device__ __forceinline__ float fade_r(float u, double rde) { return sstep_r(0.0f, rde, u) * sstep_r(0.0f, rde, 1.0f - u); }
+__device__ __forceinline__ float fade_r(float u, const Recip& rde) { return sstep_r(0.0f, rde, u) * sstep_r(0.0f, rde, 1.0f - u); }
 
 constexpr float OIII_E0 = -0.03f, OIII_D = -0.12f - (-0.03f);   // sstep(-0.03, -0.12, df)
 constexpr float SII_E0 = -0.05f, SII_D = -0.005f - (-0.05f);    // sstep(-0.05, -0.005, df)
-constexpr double R_OIII = 1.0 / (double)OIII_D, R_SII = 1.0 / (double)SII_D;
+constexpr Recip R_OIII = {1.0 / (double)OIII_D, OIII_D}, R_SII = {1.0 / (double)SII_D, SII_D};
 
 template <bool HALF>
 __global__ __launch_bounds__(256) void pack_volume_kernel(VolArgs A) {
@@ -190,7 +190,7 @@ __global__ __launch_bounds__(256) void pack_volume_kernel(VolArgs A) {
     const float p0 = (((float)i + 0.5f) * pp.dx) * pp.inv_L[0];
     const float p1 = (((float)j + 0.5f) * pp.dx) * pp.inv_L[1];
     const float p2 = (((float)k + 0.5f) * pp.dx) * pp.inv_L[2];
-    const float f = A.r_edge != 0.0 ? (fade_r(p0, A.r_edge) * fade_r(p1, A.r_edge)) * fade_r(p2, A.r_edge) : 1.0f;
+    const float f = A.r_edge.r != 0.0 ? (fade_r(p0, A.r_edge) * fade_r(p1, A.r_edge)) * fade_r(p2, A.r_edge) : 1.0f;
     const float dg = dust * geff;
     float em[3];
 #pragma unroll
@@ -210,7 +210,7 @@ __global__ __launch_bounds__(256) void pack_volume_kernel(VolArgs A) {
   }
 }
 
-constexpr double R6C = 1.0 / 6.0;
+constexpr Recip R6C = {1.0 / 6.0, 6.0f};
 
 // smooth_axis (render.py:281-298): [1 4 1]/6 along one axis of a binary16 x 4 grid, clamped at the ends
 __global__ __launch_bounds__(256) void smooth_axis_kernel(const uint16_t* __restrict__ src, uint16_t* __restrict__ dst,
@@ -294,7 +294,7 @@ extern "C" int nv_shock_field(const float* rho, const float* mx, const float* my
   NV_REQUIRE(normal_pos(2.0f * dx), "nv_shock_field: dx must be a positive normal float");
   const long long n = (long long)nx * ny * nz;
   hipLaunchKernelGGL(shock_kernel, dim3(grid_for(n)), dim3(256), 0, as_stream(stream), rho, mx, my, mz, nx, ny, nz,
-                     1.0 / (double)(2.0f * dx), out);
+                     Recip{1.0 / (double)(2.0f * dx), 2.0f * dx}, out);
   return check_launch("nv_shock_field");
 }
 
@@ -327,9 +327,9 @@ extern "C" int nv_pack_volume(const float* rho, const float* mx, const float* my
   VolArgs A;
   A.rho = rho; A.mx = mx; A.my = my; A.mz = mz; A.xi = xi; A.df = dfront; A.g0 = g0;
   A.pp = *pp;
-  A.r_gs = 1.0 / (double)pp->gs;
-  A.r_edge = pp->edge > 0.0f ? 1.0 / (double)pp->edge : 0.0;
-  A.r2dx = 1.0 / (double)(2.0f * pp->dx);
+  A.r_gs = Recip{1.0 / (double)pp->gs, pp->gs};
+  A.r_edge = pp->edge > 0.0f ? Recip{1.0 / (double)pp->edge, pp->edge} : Recip{0.0, 0.0f};
+  A.r2dx = Recip{1.0 / (double)(2.0f * pp->dx), 2.0f * pp->dx};
   A.nx = nx; A.ny = ny; A.nz = nz; A.shock_on = shock_on ? 1 : 0;
   A.out = out;
   const long long n = (long long)nx * ny * nz;

@@ -38,16 +38,13 @@
 
 namespace nv {
 
-constexpr int NT = 512;                     // threads per workgroup, either tile shape
-
-// Tile shapes (z extent TK, y rows TJ, TK * TJ = NT): 64 x 8 (one wavefront per y row) and 32 x 16 (one
-// wavefront per two rows), the latter for z extents that are multiples of 32 but not of 64 and where its
-// smaller halo wins.
-constexpr int TILE_AUTO = 0, TILE_64x8 = 1, TILE_32x16 = 2;
+// Tile shapes (z extent TK, y rows TJ, one thread per cell column): 64 x 8 (one wavefront per y row), 32 x 16
+// (one wavefront per two rows), the latter for z extents that are multiples of 32 but not of 64 and where its
+// smaller halo wins, and 64 x 16 (1024 threads: half the halo share of 64 x 8, one workgroup per CU).
+constexpr int TILE_AUTO = 0, TILE_64x8 = 1, TILE_32x16 = 2, TILE_64x16 = 3;
 template <int TK_, int TJ_>
 struct Tile {
-  static constexpr int TK = TK_, TJ = TJ_;
-  static_assert(TK * TJ == NT, "a tile has NT threads");
+  static constexpr int TK = TK_, TJ = TJ_, NT = TK * TJ;
   static constexpr int PK = TK + 4, PJ = TJ + 4;     // primitive plane with a 2-cell halo
   static constexpr int CK = TK + 2, CJ = TJ + 2;     // c^2 / c plane with a 1-cell halo
   static constexpr int NHALO = 4 * PK + 4 * TJ;      // halo cells of the primitive plane
@@ -55,6 +52,7 @@ struct Tile {
 };
 using T64 = Tile<64, 8>;
 using T32 = Tile<32, 16>;
+using T6416 = Tile<64, 16>;
 
 struct HydroArgs {
   const void *f0, *f1, *f2, *f3;            // U: rho and momentum (F32) or rho and velocity (F16V)
@@ -73,20 +71,29 @@ struct HydroArgs {
 
 // ------------------------------------------------------------------------------------------ arithmetic
 struct Exact {
-  using Rcp = double;
-  static __device__ __forceinline__ Rcp rcp(float b) { return recip_d(b); }
-  static __device__ __forceinline__ float div(float a, Rcp r) { return div_r(a, r); }
+  static constexpr bool FAST = false;
+  using Rcp = Recip;
+  static __device__ __forceinline__ Rcp rcp(float b) { return recip(b); }
+  static __device__ __forceinline__ float div(float a, const Rcp& r) { return div_r(a, r); }
+  template <int N>
+  static __device__ __forceinline__ void divn(const float (&a)[N], const Rcp& r, float (&q)[N]) { div_rn<N>(a, r, q); }
   static __device__ __forceinline__ float mad(float a, float b, float c) { return a * b + c; }
   static __device__ __forceinline__ float sqrt(float x) { return sqrtf(x); }
 };
 
 struct Fast {
+  static constexpr bool FAST = true;
   using Rcp = float;
   static __device__ __forceinline__ Rcp rcp(float b) {
     const float r = __builtin_amdgcn_rcpf(b);
     return fmaf(r, fmaf(-b, r, 1.0f), r);
   }
   static __device__ __forceinline__ float div(float a, Rcp r) { return a * r; }
+  template <int N>
+  static __device__ __forceinline__ void divn(const float (&a)[N], Rcp r, float (&q)[N]) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) q[i] = a[i] * r;
+  }
   static __device__ __forceinline__ float mad(float a, float b, float c) { return fmaf(a, b, c); }
   static __device__ __forceinline__ float sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 };
@@ -110,7 +117,10 @@ struct F32 {
     ((float*)o2)[o] = m1;
     ((float*)o3)[o] = m2;
     if (!want_v) return make_float3(0.0f, 0.0f, 0.0f);
-    return make_float3(AR::div(m0, rr), AR::div(m1, rr), AR::div(m2, rr));
+    const float m[3] = {m0, m1, m2};
+    float v[3];
+    AR::template divn<3>(m, rr, v);
+    return make_float3(v[0], v[1], v[2]);
   }
 };
 
@@ -142,7 +152,10 @@ __device__ __forceinline__ void decode(const float4& raw, float floor_, float4& 
   } else {
     const float r = fmaxf(raw.x, floor_);
     const typename AR::Rcp rr = AR::rcp(r);
-    q = make_float4(r, AR::div(raw.y, rr), AR::div(raw.z, rr), AR::div(raw.w, rr));
+    const float m[3] = {raw.y, raw.z, raw.w};
+    float v[3];
+    AR::template divn<3>(m, rr, v);
+    q = make_float4(r, v[0], v[1], v[2]);
     U = raw;
   }
 }
@@ -155,11 +168,18 @@ __device__ __forceinline__ float4 conserved(const float4& raw) {
 
 // ------------------------------------------------------------------------------------------ physics
 // slope limiters: minmod (nebula3d.py:40-46), van Leer (sim.py:375-380), MC (sim.py:366-372)
-template <int LIM>
+template <class AR, int LIM>
 __device__ __forceinline__ float slope(float a, float b) {
+  if constexpr (AR::FAST && LIM == NV_LIM_MINMOD) {
+    // the median of (a, b, 0) is minmod(a, b): 0 for opposite signs or a zero operand, else the operand nearer
+    // zero. It differs from the product test only where a*b underflows to zero (|slope| < 2^-75).
+    return __builtin_amdgcn_fmed3f(a, b, 0.0f);
+  }
   const float p = a * b;
   if constexpr (LIM == NV_LIM_VANLEER) {
-    const float v = 2.0f * a * b / (a + b);
+    float v;
+    if constexpr (AR::FAST) v = AR::div(2.0f * a * b, AR::rcp(a + b));   // a + b = 0 only where p < 0
+    else v = 2.0f * a * b / (a + b);
     return p <= 0.0f ? 0.0f : v;
   } else if constexpr (LIM == NV_LIM_MC) {
     const float s = a > 0.0f ? 1.0f : -1.0f;
@@ -187,7 +207,7 @@ __device__ __forceinline__ void edges(const float4& qm, const float4& q, const f
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
     const float v = comp(q, c);
-    const float sl = slope<LM>(v - comp(qm, c), comp(qp, c) - v);
+    const float sl = slope<AR, LM>(v - comp(qm, c), comp(qp, c) - v);
     l[c] = AR::mad(-0.5f, sl, v);
     h[c] = AR::mad(0.5f, sl, v);
   }
@@ -223,6 +243,18 @@ __device__ __forceinline__ float4 face_flux(const float4& Lq, const float4& Rq, 
     const float ha = 0.5f * amax;
 #pragma unroll
     for (int c = 0; c < 4; ++c) F[c] = AR::mad(-ha, UR[c] - UL[c], 0.5f * (FL[c] + FR[c]));
+  } else if constexpr (AR::FAST) {
+    // SL clamped to <= 0 and SR to >= 0: the one HLL formula then gives FL where SL >= 0 and FR where SR <= 0
+    // (to the rounding of SR FL / SR), with no three-way select. SR - SL is 0 only for c = 0 and unL = unR = 0,
+    // where every term of the numerator is 0 as well.
+    const float SL = fminf(fminf(unL - cb, unR - cc), 0.0f);
+    const float SR = fmaxf(fmaxf(unL + cb, unR + cc), 0.0f);
+    const float ss = SL * SR;
+    const typename AR::Rcp rden = AR::rcp(fmaxf(SR - SL, 1.17549435e-38f));
+    float N[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) N[c] = AR::mad(ss, UR[c] - UL[c], AR::mad(SR, FL[c], -(SL * FR[c])));
+    AR::template divn<4>(N, rden, F);
   } else {
     const float SL = fminf(unL - cb, unR - cc);
     const float SR = fmaxf(unL + cb, unR + cc);
@@ -235,9 +267,10 @@ __device__ __forceinline__ float4 face_flux(const float4& Lq, const float4& Rq, 
     } else {
       const float ss = SL * SR;
       const typename AR::Rcp rden = AR::rcp(SR - SL);
+      float N[4];
 #pragma unroll
-      for (int c = 0; c < 4; ++c)
-        F[c] = AR::div(AR::mad(ss, UR[c] - UL[c], AR::mad(SR, FL[c], -(SL * FR[c]))), rden);
+      for (int c = 0; c < 4; ++c) N[c] = AR::mad(ss, UR[c] - UL[c], AR::mad(SR, FL[c], -(SL * FR[c])));
+      AR::template divn<4>(N, rden, F);
     }
   }
   return make_float4(F[0], F[1], F[2], F[3]);
@@ -248,7 +281,7 @@ __device__ __forceinline__ float4 face_flux(const float4& Lq, const float4& Rq, 
 template <class TL, class ST, class AR, int SOLVER, int LIM, bool SMAX, bool STAGE1>
 __device__ __forceinline__ void stage_tile(const HydroArgs& A, int tile) {
   constexpr int TK = TL::TK, TJ = TL::TJ, PK = TL::PK, PJ = TL::PJ, CK = TL::CK, CJ = TL::CJ;
-  constexpr int NHALO = TL::NHALO, NCHALO = TL::NCHALO;
+  constexpr int NHALO = TL::NHALO, NCHALO = TL::NCHALO, NT = TL::NT;
   constexpr int LM = SOLVER == NV_SOLVER_HLL ? LIM : NV_LIM_MINMOD;   // Rusanov always uses minmod
   using Rcp = typename AR::Rcp;
   __shared__ float4 sq[PJ * PK];            // primitives of plane i, 2-cell halo
@@ -431,10 +464,13 @@ __device__ __forceinline__ void stage_tile(const HydroArgs& A, int tile) {
       const float4 Uc = Uraw[0];
       const float U[4] = {Uc.x, Uc.y, Uc.z, Uc.w};
       float dU[4];
-#pragma unroll
-      for (int c = 0; c < 4; ++c) {
+      if (A.inv_dx != 0.0f) {
         // dx = 2^e: -div * 2^-e is the same real number as -div / dx, so one rounding gives the same float
-        dU[c] = A.inv_dx != 0.0f ? -div[c] * A.inv_dx : AR::div(-div[c], rdx);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) dU[c] = -div[c] * A.inv_dx;
+      } else {
+        const float nd[4] = {-div[0], -div[1], -div[2], -div[3]};
+        AR::template divn<4>(nd, rdx, dU);
       }
       const float p0 = ((float)i + 0.5f) * dx, p1 = ((float)j + 0.5f) * dx, p2 = ((float)k + 0.5f) * dx;
       const float x_ion = xraw[0];
@@ -497,7 +533,10 @@ __device__ __forceinline__ void stage_tile(const HydroArgs& A, int tile) {
       }
       const float r = fmaxf(out[0], floor_);
       const Rcp rr = AR::rcp(r);
-      float v0 = AR::div(out[1], rr), v1 = AR::div(out[2], rr), v2 = AR::div(out[3], rr);
+      const float mo[3] = {out[1], out[2], out[3]};
+      float vo[3];
+      AR::template divn<3>(mo, rr, vo);
+      float v0 = vo[0], v1 = vo[1], v2 = vo[2];
       const float s2 = AR::mad(v2, v2, AR::mad(v1, v1, v0 * v0));
       if (s2 > A.vmax2_lo) {
         // |v| = sqrt(s2) may exceed vmax: the reference test and cap (sim.py:491-494)
@@ -545,7 +584,7 @@ __device__ __forceinline__ void stage_tile(const HydroArgs& A, int tile) {
 __device__ __forceinline__ int xcd_tile(int per_xcd) { return (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3); }
 
 template <class TL, class ST, class AR, int SOLVER, int LIM, bool SMAX, bool STAGE1>
-__global__ __launch_bounds__(NT, 4) void hydro_stage_kernel(HydroArgs A) {
+__global__ __launch_bounds__(TL::NT, 4) void hydro_stage_kernel(HydroArgs A) {
   const int tile = xcd_tile(A.per_xcd);
   if (tile >= A.tiles) return;
   stage_tile<TL, ST, AR, SOLVER, LIM, SMAX, STAGE1>(A, tile);
@@ -554,7 +593,7 @@ __global__ __launch_bounds__(NT, 4) void hydro_stage_kernel(HydroArgs A) {
 // A batch of independent grids in one launch: tile t of the flat list belongs to the member s with
 // start[s] <= t < start[s+1] (start[0] = 0, start[n] = total), whose arguments are args[s].
 template <class TL, class ST, class AR, int SOLVER, int LIM, bool SMAX, bool STAGE1>
-__global__ __launch_bounds__(NT, 4) void hydro_stage_batch_kernel(const HydroArgs* __restrict__ args,
+__global__ __launch_bounds__(TL::NT, 4) void hydro_stage_batch_kernel(const HydroArgs* __restrict__ args,
                                                                   const int* __restrict__ start, int n, int total,
                                                                   int per_xcd) {
   const int t = xcd_tile(per_xcd);
@@ -591,8 +630,10 @@ __device__ __forceinline__ float boundary_mass_flux(const float* rho, const floa
                                                     const float* mz, const float* xi, long long c, int axis,
                                                     float floor_, float cn2, float dci) {
   const float r = fmaxf(rho[c], floor_);
-  const double rr = recip_d(r);
-  const float4 q = make_float4(r, div_r(mx[c], rr), div_r(my[c], rr), div_r(mz[c], rr));
+  const float m[3] = {mx[c], my[c], mz[c]};
+  float v[3];
+  div_rn<3>(m, recip(r), v);
+  const float4 q = make_float4(r, v[0], v[1], v[2]);
   const float c2 = xi[c] * dci + cn2;
   const float cs = sqrtf(c2);
   const float4 F = axis == 0   ? face_flux<Exact, 0, SOLVER>(q, q, c2, c2, cs, cs, floor_)
@@ -705,17 +746,21 @@ static HydroKernel pick_mode_tl(Mode mode, int solver, int lim, bool sm, bool st
 }
 
 static HydroKernel pick_mode(int tile, Mode mode, int solver, int lim, bool sm, bool stage1) {
-  return tile == TILE_32x16 ? pick_mode_tl<T32>(mode, solver, lim, sm, stage1)
-                            : pick_mode_tl<T64>(mode, solver, lim, sm, stage1);
+  return tile == TILE_32x16   ? pick_mode_tl<T32>(mode, solver, lim, sm, stage1)
+         : tile == TILE_64x16 ? pick_mode_tl<T6416>(mode, solver, lim, sm, stage1)
+                              : pick_mode_tl<T64>(mode, solver, lim, sm, stage1);
 }
 
-static int tile_tk(int tile) { return tile == TILE_32x16 ? T32::TK : T64::TK; }
-static int tile_tj(int tile) { return tile == TILE_32x16 ? T32::TJ : T64::TJ; }
+static int tile_tk(int tile) { return tile == TILE_32x16 ? T32::TK : tile == TILE_64x16 ? T6416::TK : T64::TK; }
+static int tile_tj(int tile) { return tile == TILE_32x16 ? T32::TJ : tile == TILE_64x16 ? T6416::TJ : T64::TJ; }
 
-// The shape for a z extent when the caller leaves it to the library: 64 x 8 when nz is a multiple of 64 (no idle
-// lanes), 32 x 16 when it is a multiple of 32 only, otherwise the one with fewer idle lanes.
-static int auto_tile(int nz) {
-  if (nz % 64 == 0) return TILE_64x8;
+// The shape when the caller leaves it to the library. By idle lanes: 64 x 8 when nz is a multiple of 64, 32 x 16
+// when it is a multiple of 32 only, otherwise the one with fewer idle lanes. The exact second stage, which also
+// reads U^n and so moves the most bytes, takes 64 x 16 on z extents that are multiples of 64 (half the halo
+// re-reads; measured at 512^3: 2.65 against 2.88 ms); the instruction-bound launches keep the two resident
+// 512-thread workgroups per CU of 64 x 8.
+static int auto_tile(int ny, int nz, Mode mode, bool stage1) {
+  if (nz % 64 == 0) return (mode == Mode::F32_EXACT && !stage1 && ny >= 16) ? TILE_64x16 : TILE_64x8;
   const int idle64 = ((nz + 63) / 64) * 64 - nz, idle32 = ((nz + 31) / 32) * 32 - nz;
   return idle32 < idle64 ? TILE_32x16 : TILE_64x8;
 }
@@ -771,8 +816,11 @@ static bool is_stage1(const HydroArgs& A) { return A.u0_is_u && A.a == 0.0f && A
 
 static int launch_desc(Mode mode, const char* name, const NvStageDesc& d, int solver, int lim, int tile,
                        void* stream) {
-  NV_REQUIRE(tile >= TILE_AUTO && tile <= TILE_32x16, "%s: bad tile shape %d", name, tile);
-  if (tile == TILE_AUTO) tile = auto_tile(d.nz);
+  NV_REQUIRE(tile >= TILE_AUTO && tile <= TILE_64x16, "%s: bad tile shape %d", name, tile);
+  if (tile == TILE_AUTO) {
+    const bool u0u = d.u0[0] == d.u[0] && d.u0[1] == d.u[1] && d.u0[2] == d.u[2] && d.u0[3] == d.u[3];
+    tile = auto_tile(d.ny, d.nz, mode, u0u && d.a == 0.0f && d.b == 1.0f);
+  }
   HydroArgs A;
   const int rc = fill_stage(name, d, solver, lim, tile, tiles_jk(tile, d.ny, d.nz), A);
   if (rc != NV_OK) return rc;
@@ -911,7 +959,7 @@ extern "C" int nv_hydro_stage_batch_plan(const NvStageDesc* members, int n, int 
              nv::plan_bytes(n), plan_bytes);
   NV_REQUIRE(precision >= NV_PREC_F32 && precision <= NV_PREC_F16V, "nv_hydro_stage_batch_plan: bad precision %d",
              precision);
-  NV_REQUIRE(tile >= TILE_AUTO && tile <= TILE_32x16, "nv_hydro_stage_batch_plan: bad tile shape %d", tile);
+  NV_REQUIRE(tile >= TILE_AUTO && tile <= TILE_64x16, "nv_hydro_stage_batch_plan: bad tile shape %d", tile);
   long long jk_total = 0;
   for (int s = 0; s < n; ++s) {
     NV_REQUIRE(members[s].nx > 0 && members[s].ny > 0 && members[s].nz > 0,
@@ -963,8 +1011,9 @@ extern "C" int nv_hydro_stage_batch(const void* plan_host, const void* plan_dev,
   const HydroArgs* args = reinterpret_cast<const HydroArgs*>(static_cast<const char*>(plan_dev) + sizeof(BatchHeader));
   const int* start = reinterpret_cast<const int*>(args + H.n);
   const bool sm = H.smax != 0, st1 = H.stage1 != 0;
-  BatchKernel kern = H.tile == TILE_32x16 ? pick_batch_tl<T32>(H.mode, H.solver, H.lim, sm, st1)
-                                          : pick_batch_tl<T64>(H.mode, H.solver, H.lim, sm, st1);
+  BatchKernel kern = H.tile == TILE_32x16   ? pick_batch_tl<T32>(H.mode, H.solver, H.lim, sm, st1)
+                     : H.tile == TILE_64x16 ? pick_batch_tl<T6416>(H.mode, H.solver, H.lim, sm, st1)
+                                            : pick_batch_tl<T64>(H.mode, H.solver, H.lim, sm, st1);
   hipLaunchKernelGGL(kern, dim3(8 * H.per_xcd), dim3(tile_tk(H.tile), tile_tj(H.tile), 1), 0, as_stream(stream), args,
                      start, H.n, H.total, H.per_xcd);
   return check_launch("nv_hydro_stage_batch");

@@ -1,29 +1,69 @@
 // nv_divide.h - IEEE-exact float division by a shared divisor, through a double-precision reciprocal.
 //
-// For floats a, b (b normal, positive here) the exact quotient x = a/b is never within 2^-49 (relative) of a
-// float rounding midpoint: with a = A 2^p, b = B 2^q (24-bit A, B) and a midpoint m = M 2^s (M odd, 25 bits),
-// a - b m is a nonzero multiple of 2^min(p, q+s) (the odd part of B M exceeds 2^24 > A), which bounds
-// |x - m| / x below by min(2^-24, 2^-49). A double r with |r b - 1| <= 2^-52 gives q = RN_double(a r) with
-// relative error < 2^-51.9, so no midpoint separates q from x and (float)q == RN_float(a / b): exactly the
-// IEEE division of the reference and of the CPU oracle, but the reciprocal is shared by all numerators.
-// r comes from the hardware float reciprocal (<= 1 ulp, 2^-23) and two fma Newton steps in double
-// (error 2^-46, then below 2^-53 + 2^-92). Valid for normal positive divisors and quotients within the float
-// range, which is what the hydro stage divides by (density floor > 0, dx > 0, SR - SL > 0).
+// Normal quotients. For floats a, b (b normal, positive here) whose quotient x = a/b is a normal float or
+// rounds to one, x is never within 2^-49 (relative) of a float rounding midpoint unless it is one: with
+// a = A 2^p, b = B 2^q (24-bit A, B) and a midpoint m = M 2^s (M odd, 25 bits), a - b m is a nonzero multiple of
+// 2^min(p, q+s) (the odd part of B M exceeds 2^24 > A), which bounds |x - m| / x below by min(2^-24, 2^-49).
+// A double r with |r b - 1| <= 2^-52 gives q = RN_double(a r) with relative error < 2^-51.9, so no midpoint
+// separates q from x and (float)q == RN_float(a / b): exactly the IEEE division of the reference and of the CPU
+// oracle, but the reciprocal is shared by all numerators.
+//
+// Subnormal quotients. Below 2^-126 the midpoints are odd multiples of 2^-150 with fewer than 25 bits, and x can
+// be one exactly (3 2^-149 / 6 = 2^-150, whose IEEE quotient is the even neighbour 0): q, a hair off the tie,
+// may round to the other neighbour. Every other x stays at least 2^-25 (relative) from the midpoints (the same
+// divisibility argument), so only ties are at stake, and a tie resolved the wrong way always lands on a
+// subnormal float (a tie at 2^-150 that rounds down gives 0, which is the IEEE answer). div_r therefore redoes
+// the division in IEEE arithmetic when the product rounds to a subnormal: the rare branch, since that needs
+// |a| < 2^-126 b.
+//
+// The reciprocal comes from the hardware float reciprocal r0 (<= 1 ulp, 2^-23) and one cubic correction in
+// double: e = 1 - d r0 is exact (d r0 has at most 48 significant bits and |e| <= 2^-22), 1/d = r0 (1 + e + e^2 +
+// ...), so r = r0 (1 + (e + e^2)) is off by e^3 (2^-66) plus the roundings of e + e^2 (2^-53 |e|) and of the last
+// fma (2^-53): |r d - 1| < 2^-52. Valid for normal divisors (the argument is symmetric in sign), which is
+// what the kernels divide by (density floor > 0, dx > 0, SR - SL > 0, constants).
 #pragma once
 #include <hip/hip_runtime.h>
 
 namespace nv {
 
+// a float divisor b with its shared reciprocal r (recip_d(b), or the correctly rounded 1.0 / (double)b)
+struct Recip {
+  double r;
+  float b;
+};
+
 __device__ __forceinline__ double recip_d(float b) {
   const double d = (double)b;
-  double r = (double)__builtin_amdgcn_rcpf(b);
-  double e = fma(-d, r, 1.0);
-  r = fma(r, e, r);
-  e = fma(-d, r, 1.0);
-  return fma(r, e, r);
+  const double r = (double)__builtin_amdgcn_rcpf(b);
+  const double e = fma(-d, r, 1.0);
+  return fma(r, fma(e, e, e), r);
 }
 
-// RN_float(a / b) given r = recip_d(b)
-__device__ __forceinline__ float div_r(float a, double r) { return (float)((double)a * r); }
+__device__ __forceinline__ Recip recip(float b) { return Recip{recip_d(b), b}; }
+
+constexpr int SUBNORMAL = 0x0090;   // __builtin_isfpclass mask: negative or positive subnormal
+
+// RN_float(a / R.b)
+__device__ __forceinline__ float div_r(float a, const Recip& R) {
+  const float q = (float)((double)a * R.r);
+  if (__builtin_expect(__builtin_isfpclass(q, SUBNORMAL), 0)) return a / R.b;
+  return q;
+}
+
+// q[i] = RN_float(a[i] / R.b) for N numerators of one divisor, with one branch for the group: if any product
+// rounds to a subnormal, all N are redone in IEEE arithmetic
+template <int N>
+__device__ __forceinline__ void div_rn(const float (&a)[N], const Recip& R, float (&q)[N]) {
+  bool sub = false;
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    q[i] = (float)((double)a[i] * R.r);
+    sub = sub || __builtin_isfpclass(q[i], SUBNORMAL);
+  }
+  if (__builtin_expect(sub, 0)) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) q[i] = a[i] / R.b;
+  }
+}
 
 }  // namespace nv

@@ -11,7 +11,7 @@ namespace nv {
 // identical correctly rounded quotient, nv_divide.h), or p * (1/dx) when dx is a power of two, which rounds the
 // same real number and so gives the same float. A compile-time choice, so only one form is issued.
 struct ScaleDiv {
-  double rdx;
+  Recip rdx;
   __device__ __forceinline__ float operator()(float p) const { return div_r(p, rdx); }
 };
 struct ScalePow2 {

@@ -133,7 +133,7 @@ extern "C" int nv_drive(float* rho, float* mx, float* my, float* mz, int nx, int
     hipLaunchKernelGGL(drive_kernel<ScalePow2>, dim3(grid_of(n)), dim3(256), 0, as_stream(stream), A, sc);
   } else {
     ScaleDiv sc;
-    sc.rdx = 1.0 / (double)dxs;
+    sc.rdx = Recip{1.0 / (double)dxs, dxs};
     hipLaunchKernelGGL(drive_kernel<ScaleDiv>, dim3(grid_of(n)), dim3(256), 0, as_stream(stream), A, sc);
   }
   return check_launch("nv_drive");

@@ -72,7 +72,7 @@ __device__ __forceinline__ bool ray_of_thread(long long t, int nu, int nphi, int
 struct RayJob {
   const float* rho;
   float* rfront;
-  double rdx;        // 1/dx, correctly rounded (the shared reciprocal of the general cell scaling)
+  Recip rdx;         // dx and 1/dx correctly rounded (the shared reciprocal of the general cell scaling)
   int nx, ny, nz, nsteps, nu, nphi, wide;
   float sx, sy, sz, Q, dr, dx, rcap, bx, by, bz;
   float inv;         // 1/dx when dx is a power of two, else 0
@@ -188,8 +188,8 @@ __device__ __forceinline__ int wrap_phi(int ip, int nphi) {
 
 // 1 / (2 pi) as the shared double reciprocal of the float 2*PI_F (correctly rounded, so within the 2^-52 that
 // div_r needs): div_r(phi, R2PI) is phi / (2.0f * PI_F) bit for bit
-constexpr double R2PI = 1.0 / (double)(2.0f * PI_F);
-constexpr double R6 = 1.0 / 6.0;
+constexpr Recip R2PI = {1.0 / (double)(2.0f * PI_F), 2.0f * PI_F};
+constexpr Recip R6 = {1.0 / 6.0, 6.0f};
 
 // row[pa] and row[pb] for pb = pa + 1 or the wrap pb = 0: one 8-byte load of (q, q+1), q = min(pa, nphi-2), when
 // nphi >= 2 (row[0] loaded on its own only at the wrap)
@@ -229,10 +229,12 @@ __device__ __forceinline__ float front_at(const float* __restrict__ rfront, int 
 // identical IEEE quotients, nv_divide.h), IEEE division otherwise (a raw density of 0 or below the float range)
 __device__ __forceinline__ void div3(float a, float b, float c, float d, float* qa, float* qb, float* qc) {
   if (d >= 1.17549435e-38f && d <= 3.40282347e+38f) {
-    const double r = recip_d(d);
-    *qa = div_r(a, r);
-    *qb = div_r(b, r);
-    *qc = div_r(c, r);
+    const float m[3] = {a, b, c};
+    float v[3];
+    div_rn<3>(m, recip(d), v);
+    *qa = v[0];
+    *qb = v[1];
+    *qc = v[2];
   } else {
     *qa = a / d;
     *qb = b / d;
@@ -242,7 +244,7 @@ __device__ __forceinline__ void div3(float a, float b, float c, float d, float* 
 
 // apply_front (nebula3d.py:281-307)
 __global__ __launch_bounds__(256) void apply_front_kernel(const float* __restrict__ rfront, int nu, int nphi,
-                                                          float sx, float sy, float sz, float dx, double rwidth,
+                                                          float sx, float sy, float sz, float dx, Recip rwidth,
                                                           float* xi, float* dfront, int nx, int ny, int nz) {
   const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= (long long)nx * ny * nz) return;
@@ -299,7 +301,7 @@ struct IonArgs {
   int nu, nphi, nstars, nx, ny, nz, ci;
   int tiles_k, tiles_j, tiles;     // tile grid: z tiles, y tiles, all tiles (x chunks of ci planes slowest)
   float dx, cn2, ci2;
-  double rwidth;                   // 1 / width, correctly rounded (the shared reciprocal of the ramp width)
+  Recip rwidth;                    // the ramp width and 1 / width, correctly rounded (its shared reciprocal)
 };
 
 // One 64 z x 8 y tile of the fused ionize pass, marching over x chunk bi.
@@ -439,7 +441,7 @@ __global__ __launch_bounds__(256) void cast_fuv_kernel(const float* __restrict__
 
 // trilinear att at (r/binr - 1/2, u, phi) (nebula3d.py:362-387); rbinr = 1/binr as the shared reciprocal
 __device__ __forceinline__ float fuv_at(const float* __restrict__ att, int nr, int nu, int nphi, float r, float u,
-                                        float phi, double rbinr) {
+                                        float phi, const Recip& rbinr) {
   const float fr = div_r(r, rbinr) - 0.5f;
   const float fu = (u + 1.0f) * 0.5f * (float)nu - 0.5f;
   const float fp = div_r(phi, R2PI) * (float)nphi - 0.5f;
@@ -466,7 +468,7 @@ __device__ __forceinline__ float fuv_at(const float* __restrict__ att, int nr, i
 // apply_fuv (nebula3d.py:355-388)
 __global__ __launch_bounds__(256) void apply_fuv_kernel(const float* __restrict__ att, int nr, int nu, int nphi,
                                                         float sx, float sy, float sz, float lum, float dx,
-                                                        double rbinr, float* g0, int nx, int ny, int nz) {
+                                                        Recip rbinr, float* g0, int nx, int ny, int nz) {
   const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= (long long)nx * ny * nz) return;
   const int k = (int)(q % nz);
@@ -483,7 +485,7 @@ __global__ __launch_bounds__(256) void apply_fuv_kernel(const float* __restrict_
 __global__ __launch_bounds__(256) void fuv_apply_kernel(const float* __restrict__ atts, int nr, int nu, int nphi,
                                                         const float* __restrict__ star_xyz,
                                                         const float* __restrict__ lums, int nstars, float dx,
-                                                        double rbinr, uint16_t* g0_h, float* g0_f, int nx, int ny,
+                                                        Recip rbinr, uint16_t* g0_h, float* g0_f, int nx, int ny,
                                                         int nz) {
   const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= (long long)nx * ny * nz) return;
@@ -515,7 +517,7 @@ template <class F>
 static void dispatch_sampler(int nz, long long cells, float dx, F&& f) {
   const float inv = pow2_reciprocal(dx);
   ScaleDiv sd;
-  sd.rdx = 1.0 / (double)dx;   // correctly rounded: |rdx dx - 1| <= 2^-53, inside the bound div_r needs
+  sd.rdx = Recip{1.0 / (double)dx, dx};   // correctly rounded: |r dx - 1| <= 2^-53, inside the bound div_r needs
   ScalePow2 sp;
   sp.inv = inv;
   auto by_scale = [&](auto pair, auto off) {
@@ -534,7 +536,7 @@ static void fill_ray_job(const float* rho, int nx, int ny, int nz, float sx, flo
                          int nsteps, float dx, float rcap, float bx, float by, float bz, float* rfront, int nu,
                          int nphi, bool wide, RayJob& J) {
   J.rho = rho; J.rfront = rfront;
-  J.rdx = 1.0 / (double)dx;
+  J.rdx = Recip{1.0 / (double)dx, dx};
   J.nx = nx; J.ny = ny; J.nz = nz; J.nsteps = nsteps; J.nu = nu; J.nphi = nphi; J.wide = wide ? 1 : 0;
   J.sx = sx; J.sy = sy; J.sz = sz; J.Q = Q; J.dr = dr; J.dx = dx; J.rcap = rcap; J.bx = bx; J.by = by; J.bz = bz;
   J.inv = pow2_reciprocal(dx);
@@ -583,7 +585,7 @@ extern "C" int nv_apply_front(const float* rfront, int nu, int nphi, float sx, f
   NV_REQUIRE(width >= 1.17549435e-38f && width <= 3.40282347e+38f, "nv_apply_front: width must be a positive normal float");
   const long long n = (long long)nx * ny * nz;
   hipLaunchKernelGGL(apply_front_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, as_stream(stream), rfront, nu, nphi,
-                     sx, sy, sz, dx, 1.0 / (double)width, xi, dfront, nx, ny, nz);
+                     sx, sy, sz, dx, Recip{1.0 / (double)width, width}, xi, dfront, nx, ny, nz);
   return check_launch("nv_apply_front");
 }
 
@@ -628,7 +630,7 @@ static int fill_ionize(const char* name, const float* rfronts, int nu, int nphi,
   A.f0 = f0; A.f1 = f1; A.f2 = f2; A.f3 = f3; A.smax = smax_out;
   A.nu = nu; A.nphi = nphi; A.nstars = nstars; A.nx = nx; A.ny = ny; A.nz = nz;
   A.dx = dx; A.cn2 = cn2; A.ci2 = ci2;
-  A.rwidth = 1.0 / (double)width;
+  A.rwidth = Recip{1.0 / (double)width, width};
   A.ci = (int)std::min<long long>(64, std::max<long long>(4, (long long)nx * jk_total / 2048));
   A.tiles_k = (nz + IK - 1) / IK;
   A.tiles_j = (ny + IJ - 1) / IJ;
@@ -696,7 +698,7 @@ extern "C" int nv_apply_fuv(const float* att, int nr, int nu, int nphi, float sx
   NV_REQUIRE(binr >= 1.17549435e-38f && binr <= 3.40282347e+38f, "nv_apply_fuv: binr must be a positive normal float");
   const long long n = (long long)nx * ny * nz;
   hipLaunchKernelGGL(apply_fuv_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, as_stream(stream), att, nr, nu, nphi,
-                     sx, sy, sz, lum, dx, 1.0 / (double)binr, g0, nx, ny, nz);
+                     sx, sy, sz, lum, dx, Recip{1.0 / (double)binr, binr}, g0, nx, ny, nz);
   return check_launch("nv_apply_fuv");
 }
 
@@ -710,7 +712,7 @@ extern "C" int nv_fuv_apply(const float* atts, int nr, int nu, int nphi, const f
   NV_REQUIRE(binr >= 1.17549435e-38f && binr <= 3.40282347e+38f, "nv_fuv_apply: binr must be a positive normal float");
   const long long n = (long long)nx * ny * nz;
   hipLaunchKernelGGL(fuv_apply_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, as_stream(stream), atts, nr, nu, nphi,
-                     star_xyz, lums, nstars, dx, 1.0 / (double)binr, g0_h, g0_f, nx, ny, nz);
+                     star_xyz, lums, nstars, dx, Recip{1.0 / (double)binr, binr}, g0_h, g0_f, nx, ny, nz);
   return check_launch("nv_fuv_apply");
 }
 

@@ -1,6 +1,8 @@
-"""The shared-reciprocal division of nv_divide.h equals IEEE float division (the hydro kernels rely on it to
+"""The shared-reciprocal division of nv_divide.h equals IEEE float division (the kernels rely on it to
 reproduce the reference's a/b bit for bit). Built and run on the CPU with the seed reciprocal perturbed by up
-to 2 ulp, which covers the hardware reciprocal's 1-ulp error."""
+to 2 ulp, which covers the hardware reciprocal's 1-ulp error. The cases include quotients within t 2^-49 of a
+float midpoint (which a reciprocal good to 2^-46 gets wrong) and exact ties of the subnormal grid (which the
+product alone gets wrong without the IEEE fallback)."""
 import os
 import subprocess
 

@@ -110,7 +110,7 @@ def test_hydro_stage_bitwise(env, okern32, shape, solver, lim, mode):
 
 
 @pytest.mark.parametrize("shape", [(12, 9, 14), (33, 40, 96), (9, 35, 160), (17, 24, 64)])
-@pytest.mark.parametrize("tile", [1, 2])
+@pytest.mark.parametrize("tile", [1, 2, 3])
 @pytest.mark.parametrize("precision", [0, 2])
 def test_hydro_stage_tile_shapes(env, okern32, shape, tile, precision):
     """nv_hydro_stage_ex with each workgroup tile shape: bitwise the oracle (FP32 exact), bitwise the automatic
@@ -152,6 +152,58 @@ def test_hydro_stage_tile_shapes(env, okern32, shape, tile, precision):
                             0.002, float(dx), 0.5, 0.5, 2e-3, (1 / 12) ** 2, 1.0, 6.0, 1, 0)
         for c in range(4):
             np.testing.assert_array_equal(outs[tile][c], ref_out[c], err_msg=f"field {c}")
+
+
+def tie_state(shape, seed):
+    """momenta whose quotient by the density is an exact tie of the subnormal grid: rho an even integer B,
+    m = +-B M 2^-150 with M odd, so m / rho = M 2^-150 lies halfway between two subnormal floats (IEEE rounds to
+    the even one); a fifth of the cells hold ordinary momenta"""
+    rng = np.random.default_rng(seed)
+    B = (rng.integers(2, 101, shape) * 2).astype(np.int64)
+    U = [B.astype(F32)]
+    for _ in range(3):
+        M = rng.integers(0, 1024, shape) * 2 + 1
+        m = np.ldexp((B * M).astype(np.float64), -150).astype(F32)
+        assert np.array_equal(m.astype(np.float64), np.ldexp((B * M).astype(np.float64), -150))
+        m = np.where(rng.uniform(size=shape) < 0.5, -m, m).astype(F32)
+        big = rng.uniform(size=shape) < 0.2
+        m[big] = (U[0][big] * rng.normal(0, 0.5, int(big.sum()))).astype(F32)
+        U.append(m)
+    xi = rng.uniform(0, 1, shape).astype(F32)
+    return U, xi
+
+
+@pytest.mark.parametrize("tile", [1, 2, 3])
+@pytest.mark.parametrize("solver", [1, 0])
+def test_hydro_stage_subnormal_ties(env, okern32, tile, solver):
+    """quotients on exact ties of the subnormal grid (decoded velocities, and the fluxes and updated velocities
+    they feed) come out as the oracle's IEEE divisions, bit for bit"""
+    say(f"[hydro ties] tile={tile} solver={solver}")
+    torch, dev, L = env
+    shape = (10, 20, 64)
+    dx = 1.0 / 64
+    U, xi = tie_state(shape, 21)
+    U0n, _ = tie_state(shape, 22)
+    q = (U[1] / U[0]).astype(F32)          # numpy's IEEE quotient: the ties went to even neighbours
+    assert (np.abs(q[q != 0]) < np.finfo(F32).tiny).mean() > 0.5
+    gU, gU0, gxi = [T(env, u) for u in U], [T(env, u) for u in U0n], T(env, xi)
+    out = [torch.zeros(shape, dtype=torch.float32, device=dev) for _ in range(4)]
+    smax = torch.zeros(1, dtype=torch.float32, device=dev)
+    d = L.NvStageDesc()
+    for c in range(4):
+        d.u[c], d.u0[c], d.out[c] = gU[c].data_ptr(), gU0[c].data_ptr(), out[c].data_ptr()
+    d.xi, d.stars, d.jets, d.smax_out, d.dt_dev = gxi.data_ptr(), None, None, smax.data_ptr(), None
+    d.nx, d.ny, d.nz = shape
+    d.nstars, d.wind, d.jet_cos, d.jet_r, d.pulse, d.dt, d.dx = 0, 0.0, 0.9, 0.04, 1.0, 1e-4, dx
+    d.a, d.b, d.floor_, d.cn2, d.ci2, d.vmax = 0.5, 0.5, 2e-3, (1 / 12) ** 2, 1.0, 6.0
+    L.call("nv_hydro_stage_ex", ctypes.byref(d), 0, solver, 0, tile, stream(env))
+    torch.cuda.synchronize()
+    ref = [np.zeros(shape, F32) for _ in range(4)]
+    okern32.hydro_stage(U, xi, U0n, ref, shape, np.zeros((1, 4), F32), 0, 0.0, np.zeros((1, 4), F32), 0.9, 0.04, 1.0,
+                        1e-4, dx, 0.5, 0.5, 2e-3, (1 / 12) ** 2, 1.0, 6.0, solver, 0)
+    for c in range(4):
+        np.testing.assert_array_equal(N(out[c]), ref[c], err_msg=f"field {c}")
+    assert float(smax.item()) == okern32.max_speed(*ref, xi, (1 / 12) ** 2, 1.0)
 
 
 def test_hydro_stage_no_stars(env, okern32):
