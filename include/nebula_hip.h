@@ -34,7 +34,7 @@ extern "C" {
 #define NV_PREC_F32_FAST 1 /* nv_hydro_stage_fast */
 #define NV_PREC_F16V 2     /* nv_hydro_stage_h (float32 rho, binary16 velocity) */
 
-#define NV_TILE_AUTO 0     /* the library's choice from the grid shape, the precision and the stage */
+#define NV_TILE_AUTO 0     /* the library's choice from the z extent */
 #define NV_TILE_64x8 1     /* 64 z x 8 y cells per workgroup (one wavefront per y row) */
 #define NV_TILE_32x16 2    /* 32 z x 16 y cells per workgroup (one wavefront per two rows) */
 #define NV_TILE_64x16 3    /* 64 z x 16 y cells per workgroup (1024 threads, one workgroup per CU) */

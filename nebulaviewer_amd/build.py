@@ -56,5 +56,18 @@ def build(verbose=False):
     return LIB
 
 
+def build_shared(src, out, verbose=False):
+    """Compiles one more HIP source into its own shared library with the same flags (the test helpers)."""
+    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    tmp = out + ".tmp"
+    p = _run([hipcc] + COMPILE + ["-shared", "-o", tmp, src], verbose)
+    text, _ = p.communicate()
+    if p.returncode != 0:
+        sys.stderr.write(text)
+        raise RuntimeError(f"hipcc failed on {src}")
+    os.replace(tmp, out)
+    return out
+
+
 if __name__ == "__main__":
     print(build(verbose=True))

@@ -72,11 +72,16 @@ struct HydroArgs {
 // ------------------------------------------------------------------------------------------ arithmetic
 struct Exact {
   static constexpr bool FAST = false;
-  using Rcp = Recip;
-  static __device__ __forceinline__ Rcp rcp(float b) { return recip(b); }
-  static __device__ __forceinline__ float div(float a, const Rcp& r) { return div_r(a, r); }
+  using Rcp = RecipF;
+  static __device__ __forceinline__ Rcp rcp(float b) { return recip_f(b); }
+  static __device__ __forceinline__ float div(float a, const Rcp& r) {
+    const float x[1] = {a};
+    float q[1];
+    div_fn<1>(x, r, q);
+    return q[0];
+  }
   template <int N>
-  static __device__ __forceinline__ void divn(const float (&a)[N], const Rcp& r, float (&q)[N]) { div_rn<N>(a, r, q); }
+  static __device__ __forceinline__ void divn(const float (&a)[N], const Rcp& r, float (&q)[N]) { div_fn<N>(a, r, q); }
   static __device__ __forceinline__ float mad(float a, float b, float c) { return a * b + c; }
   static __device__ __forceinline__ float sqrt(float x) { return sqrtf(x); }
 };
@@ -361,7 +366,9 @@ __device__ __forceinline__ void stage_tile(const HydroArgs& A, int tile) {
     sq[cq] = qw[0];
     sc[cc] = make_float2(c2w[0], cw[0]);
     const long long pbase = (long long)i * plane;
-    for (int h = tid; h < NHALO; h += NT) {
+    // halo cells from the last wavefronts up (the first ones take the c^2 halo here and the tile-boundary edges
+    // and faces later), so that no wavefront carries two of these jobs between two barriers
+    for (int h = NT - 1 - tid; h < NHALO; h += NT) {
       int jj, kk;
       if (h < 2 * PK) {
         jj = h / PK;
@@ -754,13 +761,12 @@ static HydroKernel pick_mode(int tile, Mode mode, int solver, int lim, bool sm, 
 static int tile_tk(int tile) { return tile == TILE_32x16 ? T32::TK : tile == TILE_64x16 ? T6416::TK : T64::TK; }
 static int tile_tj(int tile) { return tile == TILE_32x16 ? T32::TJ : tile == TILE_64x16 ? T6416::TJ : T64::TJ; }
 
-// The shape when the caller leaves it to the library. By idle lanes: 64 x 8 when nz is a multiple of 64, 32 x 16
-// when it is a multiple of 32 only, otherwise the one with fewer idle lanes. The exact second stage, which also
-// reads U^n and so moves the most bytes, takes 64 x 16 on z extents that are multiples of 64 (half the halo
-// re-reads; measured at 512^3: 2.65 against 2.88 ms); the instruction-bound launches keep the two resident
-// 512-thread workgroups per CU of 64 x 8.
-static int auto_tile(int ny, int nz, Mode mode, bool stage1) {
-  if (nz % 64 == 0) return (mode == Mode::F32_EXACT && !stage1 && ny >= 16) ? TILE_64x16 : TILE_64x8;
+// The shape when the caller leaves it to the library, by idle lanes: 64 x 8 when nz is a multiple of 64, 32 x 16
+// when it is a multiple of 32 only, otherwise the one with fewer idle lanes. 64 x 16 (one 1024-thread workgroup per
+// CU) is not chosen automatically: at 512^3 it is slower in every form (2.32 / 2.70 ms against 2.21 / 2.59 for the
+// exact stages).
+static int auto_tile(int nz) {
+  if (nz % 64 == 0) return TILE_64x8;
   const int idle64 = ((nz + 63) / 64) * 64 - nz, idle32 = ((nz + 31) / 32) * 32 - nz;
   return idle32 < idle64 ? TILE_32x16 : TILE_64x8;
 }
@@ -817,10 +823,7 @@ static bool is_stage1(const HydroArgs& A) { return A.u0_is_u && A.a == 0.0f && A
 static int launch_desc(Mode mode, const char* name, const NvStageDesc& d, int solver, int lim, int tile,
                        void* stream) {
   NV_REQUIRE(tile >= TILE_AUTO && tile <= TILE_64x16, "%s: bad tile shape %d", name, tile);
-  if (tile == TILE_AUTO) {
-    const bool u0u = d.u0[0] == d.u[0] && d.u0[1] == d.u[1] && d.u0[2] == d.u[2] && d.u0[3] == d.u[3];
-    tile = auto_tile(d.ny, d.nz, mode, u0u && d.a == 0.0f && d.b == 1.0f);
-  }
+  if (tile == TILE_AUTO) tile = auto_tile(d.nz);
   HydroArgs A;
   const int rc = fill_stage(name, d, solver, lim, tile, tiles_jk(tile, d.ny, d.nz), A);
   if (rc != NV_OK) return rc;

@@ -1,20 +1,22 @@
-// nv_divide.h - IEEE-exact float division by a shared divisor, through a double-precision reciprocal.
+// nv_divide.h - IEEE-exact float division by a shared divisor, two ways: through a double-precision reciprocal
+// (Recip, div_r, div_rn: any normal divisor, the ray and render kernels) and float-only (RecipF, div_fn: the
+// instruction-bound hydro stage, below).
 //
-// Normal quotients. For floats a, b (b normal, positive here) whose quotient x = a/b is a normal float or
-// rounds to one, x is never within 2^-49 (relative) of a float rounding midpoint unless it is one: with
-// a = A 2^p, b = B 2^q (24-bit A, B) and a midpoint m = M 2^s (M odd, 25 bits), a - b m is a nonzero multiple of
-// 2^min(p, q+s) (the odd part of B M exceeds 2^24 > A), which bounds |x - m| / x below by min(2^-24, 2^-49).
-// A double r with |r b - 1| <= 2^-52 gives q = RN_double(a r) with relative error < 2^-51.9, so no midpoint
-// separates q from x and (float)q == RN_float(a / b): exactly the IEEE division of the reference and of the CPU
-// oracle, but the reciprocal is shared by all numerators.
+// Double reciprocal, normal quotients. For floats a, b (b normal, positive here) whose quotient x = a/b is a normal
+// float or rounds to one, x is never within 2^-49 (relative) of a float rounding midpoint unless it is one: with a =
+// A 2^p, b = B 2^q (24-bit A, B) and a midpoint m = M 2^s (M odd, 25 bits), a - b m is a nonzero multiple of 2^min(p,
+// q+s) (the odd part of B M exceeds 2^24 > A), which bounds |x - m| / x below by min(2^-24, 2^-49). A double r with
+// |r b - 1| <= 2^-52 gives q = RN_double(a r) with relative error < 2^-51.9, so no midpoint separates q from x and
+// (float)q == RN_float(a / b): exactly the IEEE division of the reference and of the CPU oracle, but the reciprocal
+// is shared by all numerators.
 //
-// Subnormal quotients. Below 2^-126 the midpoints are odd multiples of 2^-150 with fewer than 25 bits, and x can
-// be one exactly (3 2^-149 / 6 = 2^-150, whose IEEE quotient is the even neighbour 0): q, a hair off the tie,
-// may round to the other neighbour. Every other x stays at least 2^-25 (relative) from the midpoints (the same
-// divisibility argument), so only ties are at stake, and a tie resolved the wrong way always lands on a
-// subnormal float (a tie at 2^-150 that rounds down gives 0, which is the IEEE answer). div_r therefore redoes
-// the division in IEEE arithmetic when the product rounds to a subnormal: the rare branch, since that needs
-// |a| < 2^-126 b.
+// Double reciprocal, subnormal quotients. Below 2^-126 the midpoints are odd multiples of 2^-150 with fewer than 25
+// bits, and x can be one exactly (3 2^-149 / 6 = 2^-150, whose IEEE quotient is the even neighbour 0): q, a hair off
+// the tie, may round to the other neighbour. Every other x stays at least 2^-25 (relative) from the midpoints (the
+// same divisibility argument), so only ties are at stake, and a tie resolved the wrong way always lands on a
+// subnormal float (a tie at 2^-150 that rounds down gives 0, which is the IEEE answer). div_r therefore redoes the
+// division in IEEE arithmetic when the product rounds to a subnormal: the rare branch, since that needs |a| < 2^-126
+// b.
 //
 // The reciprocal comes from the hardware float reciprocal r0 (<= 1 ulp, 2^-23) and one cubic correction in
 // double: e = 1 - d r0 is exact (d r0 has at most 48 significant bits and |e| <= 2^-22), 1/d = r0 (1 + e + e^2 +
@@ -63,6 +65,59 @@ __device__ __forceinline__ void div_rn(const float (&a)[N], const Recip& R, floa
   if (__builtin_expect(sub, 0)) {
 #pragma unroll
     for (int i = 0; i < N; ++i) q[i] = a[i] / R.b;
+  }
+}
+
+// ------------------------------------------------------------------------------------------------------------
+// Float-only exact division (Markstein's scheme), for the instruction-bound stage.
+//
+// y = RN(1/b) from the hardware reciprocal y0 and one Newton step fma(fma(-b, y0, 1), y0, y0); q0 = RN(a y);
+// r = b q0 - a; q = RN(q0 - r y). Two exhaustive runs on gfx950 establish it: y equals RN(1/b) for every
+// normal b below 2^126, and q equals the IEEE quotient for all 2^46 pairs of significands a, b in [1, 2). The
+// sequence is a scaled copy of that case whenever nothing leaves the normal range, which the guard below
+// ensures: b in [2^-30, 2^30] (y normal) and |q0| in [2^-64, 2^64) (q0 and q normal, no overflow, and
+// |a| >= 2^-95, so that a rounded residual, which needs 25 bits and so is at least 2^(e_a - 23), is normal;
+// a residual that fits 24 bits is exact at any scale). Outside the guard the group is redone in IEEE
+// arithmetic; zero numerators stay on the fast path (r = +0 and q keeps the sign of a / b).
+struct RecipF {
+  float y, b;
+  bool ok;   // b in [2^-30, 2^30]
+};
+
+__device__ __forceinline__ RecipF recip_f(float b) {
+  const float y0 = __builtin_amdgcn_rcpf(b);
+  return RecipF{fmaf(fmaf(-b, y0, 1.0f), y0, y0), b, b >= 0x1p-30f && b <= 0x1p30f};
+}
+
+// q[i] = RN_float(a[i] / R.b) for N numerators of one divisor; one guard for the group
+template <int N>
+__device__ __forceinline__ void div_fn(const float (&a)[N], const RecipF& R, float (&q)[N]) {
+  float q0[N];
+  float lo = __builtin_inff(), hi = 0.0f;
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    q0[i] = a[i] * R.y;
+    const float r = fmaf(R.b, q0[i], -a[i]);
+    q[i] = fmaf(-r, R.y, q0[i]);
+    lo = fminf(lo, fabsf(q0[i]));
+    hi = fmaxf(hi, fabsf(q0[i]));
+  }
+  if (__builtin_expect(!(R.ok && lo >= 0x1p-64f && hi < 0x1p64f), 0)) {
+    // zeros (common: a fluid at rest) keep the fast result; anything else outside the guard takes IEEE
+    bool need[N], any = false;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      const float m = fabsf(q0[i]);
+      need[i] = !R.ok || (a[i] != 0.0f && !(m >= 0x1p-64f && m < 0x1p64f));
+      any = any || need[i];
+    }
+    if (any) {
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        const float x = a[i] / R.b;
+        q[i] = need[i] ? x : q[i];
+      }
+    }
   }
 }
 
