@@ -66,6 +66,7 @@ struct HydroArgs {
   float wind, jet_cos, jet_r, pulse, dt, dx, a, b, floor_, cn2, ci2, vmax;
   float vmax2_lo;                           // largest float <= vmax^2: |v|^2 <= vmax2_lo means no cap
   float inv_dx;                             // 1/dx when dx is a power of two (exact), else 0
+  float jet_r2_hi;                          // a float above jet_r^2 (real): |r|^2 beyond it means r >= jet_r
   const float* dt_dev;                      // if non-null, dt is read here when the stage runs
 };
 
@@ -83,7 +84,7 @@ struct Exact {
   template <int N>
   static __device__ __forceinline__ void divn(const float (&a)[N], const Rcp& r, float (&q)[N]) { div_fn<N>(a, r, q); }
   static __device__ __forceinline__ float mad(float a, float b, float c) { return a * b + c; }
-  static __device__ __forceinline__ float sqrt(float x) { return sqrtf(x); }
+  static __device__ __forceinline__ float sqrt(float x) { return sqrt_rn(x); }
 };
 
 struct Fast {
@@ -465,9 +466,8 @@ __device__ __forceinline__ void stage_tile(const HydroArgs& A, int tile) {
       const float4 fLz = fz[ty * (TK + 1) + tx], fRz = fz[ty * (TK + 1) + tx + 1];
       float div[4];
 #pragma unroll
-      for (int c = 0; c < 4; ++c)
-        div[c] = ((0.0f + (comp(fRx, c) - comp(fLx, c))) + (comp(fRy, c) - comp(fLy, c))) +
-                 (comp(fRz, c) - comp(fLz, c));
+      for (int c = 0; c < 4; ++c)   // the reference's 0 + ... only turns a -0 sum into +0: dropped
+        div[c] = ((comp(fRx, c) - comp(fLx, c)) + (comp(fRy, c) - comp(fLy, c))) + (comp(fRz, c) - comp(fLz, c));
       const float4 Uc = Uraw[0];
       const float U[4] = {Uc.x, Uc.y, Uc.z, Uc.w};
       float dU[4];
@@ -489,7 +489,7 @@ __device__ __forceinline__ void stage_tile(const HydroArgs& A, int tile) {
           if (st.w > 0.0f) {
             const float r0 = p0 - st.x, r1 = p1 - st.y, r2v = p2 - st.z;
             const float r2 = ((r0 * r0 + r1 * r1) + r2v * r2v) + 0.0004f;
-            const float f = st.w / (r2 * sqrtf(r2));
+            const float f = Exact::div(st.w, recip_f(r2 * sqrt_rn(r2)));
             acc0 = acc0 + r0 * f;
             acc1 = acc1 + r1 * f;
             acc2 = acc2 + r2v * f;
@@ -508,15 +508,19 @@ __device__ __forceinline__ void stage_tile(const HydroArgs& A, int tile) {
         const float4 st = A.stars[s];
         if (jt.w > 0.0f && st.w > 0.0f) {
           const float r0 = p0 - st.x, r1 = p1 - st.y, r2v = p2 - st.z;
-          const float r = sqrtf((r0 * r0 + r1 * r1) + r2v * r2v);
-          if (r < A.jet_r && r > 1e-6f) {
-            const float cth = ((r0 * jt.x + r1 * jt.y) + r2v * jt.z) / r;
-            if (fabsf(cth) > A.jet_cos) {
-              const float f = (cth > 0.0f ? 1.0f : -1.0f) * jt.w * A.pulse / (r * r + 0.0004f);
-              aj0 = aj0 + jt.x * f;
-              aj1 = aj1 + jt.y * f;
-              aj2 = aj2 + jt.z * f;
-              jet_hit = true;
+          const float d2 = (r0 * r0 + r1 * r1) + r2v * r2v;
+          // sqrt(d2) >= jet_r wherever d2 > jet_r2_hi: outside the cone's ball without a square root
+          if (d2 <= A.jet_r2_hi) {
+            const float r = sqrt_rn(d2);
+            if (r < A.jet_r && r > 1e-6f) {
+              const float cth = ((r0 * jt.x + r1 * jt.y) + r2v * jt.z) / r;
+              if (fabsf(cth) > A.jet_cos) {
+                const float f = (cth > 0.0f ? 1.0f : -1.0f) * jt.w * A.pulse / (r * r + 0.0004f);
+                aj0 = aj0 + jt.x * f;
+                aj1 = aj1 + jt.y * f;
+                aj2 = aj2 + jt.z * f;
+                jet_hit = true;
+              }
             }
           }
         }
@@ -805,6 +809,13 @@ static int fill_stage(const char* name, const NvStageDesc& d, int solver, int li
     A.vmax2_lo = lo;
   }
   A.inv_dx = pow2_reciprocal(d.dx);
+  {
+    // jet_r^2 (1 + 2^-20), rounded up: exceeds the real jet_r^2 by far more than the rounding of either
+    const double j2 = (double)d.jet_r * (double)d.jet_r * (1.0 + 0x1p-20);
+    float h = (float)j2;
+    if ((double)h < j2) h = nextafterf(h, INFINITY);
+    A.jet_r2_hi = d.jet_r > 0.0f ? h : -1.0f;
+  }
   // march depth: long enough to amortise the 4-plane prologue, short enough to fill the chip
   A.ci = (int)std::min<long long>(64, std::max<long long>(4, (long long)d.nx * jk_total / 2048));
   A.tiles_k = (d.nz + tile_tk(tile) - 1) / tile_tk(tile);

@@ -75,25 +75,28 @@ __device__ __forceinline__ void div_rn(const float (&a)[N], const Recip& R, floa
 // r = b q0 - a; q = RN(q0 - r y). Two exhaustive runs on gfx950 establish it: y equals RN(1/b) for every
 // normal b below 2^126, and q equals the IEEE quotient for all 2^46 pairs of significands a, b in [1, 2). The
 // sequence is a scaled copy of that case whenever nothing leaves the normal range, which the guard below
-// ensures: b in [2^-30, 2^30] (y normal) and |q0| in [2^-64, 2^64) (q0 and q normal, no overflow, and
-// |a| >= 2^-95, so that a rounded residual, which needs 25 bits and so is at least 2^(e_a - 23), is normal;
-// a residual that fits 24 bits is exact at any scale). Outside the guard the group is redone in IEEE
-// arithmetic; zero numerators stay on the fast path (r = +0 and q keeps the sign of a / b).
+// ensures: b in [2^-37, 2^91) (y normal) and |q0| in [2^-64, 2^64) (q0 and q normal, no overflow, and
+// |a| >= 2^-102). The residual r is a multiple of 2^(e_a - 47) (the lowest bits of a and of b q0), so one that
+// fits 24 bits is a multiple of 2^-149, exact at any scale, and one that must be rounded has at least 25 bits,
+// |r| >= 2^(e_a - 23) >= 2^-125, and rounds as in [1, 2). The divisor's range is tested with the quotients:
+// t = b 2^-27 lies in [2^-64, 2^64) exactly when b lies in [2^-37, 2^91), so one min/max chain over |q0| and t
+// covers both (an infinite b makes t infinite; a NaN b gives NaN quotients either way). Outside the guard the
+// group is redone in IEEE arithmetic; zero numerators stay on the fast path (r = +0 and q keeps the sign of a/b).
 struct RecipF {
   float y, b;
-  bool ok;   // b in [2^-30, 2^30]
+  float t;   // b 2^-27
 };
 
 __device__ __forceinline__ RecipF recip_f(float b) {
   const float y0 = __builtin_amdgcn_rcpf(b);
-  return RecipF{fmaf(fmaf(-b, y0, 1.0f), y0, y0), b, b >= 0x1p-30f && b <= 0x1p30f};
+  return RecipF{fmaf(fmaf(-b, y0, 1.0f), y0, y0), b, b * 0x1p-27f};
 }
 
 // q[i] = RN_float(a[i] / R.b) for N numerators of one divisor; one guard for the group
 template <int N>
 __device__ __forceinline__ void div_fn(const float (&a)[N], const RecipF& R, float (&q)[N]) {
   float q0[N];
-  float lo = __builtin_inff(), hi = 0.0f;
+  float lo = fabsf(R.t), hi = fabsf(R.t);
 #pragma unroll
   for (int i = 0; i < N; ++i) {
     q0[i] = a[i] * R.y;
@@ -102,13 +105,15 @@ __device__ __forceinline__ void div_fn(const float (&a)[N], const RecipF& R, flo
     lo = fminf(lo, fabsf(q0[i]));
     hi = fmaxf(hi, fabsf(q0[i]));
   }
-  if (__builtin_expect(!(R.ok && lo >= 0x1p-64f && hi < 0x1p64f), 0)) {
+  if (__builtin_expect(!(lo >= 0x1p-64f && hi < 0x1p64f), 0)) {
     // zeros (common: a fluid at rest) keep the fast result; anything else outside the guard takes IEEE
+    const float tb = fabsf(R.t);
+    const bool bad_b = !(tb >= 0x1p-64f && tb < 0x1p64f);
     bool need[N], any = false;
 #pragma unroll
     for (int i = 0; i < N; ++i) {
       const float m = fabsf(q0[i]);
-      need[i] = !R.ok || (a[i] != 0.0f && !(m >= 0x1p-64f && m < 0x1p64f));
+      need[i] = bad_b || (a[i] != 0.0f && !(m >= 0x1p-64f && m < 0x1p64f));
       any = any || need[i];
     }
     if (any) {
@@ -119,6 +124,18 @@ __device__ __forceinline__ void div_fn(const float (&a)[N], const RecipF& R, flo
       }
     }
   }
+}
+
+// Correctly rounded square root: the hardware square root s (<= 1 ulp) moved by one ulp where the residuals of
+// its neighbours say so (the IEEE expansion of sqrtf without its rescaling of small arguments). Exhaustively
+// verified against the IEEE square root on gfx950 for every float from 2^-96 up; smaller arguments take sqrtf.
+__device__ __forceinline__ float sqrt_rn(float x) {
+  if (__builtin_expect(x < 0x1p-96f, 0)) return sqrtf(x);
+  const float s = __builtin_amdgcn_sqrtf(x);
+  const float sm = __uint_as_float(__float_as_uint(s) - 1u), sp = __uint_as_float(__float_as_uint(s) + 1u);
+  const float rm = fmaf(-sm, s, x), rp = fmaf(-sp, s, x);
+  const float t = rm <= 0.0f ? sm : s;
+  return rp > 0.0f ? sp : t;
 }
 
 }  // namespace nv
