@@ -63,10 +63,33 @@ def cpu_baseline(setup, ion_every):
     t_hydro = time.perf_counter() - t0
     n = math.prod(setup.shape)
     per_step = t_hydro + t_ion / ion_every
+    # one host thread (the reference's CPU launch semantics), bounded sample: both RK stages on an x-slab of the
+    # same state (the stage cost per cell does not depend on the slab's thickness beyond its 2-plane rim)
+    from oracle.oracle import Kernels
+    k1 = Kernels(np.float32, threads=1)
+    nx1 = min(setup.shape[0], max(8, (1 << 22) // (setup.shape[1] * setup.shape[2])))
+    sl = (nx1,) + tuple(setup.shape[1:])
+    U = [np.ascontiguousarray(f[:nx1]) for f in (ref.rho, ref.mx, ref.my, ref.mz)]
+    xi = np.ascontiguousarray(ref.xi[:nx1])
+    U1 = [np.empty_like(u) for u in U]
+    P, C = ref.P, ref.C
+    hll = P.get("solver", "rusanov") == "hll"
+    lim = {"minmod": 0, "vanleer": 1, "mc": 2}[P.get("limiter", "minmod")] if hll else 0
+    common = dict(shape=sl, stars=ref.star_arr, nstars=len(ref.stars), wind=P["wind"], jets=ref.jet_arr,
+                  jet_cos=C["jet_cos"], jet_r=P["jet_r"], pulse=1.0, dt=ref.dt or 1e-4, dx=ref.dx, floor=P["floor"],
+                  cn2=C["cn2"], ci2=C["ci2"], vmax=6.0, solver=int(hll), lim=lim)
+    t0 = time.perf_counter()
+    k1.hydro_stage(U, xi, U, U1, a=0.0, b=1.0, **common)
+    k1.hydro_stage(U1, xi, U, U, a=0.5, b=0.5, **common)
+    t1 = time.perf_counter() - t0
+    single = math.prod(sl) / t1
     return dict(value=n / per_step, unit="cell-updates/s", cores=threads, kind="port",
                 sample=f"C++ oracle (float, -O3, OpenMP {threads} threads) on the same {setup.shape[0]}^3 grid: one "
                        f"ionization ({t_ion:.2f} s) and one CFL + two-stage hydro step ({t_hydro:.2f} s), "
-                       f"per step = hydro + ionization/{ion_every}")
+                       f"per step = hydro + ionization/{ion_every}",
+                single_thread={"value": single, "unit": "cell-updates/s", "cores": 1, "scope": "hydro (2 RK stages)",
+                               "sample": f"both RK stages of the C++ oracle on one host thread over a {sl[0]}x{sl[1]}x"
+                                         f"{sl[2]} x-slab of the same state ({t1:.2f} s)"})
 
 
 def copy_calibration(nbytes):
