@@ -92,6 +92,35 @@ def cpu_baseline(setup, ion_every):
                                          f"{sl[2]} x-slab of the same state ({t1:.2f} s)"})
 
 
+def mass_budget(sim):
+    """One more step with the FP64 conservation sums on the GPU (outside the timed region): the change of total
+    mass M against the boundary mass flux of the two RK stages, M^{n+1} - M^n = -(dt/dx) (B(U^n) + B(U1)) / 2,
+    which holds to the rounding of the cells' float updates while the density floor does not act."""
+    import torch
+    import ctypes
+    from nebulaviewer_amd import _lib
+    f = ctypes.c_float
+
+    def sums(U):
+        o = torch.zeros(2, dtype=torch.float64, device=sim.device)
+        _lib.call("nv_conservation_sums", *[t.data_ptr() for t in U], sim.xi.data_ptr(), *sim.shape,
+                  f(sim.P["floor"]), f(sim.cn2), f(sim.ci2), 1 if sim.P.get("solver", "rusanov") == "hll" else 0,
+                  o.data_ptr(), sim._stream)
+        return o.tolist()
+
+    m0, b0 = sums(sim.U)
+    rho0 = sim.rho.double()
+    bound = 12.0 * float(torch.sqrt(torch.sum((rho0.abs() * 2.0 ** -23) ** 2)))   # ~ the cells' float roundings
+    floor_on = float(sim.rho.min()) <= 1.0001 * sim.P["floor"]
+    del rho0
+    sim.step(1)
+    m1, b1 = sums([sim.stage[c] for c in range(4)])
+    m2 = sums(sim.U)[0]
+    flux = -0.5 * float(f(sim.dt).value) / sim.dx * (b0 + b1)
+    return {"dM": m2 - m0, "boundary_flux_term": flux, "residual": (m2 - m0) - flux, "rounding_bound": bound,
+            "M": m0, "floor_active": floor_on}
+
+
 def copy_calibration(nbytes):
     """A plain copy kernel of known size (reads and writes nbytes each launch), to scale the HBM byte counters."""
     import torch
@@ -227,6 +256,9 @@ def main():
                  rays=tuple(sim.rfronts.shape[1:]), nstars=len(sim.stars))
         log(f"[bench] {precision}: {r['value'] / 1e9:.2f} G cell-updates/s, {r['ms_per_step']:.3f} ms/step, "
             f"hydro {hyd_s * 1e3:.3f} ms/launch = {achieved:.0f} GB/s; kernels {r['kernels_ms']}")
+        if precision == "fp32":
+            r["mass_budget"] = mass_budget(sim)
+            log(f"[bench] {precision}: mass budget of one more step {r['mass_budget']}")
         del sim
         torch.cuda.empty_cache()
         return r
@@ -282,6 +314,7 @@ def main():
                      "algorithmic_bytes_per_launch": BYTES_PER_CELL_STEP[a.precision] / 2 * cells},
         "hydro_only_cell_updates_per_s": main_r["hydro_only_cell_updates_per_s"],
         "kernels_ms": main_r["kernels_ms"],
+        "mass_budget": main_r.get("mass_budget"),
         "variants": variants,
         "cpu_baseline": cpu,
     }

@@ -24,8 +24,9 @@
 //               through the shared double reciprocal of nv_divide.h (the identical correctly rounded
 //               quotient): results equal the CPU oracle bit for bit (up to the sign of zero, which never feeds
 //               a nonzero result here).
-//               Fast: the same expressions with explicit fused multiply-adds, the hardware reciprocal refined
-//               by one Newton step, and the hardware square root (each within about one ulp).
+//               Fast: the same expressions with explicit fused multiply-adds, the hardware reciprocal and
+//               square root (each within about one ulp), the median form of minmod, a branchless HLL and a
+//               reciprocal-square-root wind term.
 //   Storage     F32: (rho, mx, my, mz) float32, the reference layout.
 //               F16V: rho float32, (vx, vy, vz) binary16. |v| <= vmax lies in the normal fp16 range, so each
 //               velocity keeps a relative precision of 2^-11 with no scale factor; the primitive state is read
@@ -90,10 +91,7 @@ struct Exact {
 struct Fast {
   static constexpr bool FAST = true;
   using Rcp = float;
-  static __device__ __forceinline__ Rcp rcp(float b) {
-    const float r = __builtin_amdgcn_rcpf(b);
-    return fmaf(r, fmaf(-b, r, 1.0f), r);
-  }
+  static __device__ __forceinline__ Rcp rcp(float b) { return __builtin_amdgcn_rcpf(b); }   // <= 1 ulp
   static __device__ __forceinline__ float div(float a, Rcp r) { return a * r; }
   template <int N>
   static __device__ __forceinline__ void divn(const float (&a)[N], Rcp r, float (&q)[N]) {
@@ -489,7 +487,13 @@ __device__ __forceinline__ void stage_tile(const HydroArgs& A, int tile) {
           if (st.w > 0.0f) {
             const float r0 = p0 - st.x, r1 = p1 - st.y, r2v = p2 - st.z;
             const float r2 = ((r0 * r0 + r1 * r1) + r2v * r2v) + 0.0004f;
-            const float f = Exact::div(st.w, recip_f(r2 * sqrt_rn(r2)));
+            float f;
+            if constexpr (AR::FAST) {
+              const float ir = __builtin_amdgcn_rsqf(r2);   // st.w / r2^(3/2) to a few ulp
+              f = st.w * (ir * ir * ir);
+            } else {
+              f = Exact::div(st.w, recip_f(r2 * sqrt_rn(r2)));
+            }
             acc0 = acc0 + r0 * f;
             acc1 = acc1 + r1 * f;
             acc2 = acc2 + r2v * f;
