@@ -297,6 +297,8 @@ __device__ __forceinline__ void stage_tile(const HydroArgs& A, int tile) {
   __shared__ float4 fy[(TJ + 1) * TK];      // y-face fluxes: row r is the face between rows r-1 and r
   __shared__ float4 fz[TJ * (TK + 1)];      // z-face fluxes: column t is the face between columns t-1 and t
   __shared__ float red[NT / 64];
+  __shared__ int2 htab[NHALO];              // primitive-halo cell h: (LDS slot, clamped offset within a plane)
+  __shared__ int2 ctab[NCHALO];             // c^2-halo cell h: the same
 
   const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TK + tx;
   const int nx = A.nx, ny = A.ny, nz = A.nz;
@@ -316,6 +318,38 @@ __device__ __forceinline__ void stage_tile(const HydroArgs& A, int tile) {
     raw = ST::load(A.f0, A.f1, A.f2, A.f3, o);
     x = A.xi[o];
   };
+
+  // the halo cells' LDS slots and in-plane offsets depend on the tile only: tabulated once for the whole march
+  for (int h = tid; h < NHALO; h += NT) {
+    int jj, kk;
+    if (h < 2 * PK) {
+      jj = h / PK;
+      kk = h - jj * PK;
+    } else if (h < 4 * PK) {
+      const int hh = h - 2 * PK;
+      jj = TJ + 2 + hh / PK;
+      kk = hh - (hh / PK) * PK;
+    } else {
+      const int hh = h - 4 * PK;
+      jj = 2 + (hh >> 2);
+      const int c = hh & 3;
+      kk = c < 2 ? c : TK + c;
+    }
+    htab[h] = make_int2(jj * PK + kk, clampi(j0 + jj - 2, 0, ny - 1) * nz + clampi(k0 + kk - 2, 0, nz - 1));
+  }
+  for (int h = tid; h < NCHALO; h += NT) {
+    int jj, kk;
+    if (h < 2 * CK) {
+      jj = h < CK ? 0 : TJ + 1;
+      kk = h < CK ? h : h - CK;
+    } else {
+      const int hh = h - 2 * CK;
+      jj = 1 + (hh >> 1);
+      kk = (hh & 1) ? TK + 1 : 0;
+    }
+    ctab[h] = make_int2(jj * CK + kk, clampi(j0 + jj - 1, 0, ny - 1) * nz + clampi(k0 + kk - 1, 0, nz - 1));
+  }
+  __syncthreads();
 
   // register window of the own column: primitives of planes i, i+1, i+2; c^2, c of planes i, i+1; the
   // conserved state and xi of planes i, i+1, i+2; the high x edge of plane i; the flux of face i-1/2
@@ -368,38 +402,15 @@ __device__ __forceinline__ void stage_tile(const HydroArgs& A, int tile) {
     // halo cells from the last wavefronts up (the first ones take the c^2 halo here and the tile-boundary edges
     // and faces later), so that no wavefront carries two of these jobs between two barriers
     for (int h = NT - 1 - tid; h < NHALO; h += NT) {
-      int jj, kk;
-      if (h < 2 * PK) {
-        jj = h / PK;
-        kk = h - jj * PK;
-      } else if (h < 4 * PK) {
-        const int hh = h - 2 * PK;
-        jj = TJ + 2 + hh / PK;
-        kk = hh - (hh / PK) * PK;
-      } else {
-        const int hh = h - 4 * PK;
-        jj = 2 + (hh >> 2);
-        const int c = hh & 3;
-        kk = c < 2 ? c : TK + c;
-      }
-      const long long oh = pbase + (long long)clampi(j0 + jj - 2, 0, ny - 1) * nz + clampi(k0 + kk - 2, 0, nz - 1);
+      const int2 e = htab[h];
       float4 qh, Uh;
-      decode<ST, AR>(ST::load(A.f0, A.f1, A.f2, A.f3, oh), floor_, qh, Uh);
-      sq[jj * PK + kk] = qh;
+      decode<ST, AR>(ST::load(A.f0, A.f1, A.f2, A.f3, pbase + e.y), floor_, qh, Uh);
+      sq[e.x] = qh;
     }
     for (int h = tid; h < NCHALO; h += NT) {
-      int jj, kk;
-      if (h < 2 * CK) {
-        jj = h < CK ? 0 : TJ + 1;
-        kk = h < CK ? h : h - CK;
-      } else {
-        const int hh = h - 2 * CK;
-        jj = 1 + (hh >> 1);
-        kk = (hh & 1) ? TK + 1 : 0;
-      }
-      const long long oh = pbase + (long long)clampi(j0 + jj - 1, 0, ny - 1) * nz + clampi(k0 + kk - 1, 0, nz - 1);
-      const float c2 = AR::mad(A.xi[oh], dci, cn2);
-      sc[jj * CK + kk] = make_float2(c2, AR::sqrt(c2));
+      const int2 e = ctab[h];
+      const float c2 = AR::mad(A.xi[pbase + e.y], dci, cn2);
+      sc[e.x] = make_float2(c2, AR::sqrt(c2));
     }
     __syncthreads();
 
@@ -795,6 +806,8 @@ static int fill_stage(const char* name, const NvStageDesc& d, int solver, int li
   NV_REQUIRE(d.out[0] != d.u[0] && d.out[1] != d.u[1] && d.out[2] != d.u[2] && d.out[3] != d.u[3],
              "%s: output aliases the input U", name);
   NV_REQUIRE(d.floor_ > 0.0f && d.dx > 0.0f, "%s: floor and dx must be > 0", name);
+  NV_REQUIRE((long long)d.ny * d.nz < (1LL << 31), "%s: ny nz = %lld, at or above 2^31 cells per x plane", name,
+             (long long)d.ny * d.nz);
   A.f0 = d.u[0]; A.f1 = d.u[1]; A.f2 = d.u[2]; A.f3 = d.u[3]; A.xi = d.xi;
   A.g0 = d.u0[0]; A.g1 = d.u0[1]; A.g2 = d.u0[2]; A.g3 = d.u0[3];
   A.o0 = d.out[0]; A.o1 = d.out[1]; A.o2 = d.out[2]; A.o3 = d.out[3];
