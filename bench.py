@@ -12,6 +12,9 @@ two fused RK stages. value = cells * K / wall time of K steps (ionization amorti
 roofline: the dominant kernel (the fused RK stage). Algorithmic bytes per cell per RK2 step: 88 (stage 1 reads
 U + xi and writes U1: 36 B; stage 2 reads U1 + xi + U^n and writes U^n+1: 52 B), i.e. 44 B per cell per launch
 on average; achieved = 44 B x cells / (mean launch time from HIP events on the launch stream).
+ionization_roofline: the ray casts and the fused ionize pass, each against its own algorithmic bytes per cell and
+call (4 B per active star for the casts' density read; 22 B for the ionize pass: xi and dfront written, the state
+read for the fused CFL max).
 cpu_baseline: the C++ oracle (oracle/, a restatement of the reference kernels; the reference's Warp CPU device
 cannot run here) on the same 512^3 grid: one ionization and one hydro step on all host cores.
 variants: the same workload in the fast-arithmetic and binary16-velocity forms, and the uneven batch (64
@@ -92,6 +95,26 @@ def cpu_baseline(setup, ion_every):
                                          f"{sl[2]} x-slab of the same state ({t1:.2f} s)"})
 
 
+def ionization_roofline(ks, cells, nstars, precision):
+    """HBM roofline of the ionization kernels, each against its own algorithmic bytes per cell and call: the
+    brick build reads the density and writes its 16-byte bricks (20 B), the ray casts read the density once per
+    active star (4 B per star; the gathers' re-reads are not counted),
+    the fused ionize pass writes xi (4 B) and dfront (2 B) and reads the state for the fused CFL max (16 B, or
+    10 B with binary16 velocity); its unsmoothed xi planes stay in LDS."""
+    per_cell = {"ray_bricks_kernel": 20.0, "cast_rays_kernel": 4.0 * nstars,
+                "ionize_apply_kernel": 6.0 + (10.0 if precision == "fp16-vel" else 16.0)}
+    out = {"bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBS, "stars": nstars, "kernels": {}}
+    for key, name in (("ray_bricks", "ray_bricks_kernel"), ("cast_rays", "cast_rays_kernel"),
+                      ("ionize_apply", "ionize_apply_kernel")):
+        if key not in ks:
+            continue
+        ms = ks[key]["avg_ms"]
+        ach = per_cell[name] * cells / (ms / 1e3) / 1e9
+        out["kernels"][name] = {"ms": round(ms, 4), "bytes_per_cell": per_cell[name], "achieved": ach,
+                                "frac": ach / HBM_PEAK_GBS}
+    return out
+
+
 def mass_budget(sim):
     """One more step with the FP64 conservation sums on the GPU (outside the timed region): the change of total
     mass M against the boundary mass flux of the two RK stages, M^{n+1} - M^n = -(dt/dx) (B(U^n) + B(U1)) / 2,
@@ -164,7 +187,8 @@ def make_measure_batch(a, S, KernelTimer):
                  ms_per_step=wall / a.steps * 1e3, hydro_launch_ms=hyd_s * 1e3,
                  hydro_only_cell_updates_per_s=cells / (2 * hyd_s),
                  hydro_achieved_GBps=BYTES_PER_CELL_STEP["fp32"] / 2 * cells / hyd_s / 1e9,
-                 kernels_ms={k: round(v["avg_ms"], 4) for k, v in ks.items()}, graph_replay=True)
+                 kernels_ms={k: round(v["avg_ms"], 4) for k, v in ks.items()}, graph_replay=True,
+                 tile={1: "64x8", 2: "32x16", 3: "64x16"}.get(b.tile))
         log(f"[bench] batch: {r['value'] / 1e9:.2f} G cell-updates/s ({cells / 1e6:.1f} M cells, {r['stars']} stars), "
             f"{r['ms_per_step']:.3f} ms/step, hydro {hyd_s * 1e3:.3f} ms/launch; kernels {r['kernels_ms']}")
         del b
@@ -224,6 +248,7 @@ def main():
     from nebulaviewer_amd import scenarios as S
     from nebulaviewer_amd.sim import Sim
     from nebulaviewer_amd.timing import KernelTimer
+    from nebulaviewer_amd import autotune
 
     if a.config == 4:
         batch_only(a)
@@ -253,7 +278,9 @@ def main():
         r = dict(value=cells * a.steps / wall, ms_per_step=wall / a.steps * 1e3, hydro_launch_ms=hyd_s * 1e3,
                  hydro_only_cell_updates_per_s=cells / (2 * hyd_s), hydro_achieved_GBps=achieved,
                  kernels_ms={k: round(v["avg_ms"], 4) for k, v in ks.items()}, ion_every=sim.ion_every,
-                 rays=tuple(sim.rfronts.shape[1:]), nstars=len(sim.stars))
+                 rays=tuple(sim.rfronts.shape[1:]), nstars=len(sim.stars), tile=autotune.TILES.get(sim.tile_shape),
+                 ionization_roofline=ionization_roofline(ks, cells, sum(1 for st in sim.stars if sim.t >= st[4]),
+                                                         precision))
         log(f"[bench] {precision}: {r['value'] / 1e9:.2f} G cell-updates/s, {r['ms_per_step']:.3f} ms/step, "
             f"hydro {hyd_s * 1e3:.3f} ms/launch = {achieved:.0f} GB/s; kernels {r['kernels_ms']}")
         if precision == "fp32":
@@ -314,7 +341,9 @@ def main():
                      "algorithmic_bytes_per_launch": BYTES_PER_CELL_STEP[a.precision] / 2 * cells},
         "hydro_only_cell_updates_per_s": main_r["hydro_only_cell_updates_per_s"],
         "kernels_ms": main_r["kernels_ms"],
+        "ionization_roofline": main_r["ionization_roofline"],
         "mass_budget": main_r.get("mass_budget"),
+        "autotune": autotune.report(),
         "variants": variants,
         "cpu_baseline": cpu,
     }

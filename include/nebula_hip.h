@@ -192,6 +192,20 @@ int nv_cast_rays_wide(const float* rho, int nx, int ny, int nz, float sx, float 
                       int nsteps, float dx, float rcap, float bx, float by, float bz, float* rfront, int nu,
                       int nphi, void* stream);
 
+/* The brick array of a density field for nv_cast_rays_ex: bricks[i][j][k] = (rho(i,j,k), rho(i,j,k+1),
+ * rho(i,j+1,k), rho(i,j+1,k+1)) as 4 floats (16 bytes per cell, 16-byte aligned; the +1 neighbours clamped to
+ * the grid). A caller with a free 16 B/cell scratch block (Sim's RK stage block is free while ionize runs) builds
+ * it once per ionization. No reference counterpart: it is the sampler's data layout, not a reference step. */
+int nv_ray_bricks(const float* rho, int nx, int ny, int nz, float* bricks, void* stream);
+
+/* nv_cast_rays (wide == 0) or nv_cast_rays_wide (wide != 0), sampling the density from its brick array when
+ * bricks is non-null: two 16-byte loads per trilinear sample instead of four paired loads, the identical rfront
+ * bit for bit. bricks == null: exactly nv_cast_rays / nv_cast_rays_wide. Replaces the same launch as
+ * nv_cast_rays (nebula3d.cast_rays, nebula3d.py:247-270; sim.py:936-937). */
+int nv_cast_rays_ex(const float* rho, const float* bricks, int nx, int ny, int nz, float sx, float sy, float sz,
+                    float Q, float dr, int nsteps, float dx, float rcap, float bx, float by, float bz, float* rfront,
+                    int nu, int nphi, int wide, void* stream);
+
 /* Batched ray casts: every (grid, star) job in one launch, each job's rfront bit-identical to nv_cast_rays
  * (nv_cast_rays_wide when wide != 0) on that job alone. No reference counterpart (Sim.ionize launches per star,
  * sim.py:936-937). Plan and launch as for nv_hydro_stage_batch. */

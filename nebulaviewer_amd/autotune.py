@@ -1,0 +1,100 @@
+"""Measured choice of the RK stage's workgroup tile.
+
+The stage kernel comes in three tile shapes (NV_TILE_* in include/nebula_hip.h): 64 z x 8 y and 32 z x 16 y
+(8 wavefronts per workgroup, two workgroups per CU) and 64 z x 16 y (16 wavefronts, one workgroup per CU, half
+the halo share). The shape changes the work decomposition only, never a bit of the result. Which one is fastest
+depends on the z extent (idle lanes of the last z tile), on how many tiles the launch has to fill 256 CUs with,
+and on how the halo share trades against occupancy, so it is measured rather than guessed: each candidate runs
+the stage on the caller's own fields, timed with HIP events on the launch stream, and the fastest is kept. The
+choice is cached per problem signature for the life of the process; the library's own rule (NV_TILE_AUTO, by idle
+lanes) is what runs when tuning is switched off.
+
+    tile = tune_sim(sim)        # Sim(tile="tune") does this on its first RK stage
+    tile = tune_batch(batch)    # SimBatch(tile="tune") does this when it builds its stage plans
+    report()                    # {signature: {"tile": t, "ms": {name: ms, ...}}}
+"""
+import ctypes
+
+import torch
+
+from . import _lib
+
+TILES = {1: "64x8", 2: "32x16", 3: "64x16"}
+CANDIDATES = (1, 2, 3)
+PRECISION_CODES = {"fp32": 0, "fp32-fast": 1, "fp16-vel": 2}
+
+_cache = {}
+
+
+def report():
+    """every tuning result of this process: {signature: {"tile": name, "ms": {tile name: ms per launch}}}"""
+    return {str(k): {"tile": TILES[v["tile"]], "ms": {TILES[t]: round(ms, 4) for t, ms in v["ms"].items()}}
+            for k, v in _cache.items()}
+
+
+def _time(launch, stream_dev, reps, warm=2):
+    """mean ms per call of launch() over reps calls, after warm untimed calls (HIP events on the launch stream)"""
+    strm = torch.cuda.current_stream(stream_dev)
+    for _ in range(warm):
+        launch()
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s.record(strm)
+    for _ in range(reps):
+        launch()
+    e.record(strm)
+    e.synchronize()
+    return s.elapsed_time(e) / reps
+
+
+def _pick(key, measure, candidates):
+    if key in _cache:
+        return _cache[key]["tile"]
+    ms = {t: measure(t) for t in candidates}
+    best = min(ms, key=ms.get)
+    _cache[key] = {"tile": best, "ms": ms}
+    return best
+
+
+def tune_sim(sim, candidates=CANDIDATES, reps=5):
+    """The fastest tile for this Sim's grid, precision, solver and limiter. Times the two stage forms the step uses
+    (the stage-1 form and the general a U0 + b Un form with the fused CFL reduction), both writing only the free
+    scratch block sim.stage and a scratch maximum, so the simulation state is untouched."""
+    solver, lim = sim._solver_lim()
+    key = ("sim", tuple(sim.shape), sim.precision, solver, lim)
+    if key in _cache:
+        return _cache[key]["tile"]
+    code = PRECISION_CODES[sim.precision]
+    scratch = torch.zeros(1, dtype=torch.float32, device=sim.device)
+    U = [t.data_ptr() for t in sim.U]
+    U1 = [sim.stage[c].data_ptr() for c in range(4)]
+    d1 = sim._stage_desc(U, U, U1, 0.0, 1.0, 0.0, len(sim.stars), sim.P["wind"], 1.0, None)
+    d2 = sim._stage_desc(U, U, U1, 0.5, 0.5, 0.0, len(sim.stars), sim.P["wind"], 1.0, scratch.data_ptr())
+
+    def measure(tile):
+        strm = sim._stream
+
+        def launch():
+            for d in (d1, d2):
+                _lib.call("nv_hydro_stage_ex", ctypes.byref(d), code, solver, lim, tile, strm)
+        return _time(launch, sim.device, reps) / 2
+
+    return _pick(key, measure, candidates)
+
+
+def tune_batch(batch, pulses, candidates=CANDIDATES, reps=3):
+    """The fastest tile for a SimBatch's member grids: builds the stage-1 plan of each candidate and times its
+    launch (it writes only the members' scratch blocks)."""
+    key = ("batch", tuple(tuple(m.shape) for m in batch.members), batch.precision, batch.solver, batch.lim)
+    if key in _cache:
+        return _cache[key]["tile"]
+
+    def measure(tile):
+        batch.tile = tile
+        batch._stage_key = None
+        batch._stage_plans(pulses)
+        plan = batch._plans_stage[0]
+        return _time(lambda: _lib.call("nv_hydro_stage_batch", *plan.args, batch._stream), batch.device, reps)
+
+    best = _pick(key, measure, candidates)
+    batch._stage_key = None
+    return best

@@ -47,12 +47,15 @@ class _Plan:
 
 class SimBatch:
     def __init__(self, setups, precision="fp32", device="cuda:0", rays=None, ion_every=None, graph=False,
-                 verbose=False, tile=0):
+                 verbose=False, tile="tune"):
         if not setups:
             raise ValueError("a batch needs at least one member")
         self.device = torch.device(device)
         self.precision = precision
-        self.tile = tile          # workgroup tile shape of the stage (0: the library's choice; 1: 64x8; 2: 32x16)
+        # workgroup tile shape of the stage: "tune" (measured, nebulaviewer_amd.autotune), 0 (the library's rule),
+        # or 1 / 2 / 3 (64x8, 32x16, 64x16); no shape changes a bit of the result
+        self.tune = tile == "tune"
+        self.tile = 0 if self.tune else tile
         self.members = [Sim(setup=st, verbose=verbose, precision=precision, device=device, rays=rays,
                             ion_every=ion_every) for st in setups]
         m0 = self.members[0]
@@ -132,6 +135,10 @@ class SimBatch:
 
     # ------------------------------------------------------------------ plans
     def _stage_plans(self, pulses):
+        if self.tune:
+            from . import autotune
+            self.tune = False
+            self.tile = autotune.tune_batch(self, pulses)
         key = tuple(pulses)
         if key == self._stage_key:
             return
@@ -200,7 +207,7 @@ class SimBatch:
                 j.bx, j.by, j.bz = Lx, Ly, Lz
                 j.rfront = m.rfronts[slot].data_ptr()
                 j.nu, j.nphi = nu, nphi
-                j.wide = 0
+                j.wide = 1 if m.long_rays == "fp64" else 0
                 jobs.append(j)
             xyz = np.array([m.stars[i][:3] for i in on], dtype=np.float32).reshape(-1, 3)
             if len(on):

@@ -4,14 +4,16 @@
 // wp.atan2 (apply_front nebula3d.py:287, apply_fuv nebula3d.py:361) and wp.exp (cast_fuv nebula3d.py:345,351).
 // Device math libraries and host libm round these differently in the last place, which would make the GPU
 // path and the CPU oracle disagree bit-wise. Both sides therefore use the functions below: float in, float
-// out, evaluated in IEEE double with only +, -, *, / (no FMA: every translation unit that includes this is
-// built with -ffp-contract=off), so hipcc and gcc produce identical bits. The double evaluation makes each
-// result the correctly rounded float except within ~1e-16 relative of a rounding midpoint.
+// out, evaluated with only IEEE +, -, *, / (no FMA: every translation unit that includes this is built with
+// -ffp-contract=off), so hipcc and gcc produce identical bits. sin, cos and exp are evaluated in double, which
+// makes each result the correctly rounded float except within ~1e-16 relative of a rounding midpoint; atan2 is
+// evaluated in float (within 2 ulp).
 //
 // Methods (textbook, no library code reproduced):
 //   sin / cos : Cody-Waite reduction by pi/2 (two-part constant), Taylor series to degree 17/18 on |r|<=pi/4.
-//   atan2     : quadrant folding, atan(t) = pi/6 + atan((sqrt3 t - 1)/(t + sqrt3)) for t > tan(pi/12) folded
-//               into one division, Taylor series of atan to degree 19 on |t| <= tan(pi/12).
+//   atan2     : float only (cheaper per cell; within 2 ulp): octant folding, atan(t) = pi/6 +
+//               atan((sqrt3 t - 1)/(t + sqrt3)) for t > tan(pi/12) folded into one division, Taylor series of
+//               atan to degree 11 on |t| <= tan(pi/12).
 //   exp       : Cody-Waite reduction by ln2, Taylor series to degree 13 on |r| <= ln2/2, exact 2^k scaling.
 // Valid domain: finite arguments; sincos is used for |x| < 2^20 (ray angles lie in [0, 2pi)).
 #pragma once
@@ -28,14 +30,14 @@ namespace nvm {
 static constexpr double PIO2_HI = 0x1.921fb544p+0;            // pi/2 truncated to 33 bits (k*PIO2_HI exact)
 static constexpr double PIO2_LO = 0x1.0b4611a626331p-34;      // pi/2 - PIO2_HI
 static constexpr double TWO_OVER_PI = 0x1.45f306dc9c883p-1;
-static constexpr double PI_D = 0x1.921fb54442d18p+1;
-static constexpr double PIO2_D = 0x1.921fb54442d18p+0;
-static constexpr double PIO6_D = 0x1.0c152382d7366p-1;
-static constexpr double SQRT3_D = 0x1.bb67ae8584caap+0;
-static constexpr double TAN_PI_12 = 0x1.126145e9ecd56p-2;
 static constexpr double LN2_HI = 0x1.62e42feep-1;             // ln2 truncated to 32 bits
 static constexpr double LN2_LO = 0x1.a39ef35793c76p-33;
 static constexpr double INV_LN2 = 0x1.71547652b82fep+0;
+static constexpr float PI_F32 = 3.14159265358979323846f;
+static constexpr float PIO2_F32 = 1.57079632679489661923f;
+static constexpr float PIO6_F32 = 0.52359877559829887308f;
+static constexpr float SQRT3_F32 = 1.73205080756887729353f;
+static constexpr float TAN_PI_12_F32 = 0.26794919243112270647f;
 
 // sin(r), cos(r) for |r| <= pi/4 (Taylor, Horner in r^2)
 NV_HD inline double sin_poly(double r) {
@@ -86,50 +88,40 @@ NV_HD inline void sincosf_det(float xf, float* s, float* c) {
 NV_HD inline float sinf_det(float x) { float s, c; sincosf_det(x, &s, &c); return s; }
 NV_HD inline float cosf_det(float x) { float s, c; sincosf_det(x, &s, &c); return c; }
 
-// atan(a / b) for 0 <= a <= b, b > 0, with one division: for a/b > tan(pi/12) the identity
-// atan(t) = pi/6 + atan((sqrt3 t - 1)/(t + sqrt3)) is applied to t = a/b as (sqrt3 a - b)/(a + sqrt3 b), so the
-// series argument s satisfies |s| <= tan(pi/12) = 0.268 either way; the Taylor series of atan to degree 19
-// leaves a relative error below s^20/21 < 2e-13. The two cases select their numerator and denominator and
-// share the division and the series (one code path for every lane of a wavefront).
-NV_HD inline double atan_ratio(double a, double b) {
-  const bool big = a > TAN_PI_12 * b;
-  const double num = big ? SQRT3_D * a - b : a;
-  const double den = big ? a + SQRT3_D * b : b;
-  const double s = num / den;
-  const double s2 = s * s;
-  // sum_{k=0}^{9} (-1)^k s^(2k+1) / (2k+1), Horner from the top
-  double p = -1.0 / 19.0;
-  p = p * s2 + 1.0 / 17.0;
-  p = p * s2 - 1.0 / 15.0;
-  p = p * s2 + 1.0 / 13.0;
-  p = p * s2 - 1.0 / 11.0;
-  p = p * s2 + 1.0 / 9.0;
-  p = p * s2 - 1.0 / 7.0;
-  p = p * s2 + 1.0 / 5.0;
-  p = p * s2 - 1.0 / 3.0;
-  p = p * s2 + 1.0;
-  return (big ? PIO6_D : 0.0) + s * p;
-}
-
-// atan2 with the C99 conventions for signed zeros; finite arguments only. The octant is folded first so that
-// one atan_ratio serves every case: atan2 = atan(min/max), or pi/2 - atan(min/max) above the diagonal,
-// reflected to pi - a for x < 0 and negated for y < 0.
+// atan2 in float arithmetic only (the per-cell ionize and FUV passes evaluate it once per cell and star, where
+// the double form cost a third of the pass). The octant is folded first: t = lo/hi with lo = min(|y|, |x|),
+// hi = max(|y|, |x|), so t lies in [0, 1]; for t > tan(pi/12) the identity atan(t) = pi/6 + atan((sqrt3 t - 1) /
+// (t + sqrt3)) is applied as (sqrt3 lo - hi) / (lo + sqrt3 hi), one division either way, so the series argument
+// satisfies |s| <= tan(pi/12) = 0.268. atan(s) = s + s q with q = -s^2/3 + s^4/5 - ... - s^10/11 (Taylor to
+// degree 11: truncation below s^12/13 < 1.1e-8 relative); the correction s q is under 2.4% of s, so its own
+// rounding is damped. Then pi/2 - a above the diagonal, pi - a for x < 0, and the sign of y. At most 2 ulp from
+// the correctly rounded atan2 (tests/test_oracle_math.py sweeps it against libm); C99 signed-zero conventions.
 NV_HD inline float atan2f_det(float yf, float xf) {
-  const double y = (double)yf, x = (double)xf;
-  const double ay = fabs(y), ax = fabs(x);
-  const bool yneg = signbit(yf) != 0;
+  const float ay = fabsf(yf), ax = fabsf(xf);
+  const bool steep = ay > ax;
+  const float lo = steep ? ax : ay, hi = steep ? ay : ax;
   const bool xneg = signbit(xf) != 0;
-  double a;
-  if (ay == 0.0 && ax == 0.0) {
-    a = xneg ? PI_D : 0.0;
+  float a;
+  if (hi == 0.0f) {
+    a = xneg ? PI_F32 : 0.0f;
   } else {
-    const bool steep = ay > ax;
-    const double t = atan_ratio(steep ? ax : ay, steep ? ay : ax);
-    a = steep ? PIO2_D - t : t;
-    if (xneg) a = PI_D - a;
+    const bool big = lo > TAN_PI_12_F32 * hi;
+    const float num = big ? SQRT3_F32 * lo - hi : lo;
+    const float den = big ? lo + SQRT3_F32 * hi : hi;
+    const float s = num / den;
+    const float s2 = s * s;
+    float q = -1.0f / 11.0f;
+    q = q * s2 + 1.0f / 9.0f;
+    q = q * s2 - 1.0f / 7.0f;
+    q = q * s2 + 1.0f / 5.0f;
+    q = q * s2 - 1.0f / 3.0f;
+    q = q * s2;
+    float t = s + s * q;
+    if (big) t = PIO6_F32 + t;
+    a = steep ? PIO2_F32 - t : t;
+    if (xneg) a = PI_F32 - a;
   }
-  float r = (float)a;
-  return yneg ? -r : r;
+  return signbit(yf) ? -a : a;
 }
 
 // exp(x) for finite x; results below the float range flush to +0 (x < -104), above it to +inf (x > 89).

@@ -79,4 +79,41 @@ __device__ __forceinline__ float sample_r(const float* __restrict__ f, int nx, i
   return c0 + (c1 - c0) * fz;
 }
 
+// The same trilinear sample from a brick array: B[i][j][k] = (f(i,j,k), f(i,j,k+1), f(i,j+1,k), f(i,j+1,k+1)) with
+// the +1 neighbours clamped to the grid (nv_ray_bricks builds it). A sample's eight corners are then two
+// 16-byte loads, the bricks at (ia, jp, kp) and (ib, jp, kp) with jp = clamp(j0), kp = clamp(k0), instead of four
+// paired loads: half the load instructions, each touching no more cache lines. The clamped corners of sample_r
+// follow: jb = clamp(j0 + 1) is the brick's second row, except for j0 = -1, where jb = ja = 0 (the brick's first
+// row); j0 = ny - 1 gives jb = ny - 1, which the build already clamped into the second row. Likewise in k. Every
+// corner is the value sample_r reads and the interpolation is the same expression, so the result is identical.
+template <class OFF>
+__device__ __forceinline__ float4 ld4(const float4* __restrict__ B, unsigned i) {
+  return *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(B) + (OFF)i * 16u);
+}
+
+template <class OFF, class SC>
+__device__ __forceinline__ float sample_brick(const float4* __restrict__ B, int nx, int ny, int nz, float px, float py,
+                                              float pz, const SC& cs) {
+  const float gx = cs(px) - 0.5f, gy = cs(py) - 0.5f, gz = cs(pz) - 0.5f;
+  if (gx < -0.5f || gy < -0.5f || gz < -0.5f) return 0.0f;
+  if (gx > (float)nx - 0.5f || gy > (float)ny - 0.5f || gz > (float)nz - 0.5f) return 0.0f;
+  const int i0 = (int)floorf(gx), j0 = (int)floorf(gy), k0 = (int)floorf(gz);
+  const float fx = gx - (float)i0, fy = gy - (float)j0, fz = gz - (float)k0;
+  const unsigned ia = clampi(i0, 0, nx - 1), ib = clampi(i0 + 1, 0, nx - 1);
+  const unsigned jp = clampi(j0, 0, ny - 1), kp = clampi(k0, 0, nz - 1);
+  const bool jlow = j0 < 0, klow = k0 < 0;
+  const float4 a = ld4<OFF>(B, (ia * ny + jp) * nz + kp), b = ld4<OFF>(B, (ib * ny + jp) * nz + kp);
+  const float c000 = a.x, c100 = b.x;
+  const float c001 = klow ? a.x : a.y, c101 = klow ? b.x : b.y;
+  const float c010 = jlow ? a.x : a.z, c110 = jlow ? b.x : b.z;
+  const float c011 = jlow ? c001 : (klow ? a.z : a.w), c111 = jlow ? c101 : (klow ? b.z : b.w);
+  const float c00 = c000 + (c100 - c000) * fx;
+  const float c10 = c010 + (c110 - c010) * fx;
+  const float c01 = c001 + (c101 - c001) * fx;
+  const float c11 = c011 + (c111 - c011) * fx;
+  const float c0 = c00 + (c10 - c00) * fy;
+  const float c1 = c01 + (c11 - c01) * fy;
+  return c0 + (c1 - c0) * fz;
+}
+
 }  // namespace nv

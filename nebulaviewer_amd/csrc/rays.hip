@@ -71,6 +71,7 @@ __device__ __forceinline__ bool ray_of_thread(long long t, int nu, int nphi, int
 // One ray-cast job: a star, its direction grid and the density grid it marches through.
 struct RayJob {
   const float* rho;
+  const float4* bricks;   // nv_ray_bricks of rho (the two-load sampler), or null (the paired sampler)
   float* rfront;
   Recip rdx;         // dx and 1/dx correctly rounded (the shared reciprocal of the general cell scaling)
   int nx, ny, nz, nsteps, nu, nphi, wide;
@@ -82,7 +83,7 @@ struct RayJob {
 // of sample n+1 are in flight while sample n is tested, and the column is still accumulated sample by sample in
 // the reference order. WIDE: a ray whose marched length (n1 - n0) dr exceeds 512 cells keeps its column in FP64
 // (the test col + inc > Q and the interpolation of rf in double); shorter rays are unchanged.
-template <bool PAIR, bool WIDE, class OFF, class SC>
+template <bool PAIR, bool WIDE, class OFF, class SC, bool BRICK>
 __device__ __forceinline__ void cast_one(const RayJob& J, const SC& cs, long long t) {
   int iu, ip;
   if (!ray_of_thread(t, J.nu, J.nphi, &iu, &ip)) return;
@@ -118,8 +119,14 @@ __device__ __forceinline__ void cast_one(const RayJob& J, const SC& cs, long lon
   for (int n = n0; n < n1; n += 2) {
     const float ra = ((float)n + 0.5f) * dr;
     const float rb = ((float)(n + 1) + 0.5f) * dr;
-    const float na = sample_r<PAIR, OFF>(J.rho, nx, ny, nz, sx + d[0] * ra, sy + d[1] * ra, sz + d[2] * ra, cs);
-    const float nb = sample_r<PAIR, OFF>(J.rho, nx, ny, nz, sx + d[0] * rb, sy + d[1] * rb, sz + d[2] * rb, cs);
+    float na, nb;
+    if constexpr (BRICK) {
+      na = sample_brick<OFF>(J.bricks, nx, ny, nz, sx + d[0] * ra, sy + d[1] * ra, sz + d[2] * ra, cs);
+      nb = sample_brick<OFF>(J.bricks, nx, ny, nz, sx + d[0] * rb, sy + d[1] * rb, sz + d[2] * rb, cs);
+    } else {
+      na = sample_r<PAIR, OFF>(J.rho, nx, ny, nz, sx + d[0] * ra, sy + d[1] * ra, sz + d[2] * ra, cs);
+      nb = sample_r<PAIR, OFF>(J.rho, nx, ny, nz, sx + d[0] * rb, sy + d[1] * rb, sz + d[2] * rb, cs);
+    }
     if (step(ra, na * na * ra * ra * dr)) break;
     if (n + 1 >= n1) break;
     if (step(rb, nb * nb * rb * rb * dr)) break;
@@ -127,9 +134,22 @@ __device__ __forceinline__ void cast_one(const RayJob& J, const SC& cs, long lon
   J.rfront[(long long)iu * J.nphi + ip] = rf;
 }
 
-template <bool PAIR, bool WIDE, class OFF, class SC>
+template <bool PAIR, bool WIDE, class OFF, class SC, bool BRICK>
 __global__ __launch_bounds__(256) void cast_rays_kernel(RayJob J, SC cs) {
-  cast_one<PAIR, WIDE, OFF, SC>(J, cs, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+  cast_one<PAIR, WIDE, OFF, SC, BRICK>(J, cs, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// The brick array of a density field for the two-load sampler (sample_brick): B[i][j][k] = (f(i,j,k), f(i,j,k+1),
+// f(i,j+1,k), f(i,j+1,k+1)), the +1 neighbours clamped to the grid. One thread per cell of a z row.
+__global__ __launch_bounds__(256) void ray_bricks_kernel(const float* __restrict__ f, float4* __restrict__ B, int ny,
+                                                         int nz) {
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= nz) return;
+  const int j = blockIdx.y, i = blockIdx.z;
+  const long long c = ((long long)i * ny + j) * nz + k;
+  const int dk = k + 1 < nz ? 1 : 0;
+  const long long dj = j + 1 < ny ? nz : 0;
+  B[c] = make_float4(f[c], f[c + dk], f[c + dj], f[c + dj + dk]);
 }
 
 // blocks of 256 rays each job needs (64-ray direction patches, nebula ray_of_thread)
@@ -155,13 +175,13 @@ __global__ __launch_bounds__(256) void cast_rays_batch_kernel(const RayJob* __re
   if (J.inv != 0.0f) {
     ScalePow2 sc;
     sc.inv = J.inv;
-    if (J.wide) cast_one<PAIR, true, OFF>(J, sc, t);
-    else cast_one<PAIR, false, OFF>(J, sc, t);
+    if (J.wide) cast_one<PAIR, true, OFF, ScalePow2, false>(J, sc, t);
+    else cast_one<PAIR, false, OFF, ScalePow2, false>(J, sc, t);
   } else {
     ScaleDiv sc;
     sc.rdx = J.rdx;
-    if (J.wide) cast_one<PAIR, true, OFF>(J, sc, t);
-    else cast_one<PAIR, false, OFF>(J, sc, t);
+    if (J.wide) cast_one<PAIR, true, OFF, ScaleDiv, false>(J, sc, t);
+    else cast_one<PAIR, false, OFF, ScaleDiv, false>(J, sc, t);
   }
 }
 
@@ -535,46 +555,80 @@ static void dispatch_sampler(int nz, long long cells, float dx, F&& f) {
 static void fill_ray_job(const float* rho, int nx, int ny, int nz, float sx, float sy, float sz, float Q, float dr,
                          int nsteps, float dx, float rcap, float bx, float by, float bz, float* rfront, int nu,
                          int nphi, bool wide, RayJob& J) {
-  J.rho = rho; J.rfront = rfront;
+  J.rho = rho; J.bricks = nullptr; J.rfront = rfront;
   J.rdx = Recip{1.0 / (double)dx, dx};
   J.nx = nx; J.ny = ny; J.nz = nz; J.nsteps = nsteps; J.nu = nu; J.nphi = nphi; J.wide = wide ? 1 : 0;
   J.sx = sx; J.sy = sy; J.sz = sz; J.Q = Q; J.dr = dr; J.dx = dx; J.rcap = rcap; J.bx = bx; J.by = by; J.bz = bz;
   J.inv = pow2_reciprocal(dx);
 }
 
-static int cast_rays_launch(const char* name, bool wide, const float* rho, int nx, int ny, int nz, float sx,
-                            float sy, float sz, float Q, float dr, int nsteps, float dx, float rcap, float bx,
-                            float by, float bz, float* rfront, int nu, int nphi, void* stream) {
+static int cast_rays_launch(const char* name, bool wide, const float* rho, const float* bricks, int nx, int ny,
+                            int nz, float sx, float sy, float sz, float Q, float dr, int nsteps, float dx, float rcap,
+                            float bx, float by, float bz, float* rfront, int nu, int nphi, void* stream) {
   clear_error();
   NV_REQUIRE(rho && rfront, "%s: null pointer", name);
   NV_REQUIRE(nx > 0 && ny > 0 && nz > 0 && nu > 0 && nphi > 0, "%s: bad sizes", name);
   NV_REQUIRE(dr > 0.0f && dx > 0.0f, "%s: dr and dx must be > 0", name);
   NV_REQUIRE((long long)nx * ny * nz < (1LL << 32), "%s: grids above 2^32 cells are not supported", name);
+  NV_REQUIRE(!bricks || ((uintptr_t)bricks & 15) == 0, "%s: the brick array must be 16-byte aligned", name);
   RayJob J;
   fill_ray_job(rho, nx, ny, nz, sx, sy, sz, Q, dr, nsteps, dx, rcap, bx, by, bz, rfront, nu, nphi, wide, J);
+  J.bricks = reinterpret_cast<const float4*>(bricks);
   const long long blocks = ray_blocks(nu, nphi);
-  dispatch_sampler(nz, (long long)nx * ny * nz, dx, [&](auto pair, auto off, auto sc) {
-    constexpr bool P = decltype(pair)::value;
-    using O = decltype(off);
-    using SC = decltype(sc);
-    auto kern = wide ? cast_rays_kernel<P, true, O, SC> : cast_rays_kernel<P, false, O, SC>;
-    hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(256), 0, as_stream(stream), J, sc);
-  });
+  const long long cells = (long long)nx * ny * nz;
+  if (bricks) {
+    // 16-byte elements: 32-bit byte offsets up to 2^28 cells
+    dispatch_sampler(2, cells * 4, dx, [&](auto, auto off, auto sc) {
+      using O = decltype(off);
+      using SC = decltype(sc);
+      auto kern = wide ? cast_rays_kernel<true, true, O, SC, true> : cast_rays_kernel<true, false, O, SC, true>;
+      hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(256), 0, as_stream(stream), J, sc);
+    });
+  } else {
+    dispatch_sampler(nz, cells, dx, [&](auto pair, auto off, auto sc) {
+      constexpr bool P = decltype(pair)::value;
+      using O = decltype(off);
+      using SC = decltype(sc);
+      auto kern = wide ? cast_rays_kernel<P, true, O, SC, false> : cast_rays_kernel<P, false, O, SC, false>;
+      hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(256), 0, as_stream(stream), J, sc);
+    });
+  }
   return check_launch(name);
 }
 
 extern "C" int nv_cast_rays(const float* rho, int nx, int ny, int nz, float sx, float sy, float sz, float Q, float dr,
                             int nsteps, float dx, float rcap, float bx, float by, float bz, float* rfront, int nu,
                             int nphi, void* stream) {
-  return cast_rays_launch("nv_cast_rays", false, rho, nx, ny, nz, sx, sy, sz, Q, dr, nsteps, dx, rcap, bx, by, bz,
-                          rfront, nu, nphi, stream);
+  return cast_rays_launch("nv_cast_rays", false, rho, nullptr, nx, ny, nz, sx, sy, sz, Q, dr, nsteps, dx, rcap, bx,
+                          by, bz, rfront, nu, nphi, stream);
 }
 
 extern "C" int nv_cast_rays_wide(const float* rho, int nx, int ny, int nz, float sx, float sy, float sz, float Q,
                                  float dr, int nsteps, float dx, float rcap, float bx, float by, float bz,
                                  float* rfront, int nu, int nphi, void* stream) {
-  return cast_rays_launch("nv_cast_rays_wide", true, rho, nx, ny, nz, sx, sy, sz, Q, dr, nsteps, dx, rcap, bx, by,
-                          bz, rfront, nu, nphi, stream);
+  return cast_rays_launch("nv_cast_rays_wide", true, rho, nullptr, nx, ny, nz, sx, sy, sz, Q, dr, nsteps, dx, rcap,
+                          bx, by, bz, rfront, nu, nphi, stream);
+}
+
+extern "C" int nv_ray_bricks(const float* rho, int nx, int ny, int nz, float* bricks, void* stream) {
+  clear_error();
+  NV_REQUIRE(rho && bricks, "nv_ray_bricks: null pointer");
+  NV_REQUIRE(((uintptr_t)bricks & 15) == 0, "nv_ray_bricks: the brick array must be 16-byte aligned");
+  NV_REQUIRE(nx > 0 && ny > 0 && nz > 0 && nx <= 65535 && ny <= 65535, "nv_ray_bricks: bad sizes");
+  const long long cells = (long long)nx * ny * nz;
+  const char* r = reinterpret_cast<const char*>(rho);
+  const char* b = reinterpret_cast<const char*>(bricks);
+  NV_REQUIRE(b + 16 * cells <= r || r + 4 * cells <= b, "nv_ray_bricks: the brick array overlaps the field");
+  hipLaunchKernelGGL(ray_bricks_kernel, dim3((unsigned)((nz + 255) / 256), (unsigned)ny, (unsigned)nx), dim3(256), 0,
+                     as_stream(stream), rho, reinterpret_cast<float4*>(bricks), ny, nz);
+  return check_launch("nv_ray_bricks");
+}
+
+extern "C" int nv_cast_rays_ex(const float* rho, const float* bricks, int nx, int ny, int nz, float sx, float sy,
+                               float sz, float Q, float dr, int nsteps, float dx, float rcap, float bx, float by,
+                               float bz, float* rfront, int nu, int nphi, int wide, void* stream) {
+  return cast_rays_launch("nv_cast_rays_ex", wide != 0, rho, bricks, nx, ny, nz, sx, sy, sz, Q, dr, nsteps, dx, rcap,
+                          bx, by, bz, rfront, nu, nphi, stream);
 }
 
 extern "C" int nv_apply_front(const float* rfront, int nu, int nphi, float sx, float sy, float sz, float dx,

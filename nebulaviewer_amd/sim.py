@@ -18,6 +18,11 @@ two RK stages, clocks. Two fusions change no bits:
 precision: "fp32" reproduces the reference's float32 step bit for bit; "fp32-fast" uses fused multiply-adds and
 hardware reciprocals / square roots in the RK stage; "fp16-vel" stores float32 density with binary16 velocity
 (58 instead of 88 bytes of state traffic per cell and step) and then mx, my, mz are computed on access.
+
+long_rays: "fp64" (default) accumulates the recombination column of a ray in FP64 when the ray's chord through
+the box exceeds 512 cells, where float summation error moves the Stromgren front (nv_cast_rays_wide); shorter
+rays, and every ray of grids up to 294 cells on a side, are bit-identical to the float column. "fp32" keeps the
+reference's float column on every ray (nebula3d.py:262-268).
 """
 import copy
 import math
@@ -42,10 +47,18 @@ class Sim:
     PRECISIONS = ("fp32", "fp32-fast", "fp16-vel")
 
     def __init__(self, scenario="torus", res=None, params=None, rays=None, fuv_grid=None, ion_every=None,
-                 device="cuda:0", verbose=True, fast=False, setup=None, fused=True, seed=None, precision="fp32"):
+                 device="cuda:0", verbose=True, fast=False, setup=None, fused=True, seed=None, precision="fp32",
+                 long_rays="fp64", tile="tune"):
         if precision not in self.PRECISIONS:
             raise ValueError(f"precision must be one of {self.PRECISIONS}")
+        if long_rays not in ("fp64", "fp32"):
+            raise ValueError("long_rays must be 'fp64' or 'fp32'")
+        if tile != "tune" and tile not in (0, 1, 2, 3):
+            raise ValueError("tile must be 'tune' or an NV_TILE_* code (0 = the library's rule)")
         self.precision = precision
+        self.long_rays = long_rays
+        self.tile = tile
+        self.ray_bricks = True      # sample the density from its brick array in the ray casts (same bits)
         self.device = torch.device(device)
         self.fast = fast
         self.verbose = verbose
@@ -339,13 +352,22 @@ class Sim:
         strm = self._stream
         on = [i for i, s in enumerate(self.stars) if self.t >= s[4]]
         Lx, Ly, Lz = self.bmax
+        # the casts sample the density from its brick array (two 16-byte loads per sample), built in the float32
+        # RK stage block, which is free while ionize runs; the binary16-velocity form has no such block and
+        # samples the density field directly. Both give the same fronts bit for bit.
+        bricks = None
+        if self.ray_bricks and on and not self.vel_state:
+            bricks = self._p(self.stage)
+            tk = self._t0("ray_bricks")
+            _lib.call("nv_ray_bricks", self._p(self.U[0]), nx, ny, nz, bricks, strm)
+            self._t1(tk)
         tk = self._t0("cast_rays")
         for slot, i in enumerate(on):
             sx, sy, sz = self.stars[i][:3]
-            _lib.call("nv_cast_rays", self._p(self.U[0]), nx, ny, nz,
+            _lib.call("nv_cast_rays_ex", self._p(self.U[0]), bricks, nx, ny, nz,
                       _f(sx), _f(sy), _f(sz), _f(self.q_eff(i)),
                       _f(self.dr), self.nsteps, _f(self.dx), _f(self.diag), _f(Lx), _f(Ly), _f(Lz),
-                      self._p(self.rfronts[slot]), nu, nphi, strm)
+                      self._p(self.rfronts[slot]), nu, nphi, 1 if self.long_rays == "fp64" else 0, strm)
         self._t1(tk)
         width = 1.2 * self.dx
         half = self.dfront.dtype == torch.float16
@@ -447,27 +469,51 @@ class Sim:
         self._smax_ok = False
         return self.P["cfl"] * self.dx / smax
 
+    def _solver_lim(self):
+        solver = S.SOLVERS[self.P.get("solver", "rusanov")]
+        return solver, (S.LIMITERS[self.P.get("limiter", "minmod")] if solver == 1 else 0)
+
+    def _stage_desc(self, src, u0, dst, a, b, dt, nstars, wind, pulse, smax_ptr):
+        """the arguments of one RK stage (nv_hydro_stage, sim.py:1017-1028) as an NvStageDesc; float fields are
+        rounded to float32 as a Warp launch rounds its scalar arguments"""
+        P = self.P
+        d = _lib.NvStageDesc()
+        for c in range(4):
+            d.u[c], d.u0[c], d.out[c] = src[c], u0[c], dst[c]
+        d.xi = self._p(self.xi)
+        d.stars, d.jets = self._p(self.star_arr), self._p(self.jet_arr)
+        d.smax_out = smax_ptr
+        d.dt_dev = None
+        d.nx, d.ny, d.nz = self.shape
+        d.nstars = nstars
+        d.wind, d.jet_cos, d.jet_r, d.pulse = wind, self.C["jet_cos"], P["jet_r"], pulse
+        d.dt, d.dx, d.a, d.b = dt, self.dx, a, b
+        d.floor_, d.cn2, d.ci2, d.vmax = P["floor"], self.cn2, self.ci2, 6.0
+        return d
+
     def _hydro(self, dt, nstars, wind, pulse):
         """Two RK stages (Sim._hydro, sim.py:1014-1028): U1 = U + dt L(U); U = U/2 + (U1 + dt L(U1))/2 in place.
-        Stage 2 also reduces the next step's CFL signal speed into self.smax."""
-        P = self.P
-        nx, ny, nz = self.shape
-        solver = S.SOLVERS[P.get("solver", "rusanov")]
-        lim = S.LIMITERS[P.get("limiter", "minmod")] if solver == 1 else 0
+        Stage 2 also reduces the next step's CFL signal speed into self.smax. The workgroup tile is the measured
+        fastest for this grid (tile="tune", nebulaviewer_amd.autotune) or the one given; no tile changes a bit."""
+        import ctypes
+        solver, lim = self._solver_lim()
+        if self.tile == "tune":
+            from . import autotune
+            self.tile_shape = autotune.tune_sim(self)
+        else:
+            self.tile_shape = self.tile
+        code = {"fp32": 0, "fp32-fast": 1, "fp16-vel": 2}[self.precision]
         strm = self._stream
         U = [self._p(t) for t in self.U]
         U1 = [self._p(self.stage[c]) for c in range(4)]
-        entry = {"fp32": "nv_hydro_stage", "fp32-fast": "nv_hydro_stage_fast",
-                 "fp16-vel": "nv_hydro_stage_h"}[self.precision]
-        common = [nx, ny, nz, self._p(self.star_arr), nstars, _f(wind), self._p(self.jet_arr), _f(self.C["jet_cos"]),
-                  _f(P["jet_r"]), _f(pulse), _f(dt), _f(self.dx)]
-        tail = [_f(P["floor"]), _f(self.cn2), _f(self.ci2), _f(6.0), solver, lim]
+        d1 = self._stage_desc(U, U, U1, 0.0, 1.0, dt, nstars, wind, pulse, None)
+        d2 = self._stage_desc(U1, U, U, 0.5, 0.5, dt, nstars, wind, pulse, self._p(self.smax))
         tk = self._t0("hydro_stage")
-        _lib.call(entry, *U, self._p(self.xi), *U, *U1, *common, _f(0.0), _f(1.0), *tail, None, strm)
+        _lib.call("nv_hydro_stage_ex", ctypes.byref(d1), code, solver, lim, self.tile_shape, strm)
         self._t1(tk)
         self.smax.zero_()
         tk = self._t0("hydro_stage")
-        _lib.call(entry, *U1, self._p(self.xi), *U, *U, *common, _f(0.5), _f(0.5), *tail, self._p(self.smax), strm)
+        _lib.call("nv_hydro_stage_ex", ctypes.byref(d2), code, solver, lim, self.tile_shape, strm)
         self._t1(tk)
         self._smax_ok = True
 

@@ -656,6 +656,10 @@ void or_to_half(const float* src, uint16_t* dst, long long n, int nthreads) {
 float or_sinf_det(float x) { return nvm::sinf_det(x); }
 float or_cosf_det(float x) { return nvm::cosf_det(x); }
 float or_atan2f_det(float y, float x) { return nvm::atan2f_det(y, x); }
+void or_atan2f_det_n(const float* y, const float* x, float* out, long long n, int nthreads) {
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+  for (long long q = 0; q < n; ++q) out[q] = nvm::atan2f_det(y[q], x[q]);
+}
 float or_expf_det(float x) { return nvm::expf_det(x); }
 
 }  // extern "C"

@@ -61,6 +61,7 @@ def load():
             getattr(_lib, fn).restype = ctypes.c_float
         _lib.or_atan2f_det.argtypes = [ctypes.c_float, ctypes.c_float]
         _lib.or_atan2f_det.restype = ctypes.c_float
+        _lib.or_atan2f_det_n.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_longlong, ctypes.c_int]
     return _lib
 
 
@@ -195,10 +196,12 @@ class OracleSim:
     """The reference Sim step on the CPU (float32 = parity oracle, float64 = error-budget oracle).
 
     setup: nebulaviewer_amd.scenarios.Setup; consts: nebulaviewer_amd.scenarios.constants(setup, ...);
-    rays = (nu, nphi); fuv = (nr, nu, nphi)."""
+    rays = (nu, nphi); fuv = (nr, nu, nphi); long_rays = "fp64": the float oracle keeps the column of rays whose
+    chord exceeds 512 cells in FP64 (or_cast_rays_wide_f32), "fp32": the reference's float column everywhere."""
 
-    def __init__(self, setup, consts, rays, fuv, dtype=np.float32, threads=None):
+    def __init__(self, setup, consts, rays, fuv, dtype=np.float32, threads=None, long_rays="fp64"):
         self.K = Kernels(dtype, threads)
+        self.wide = long_rays == "fp64" and self.K.s == "f32"
         dt_ = self.K.dtype
         self.setup, self.C = setup, consts
         self.P = dict(setup.P)
@@ -308,7 +311,7 @@ class OracleSim:
             if self.t < ton:
                 continue
             K.cast_rays(self.rho, (sx, sy, sz), self.q_eff(i), C["dr"], C["nsteps"], self.dx, C["diag"], C["bmax"],
-                        self.rfront)
+                        self.rfront, wide=self.wide)
             K.apply_front(self.rfront, (sx, sy, sz), self.dx, 1.2 * self.dx, self.xi, df)
         K.smooth_xi(self.xi, self.stage[0])
         self.xi[...] = self.stage[0]

@@ -147,6 +147,14 @@ def test_rejects_bad_enums_and_sizes(lib):
     rc = lib.nv_cast_rays(p, 8, 8, 8, f(0), f(0), f(0), f(1), f(0), 10, f(0.1), f(1), f(1), f(1), f(1), p, 4, 4,
                           None)
     assert rc == 1001 and b"dr" in lib.nv_last_error()
+    # the brick array: 16-byte aligned, not overlapping the field
+    rc = lib.nv_ray_bricks(ctypes.c_void_p(4096), 8, 8, 8, ctypes.c_void_p(4096 + 8), None)
+    assert rc == 1001 and b"aligned" in lib.nv_last_error()
+    rc = lib.nv_ray_bricks(ctypes.c_void_p(4096), 8, 8, 8, ctypes.c_void_p(4096 + 16 * 64), None)
+    assert rc == 1001 and b"overlaps" in lib.nv_last_error()
+    rc = lib.nv_cast_rays_ex(p, ctypes.c_void_p(24), 8, 8, 8, f(0), f(0), f(0), f(1), f(0.1), 10, f(0.1), f(1), f(1),
+                             f(1), f(1), p, 4, 4, 0, None)
+    assert rc == 1001 and b"aligned" in lib.nv_last_error()
 
 
 def test_error_is_raised_in_python(lib):

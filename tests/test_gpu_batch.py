@@ -91,3 +91,30 @@ def test_batch_fp16_velocity_matches_single(torch_dev):
     b = SimBatch(setups(), precision="fp16-vel", rays=RAYS)
     b.step(4)
     check_members(b, 4, "fp16-vel")
+
+
+def test_autotuned_tiles_change_no_bit(torch_dev):
+    """The measured tile choice (Sim / SimBatch tile="tune") against every fixed tile: identical fields and clocks;
+    the tuner timed all three shapes and kept the fastest."""
+    from nebulaviewer_amd import autotune
+    from nebulaviewer_amd.batch import SimBatch
+    from nebulaviewer_amd.sim import Sim
+    from nebulaviewer_amd import scenarios as S
+    say("[autotune] Sim 40^3 torus and a batch, tuned vs fixed tiles")
+    runs = {}
+    for tile in ("tune", 0, 1, 2, 3):
+        sim = Sim(setup=S.make_setup("torus", 40, 0), verbose=False, rays=RAYS, tile=tile)
+        sim.step(4)
+        runs[tile] = sim
+    for tile in (0, 1, 2, 3):
+        for name in ("rho", "mx", "my", "mz", "xi"):
+            same(getattr(runs["tune"], name), getattr(runs[tile], name), f"tile {tile} {name}")
+        assert runs["tune"].t == runs[tile].t
+    assert runs["tune"].tile_shape in (1, 2, 3)
+    rep = autotune.report()
+    key = [k for k in rep if "'sim', (40, 40, 40)" in k]
+    assert key and set(rep[key[0]]["ms"]) == {"64x8", "32x16", "64x16"}
+    b = SimBatch(setups(), rays=RAYS)
+    b.step(3)
+    assert b.tile in (1, 2, 3)
+    check_members(b, 3, "fp32")

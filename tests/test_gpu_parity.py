@@ -288,6 +288,49 @@ def test_cast_rays_sampler_forms_bitwise(env, okern32, shape, dx):
         np.testing.assert_array_equal(N(g), ref)
 
 
+def bricks_numpy(rho):
+    """(rho(i,j,k), rho(i,j,k+1), rho(i,j+1,k), rho(i,j+1,k+1)), the +1 neighbours clamped"""
+    jn = np.minimum(np.arange(rho.shape[1]) + 1, rho.shape[1] - 1)
+    kn = np.minimum(np.arange(rho.shape[2]) + 1, rho.shape[2] - 1)
+    return np.stack([rho, rho[:, :, kn], rho[:, jn, :], rho[:, jn][:, :, kn]], axis=-1)
+
+
+@pytest.mark.parametrize("shape,dx", [((40, 36, 44), 1 / 44), ((32, 16, 64), 1 / 64), ((20, 24, 1), 1 / 24),
+                                      ((9, 7, 2), 0.1), ((7, 1, 30), 1 / 30), ((1, 1, 1), 0.5)])
+def test_cast_rays_bricks_bitwise(env, okern32, shape, dx):
+    """nv_ray_bricks against its numpy construction, and nv_cast_rays_ex sampling the bricks against the oracle (float
+    and FP64-column forms), with stars inside, at the faces and outside the box; ny = 1, nz = 1 and a single cell
+    exercise every clamped corner"""
+    say(f"[cast_rays bricks] shape={shape} dx={dx}")
+    torch, dev, L = env
+    rho = np.random.default_rng(41).lognormal(0, 1.0, shape).astype(F32)
+    grho = T(env, rho)
+    gb = torch.full(shape + (4,), -1.0, dtype=torch.float32, device=dev)
+    L.call("nv_ray_bricks", grho.data_ptr(), *shape, gb.data_ptr(), stream(env))
+    np.testing.assert_array_equal(N(gb), bricks_numpy(rho))
+    Lb = tuple(n * dx for n in shape)
+    diag = math.sqrt(sum(v * v for v in Lb)) + 0.6
+    dr = 0.5 * dx
+    nsteps = int(diag / dr) + 1
+    nu, nphi = 24, 40
+    for star in ((0.37 * Lb[0], 0.52 * Lb[1], 0.45 * Lb[2]), (0.0, Lb[1], 0.5 * Lb[2]), (-0.3, 0.4, 1.2 * Lb[2])):
+        for Q in (1e-4, 3e-3, 1e3):
+            for wide in (0, 1):
+                ref = np.zeros((nu, nphi), F32)
+                okern32.cast_rays(rho, star, Q, dr, nsteps, dx, diag, Lb, ref, wide=bool(wide))
+                g = torch.zeros((nu, nphi), dtype=torch.float32, device=dev)
+                L.call("nv_cast_rays_ex", grho.data_ptr(), gb.data_ptr(), *shape, *(f(v) for v in star), f(Q), f(dr),
+                       nsteps, f(dx), f(diag), *(f(v) for v in Lb), g.data_ptr(), nu, nphi, wide, stream(env))
+                np.testing.assert_array_equal(N(g), ref, err_msg=f"star {star} Q {Q} wide {wide}")
+    # bricks = null: the paired sampler, the same entry point
+    ref = np.zeros((nu, nphi), F32)
+    okern32.cast_rays(rho, star, 3e-3, dr, nsteps, dx, diag, Lb, ref)
+    g = torch.zeros((nu, nphi), dtype=torch.float32, device=dev)
+    L.call("nv_cast_rays_ex", grho.data_ptr(), None, *shape, *(f(v) for v in star), f(3e-3), f(dr), nsteps, f(dx),
+           f(diag), *(f(v) for v in Lb), g.data_ptr(), nu, nphi, 0, stream(env))
+    np.testing.assert_array_equal(N(g), ref)
+
+
 def test_cast_rays_wide_bitwise(env, okern32, okern64):
     """The FP64 long-ray column against its oracle on the stalled-column case, and the float form beside it."""
     import long_ray_case as LR
@@ -471,6 +514,35 @@ def test_sim_sheet_256_matches_oracle(env):
     """The config-1 grid (256^3, 4 stars, wind, Mach-5 velocity), 2 steps with one ionization, bitwise."""
     sim, ref = run_pair(env, "sheet", 256, 2, (512, 1024), (160, 160, 320))
     assert_same_state(sim, ref)
+
+
+def test_sim_long_rays_matches_oracle(env, okern64):
+    """Rays whose chord exceeds 512 cells (a 640 x 24 x 24 grid, the stalled-column density of
+    tests/long_ray_case.py, the star at the low-x face): with long_rays="fp64" (the default) Sim keeps their column in
+    FP64 and equals the oracle's FP64-column run bit for bit, with "fp32" the float-column run; the two differ."""
+    sys.path.insert(0, __import__("os").path.dirname(__file__))
+    import long_ray_case as LR
+    from nebulaviewer_amd import scenarios as S
+    from nebulaviewer_amd.sim import Sim
+    from oracle.oracle import OracleSim
+    rho, star, dr, nsteps, dx, diag, Lb = LR.geometry()
+    Q = LR.find_q(okern64)
+    P = S.make_params(dict(q_main=Q))
+
+    def setup():
+        return S.Setup("long", LR.SHAPE, LR.SHAPE[0], dict(P), rho.copy(), stars=[(*star, 1.0, 0.0)],
+                       kinds=["main"], jets=[(0.0, 0.0, 1.0)], jkinds=["none"])
+
+    xis = {}
+    for mode in ("fp64", "fp32"):
+        say(f"[sim] long rays {LR.SHAPE}, long_rays={mode}, 3 steps")
+        sim = Sim(setup=setup(), rays=LR.RAYS, fuv_grid=(8, 8, 16), verbose=False, long_rays=mode)
+        ref = OracleSim(setup(), S.constants(setup()), LR.RAYS, (8, 8, 16), np.float32, threads=16, long_rays=mode)
+        sim.step(3)
+        ref.step(3)
+        assert_same_state(sim, ref)
+        xis[mode] = N(sim.xi)
+    assert not np.array_equal(xis["fp64"], xis["fp32"])
 
 
 # ------------------------------------------------------------------ size-independent properties at full size

@@ -143,6 +143,40 @@ def test_half_and_fp32_against_fp64_oracle(env):
         assert e <= tol, (prec, e)
 
 
+# Per configuration, at a reduced grid with the configuration's own step count: every field of each form against
+# the FP64 oracle run (rel-L1 per field, xi disagreement among front cells). Bars: fp32 and fp32-fast meet the
+# drop-in tolerance (1e-5 on rho and momentum, 1e-4 of front cells) on every configuration. fp16-vel meets the
+# 1e-3 density bar on the torus, the wall and the torus with jet (measured 1.4e-4, 3.5e-4, 6.5e-4), but not on the
+# Mach-5 sheet: binary16 velocity loses 2^-12 per stage, and over 1000 stages of supersonic turbulence the density
+# drifts to 1.7e-2 (its bar here is that measured value with margin, a regression bound, not the target).
+PER_CONFIG = [("torus", 64, 100), ("sheet", 48, 500), ("wall", 48, 300), ("torus_jet", 48, 200)]
+HALF_RHO_BAR = {"torus": 1e-3, "sheet": 3e-2, "wall": 1e-3, "torus_jet": 1e-3}
+
+
+@pytest.mark.parametrize("scenario,n,steps", PER_CONFIG)
+def test_forms_against_fp64_per_config(env, scenario, n, steps):
+    from nebulaviewer_amd import scenarios as S
+    from nebulaviewer_amd.sim import Sim
+    from oracle.oracle import OracleSim
+    say(f"[per-config precision] {scenario} {n}^3 {steps} steps: FP64 oracle, then fp32 / fp32-fast / fp16-vel")
+    setup = S.make_setup(scenario, n)
+    ref = OracleSim(setup, S.constants(setup), S.ray_grid(n), S.fuv_grid(n), np.float64, threads=16)
+    ref.step(steps)
+    for prec in ("fp32", "fp32-fast", "fp16-vel"):
+        sim = Sim(setup=S.make_setup(scenario, n), verbose=False, precision=prec)
+        sim.step(steps)
+        errs = {name: rel_l1(getattr(sim, name).cpu().numpy(), getattr(ref, name)) for name in ("rho", "mx", "my", "mz")}
+        frac, nf = front_disagreement(sim.xi.cpu().numpy(), ref.xi)
+        say(f"  {prec:9s} rel-L1 " + " ".join(f"{k} {v:.2e}" for k, v in errs.items()) +
+            f"; xi front disagreement {frac:.2e} of {nf}")
+        if prec == "fp16-vel":
+            assert errs["rho"] <= HALF_RHO_BAR[scenario], (scenario, errs)
+        else:
+            for name, e in errs.items():
+                assert e <= 1e-5, (prec, name, e)
+            assert frac <= 1e-4, (prec, frac, nf)
+
+
 def test_half_velocity_wall_512_300_steps(env):
     from nebulaviewer_amd import scenarios as S
     from nebulaviewer_amd.sim import Sim

@@ -42,19 +42,44 @@ def test_sincos_ray_angles(lib):
 
 
 def test_atan2_quadrants_and_zeros(lib):
+    # float-only evaluation: within 3 ulp of the correctly rounded atan2 on random pairs of every scale and sign,
+    # on the octant diagonal, around the reduction's switch at tan(pi/12) and its cancellation at 1/sqrt3, and
+    # on the cell-centre offsets the ionize pass feeds it
     rng = np.random.default_rng(1)
-    y = (rng.standard_normal(20000) * 10.0 ** rng.integers(-3, 3, 20000)).astype(np.float32)
-    x = (rng.standard_normal(20000) * 10.0 ** rng.integers(-3, 3, 20000)).astype(np.float32)
-    y = np.concatenate([y, np.float32([1, -1, 1, -1, 0.5, 3])])
-    x = np.concatenate([x, np.float32([1, 1, -1, -1, 0.5, 3])])
-    got = np.array([lib.or_atan2f_det(float(a), float(b)) for a, b in zip(y, x)], np.float32)
-    ref = np.array([math.atan2(float(a), float(b)) for a, b in zip(y, x)]).astype(np.float32)
-    assert ulp_dist(got, ref).max() <= 1
-    # C99 signed-zero conventions
+    n = 400_000
+    sgn = lambda: rng.choice([-1.0, 1.0], n)
+    mag = rng.uniform(1e-3, 10.0, n)
+    cases = [
+        (rng.standard_normal(n) * 10.0 ** rng.integers(-6, 6, n), rng.standard_normal(n) * 10.0 ** rng.integers(-6, 6, n)),
+        (mag * (1 + rng.uniform(-1e-3, 1e-3, n)) * sgn(), mag * sgn()),
+        (mag * 0.2679491924311227 * (1 + rng.uniform(-1e-4, 1e-4, n)) * sgn(), mag * sgn()),
+        (mag * 0.5773502691896258 * (1 + rng.uniform(-1e-3, 1e-3, n)) * sgn(), mag * sgn()),
+    ]
+    dx = np.float32(1.0 / 512)
+    ij = rng.integers(0, 512, (2, n)).astype(np.float32)
+    cases.append(((ij[1] + np.float32(0.5)) * dx - np.float32(0.37), (ij[0] + np.float32(0.5)) * dx - np.float32(0.5)))
+    worst = 0
+    for y, x in cases:
+        y = np.ascontiguousarray(y, np.float32)
+        x = np.ascontiguousarray(x, np.float32)
+        got = np.empty_like(y)
+        lib.or_atan2f_det_n(y.ctypes.data, x.ctypes.data, got.ctypes.data, n, 4)
+        ref = np.arctan2(y.astype(np.float64), x.astype(np.float64)).astype(np.float32)
+        worst = max(worst, int(ulp_dist(got, ref).max()))
+    assert worst <= 3
+    # the scalar entry point is the same function
+    y, x = np.float32([0.5, -3.0, 1e-3]), np.float32([3.0, 0.25, -7.0])
+    many = np.empty_like(y)
+    lib.or_atan2f_det_n(y.ctypes.data, x.ctypes.data, many.ctypes.data, 3, 1)
+    assert [np.float32(lib.or_atan2f_det(float(a), float(b))) for a, b in zip(y, x)] == list(many)
+    # axes and C99 signed-zero conventions
     assert lib.or_atan2f_det(0.0, 1.0) == 0.0 and math.copysign(1, lib.or_atan2f_det(-0.0, 1.0)) < 0
     assert lib.or_atan2f_det(0.0, -1.0) == np.float32(math.pi)
     assert lib.or_atan2f_det(-0.0, -1.0) == -np.float32(math.pi)
     assert lib.or_atan2f_det(0.0, -0.0) == np.float32(math.pi)
+    assert lib.or_atan2f_det(1.0, 0.0) == np.float32(math.pi / 2)
+    assert lib.or_atan2f_det(-2.0, 0.0) == -np.float32(math.pi / 2)
+    assert ulp_dist(lib.or_atan2f_det(1.0, 1.0), np.float32(math.pi / 4)) <= 1
 
 
 def test_exp_range(lib):
