@@ -188,7 +188,7 @@ def make_measure_batch(a, S, KernelTimer):
                  hydro_only_cell_updates_per_s=cells / (2 * hyd_s),
                  hydro_achieved_GBps=BYTES_PER_CELL_STEP["fp32"] / 2 * cells / hyd_s / 1e9,
                  kernels_ms={k: round(v["avg_ms"], 4) for k, v in ks.items()}, graph_replay=True,
-                 tile={1: "64x8", 2: "32x16", 3: "64x16"}.get(b.tile))
+                 tile={1: "64x8", 2: "32x16", 3: "64x16"}.get(b.tile), ray_bricks=getattr(b, "ray_bricks_used", None))
         log(f"[bench] batch: {r['value'] / 1e9:.2f} G cell-updates/s ({cells / 1e6:.1f} M cells, {r['stars']} stars), "
             f"{r['ms_per_step']:.3f} ms/step, hydro {hyd_s * 1e3:.3f} ms/launch; kernels {r['kernels_ms']}")
         del b
@@ -279,6 +279,7 @@ def main():
                  hydro_only_cell_updates_per_s=cells / (2 * hyd_s), hydro_achieved_GBps=achieved,
                  kernels_ms={k: round(v["avg_ms"], 4) for k, v in ks.items()}, ion_every=sim.ion_every,
                  rays=tuple(sim.rfronts.shape[1:]), nstars=len(sim.stars), tile=autotune.TILES.get(sim.tile_shape),
+                 ray_bricks=getattr(sim, "ray_bricks_used", None),
                  ionization_roofline=ionization_roofline(ks, cells, sum(1 for st in sim.stars if sim.t >= st[4]),
                                                          precision))
         log(f"[bench] {precision}: {r['value'] / 1e9:.2f} G cell-updates/s, {r['ms_per_step']:.3f} ms/step, "

@@ -55,7 +55,9 @@ typedef struct NvStageDesc {
   float wind, jet_cos, jet_r, pulse, dt, dx, a, b, floor_, cn2, ci2, vmax;
 } NvStageDesc;
 
-/* One ray-cast job of a batch: the arguments of nv_cast_rays (wide != 0: those of nv_cast_rays_wide). */
+/* One ray-cast job of a batch: the arguments of nv_cast_rays (wide != 0: those of nv_cast_rays_wide). bricks:
+ * null, or a 16-byte aligned scratch of 16 bytes per cell that nv_cast_rays_batch fills with rho's brick array
+ * (as nv_ray_bricks) before the casts sample it; jobs on one grid pass the same rho and bricks, built once. */
 typedef struct NvRayDesc {
   const float* rho;
   int nx, ny, nz;
@@ -64,6 +66,7 @@ typedef struct NvRayDesc {
   float dx, rcap, bx, by, bz;
   float* rfront;
   int nu, nphi, wide;
+  float* bricks;
 } NvRayDesc;
 
 /* One grid of a batched ionize pass: the arguments of nv_ionize_apply (state = rho, mx, my, mz, or rho, vx, vy, vz
@@ -207,7 +210,8 @@ int nv_cast_rays_ex(const float* rho, const float* bricks, int nx, int ny, int n
                     int nu, int nphi, int wide, void* stream);
 
 /* Batched ray casts: every (grid, star) job in one launch, each job's rfront bit-identical to nv_cast_rays
- * (nv_cast_rays_wide when wide != 0) on that job alone. No reference counterpart (Sim.ionize launches per star,
+ * (nv_cast_rays_wide when wide != 0) on that job alone. Jobs with bricks get their grid's brick array built first,
+ * in one more launch over all such grids. No reference counterpart (Sim.ionize launches per star,
  * sim.py:936-937). Plan and launch as for nv_hydro_stage_batch. */
 long long nv_cast_rays_batch_plan_size(int n);
 int nv_cast_rays_batch_plan(const NvRayDesc* jobs, int n, void* plan, long long plan_bytes);

@@ -76,7 +76,7 @@ class NvRayDesc(ctypes.Structure):
     """one job of nv_cast_rays_batch_plan"""
     _fields_ = [("rho", P), ("nx", I), ("ny", I), ("nz", I), ("sx", F), ("sy", F), ("sz", F), ("Q", F), ("dr", F),
                 ("nsteps", I), ("dx", F), ("rcap", F), ("bx", F), ("by", F), ("bz", F), ("rfront", P), ("nu", I),
-                ("nphi", I), ("wide", I)]
+                ("nphi", I), ("wide", I), ("bricks", P)]
 
 
 class NvIonDesc(ctypes.Structure):
@@ -111,7 +111,7 @@ def load(path=LIB_PATH):
         fn = getattr(lib, name)
         fn.argtypes = args
         fn.restype = RESTYPES.get(name, ctypes.c_int)
-    if lib.nv_abi_version() != 1:
+    if lib.nv_abi_version() != 2:
         raise NebulaHipError("libnebula_hip.so ABI version mismatch")
     _lib = lib
     return lib

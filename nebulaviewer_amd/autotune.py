@@ -1,4 +1,4 @@
-"""Measured choice of the RK stage's workgroup tile.
+"""Measured launch choices: the RK stage's workgroup tile, and whether the ray casts sample density bricks.
 
 The stage kernel comes in three tile shapes (NV_TILE_* in include/nebula_hip.h): 64 z x 8 y and 32 z x 16 y
 (8 wavefronts per workgroup, two workgroups per CU) and 64 z x 16 y (16 wavefronts, one workgroup per CU, half
@@ -9,9 +9,16 @@ the stage on the caller's own fields, timed with HIP events on the launch stream
 choice is cached per problem signature for the life of the process; the library's own rule (NV_TILE_AUTO, by idle
 lanes) is what runs when tuning is switched off.
 
-    tile = tune_sim(sim)        # Sim(tile="tune") does this on its first RK stage
-    tile = tune_batch(batch)    # SimBatch(tile="tune") does this when it builds its stage plans
-    report()                    # {signature: {"tile": t, "ms": {name: ms, ...}}}
+The ray casts can sample the density through its brick array (nv_ray_bricks, two 16-byte loads per trilinear
+sample) or directly (four paired loads); both give the same fronts bit for bit. Bricks pay on large grids, where
+the casts are bound by the cache lines their gathers touch (512^3: 3.2 ms of casts become 1.8 ms plus a 0.5 ms
+build), and lose on small ones, whose density stays in the caches anyway (the 64-member batch of 48^3-160^3 grids:
+2.7 ms direct, 3.0 ms with bricks), so that choice is measured too, on the first ionization.
+
+    tile = tune_sim(sim)                 # Sim(tile="tune") does this on its first RK stage
+    tile = tune_batch(batch, pulses)     # SimBatch(tile="tune") does this when it builds its stage plans
+    use = tune_rays(sim, nstars, casts)  # Sim(ray_bricks="auto") on its first ionization
+    report()                             # {signature: {"choice": name, "ms": {name: ms, ...}}}
 """
 import ctypes
 
@@ -26,10 +33,17 @@ PRECISION_CODES = {"fp32": 0, "fp32-fast": 1, "fp16-vel": 2}
 _cache = {}
 
 
+NAMES = {"sim": TILES.get, "batch": TILES.get, "rays": {False: "direct", True: "bricks"}.get,
+         "batch_rays": {False: "direct", True: "bricks"}.get}
+
+
 def report():
-    """every tuning result of this process: {signature: {"tile": name, "ms": {tile name: ms per launch}}}"""
-    return {str(k): {"tile": TILES[v["tile"]], "ms": {TILES[t]: round(ms, 4) for t, ms in v["ms"].items()}}
-            for k, v in _cache.items()}
+    """every tuning result of this process: {signature: {"choice": name, "ms": {candidate name: ms per launch}}}"""
+    out = {}
+    for k, v in _cache.items():
+        name = NAMES[k[0]]
+        out[str(k)] = {"choice": name(v["choice"]), "ms": {name(c): round(ms, 4) for c, ms in v["ms"].items()}}
+    return out
 
 
 def _time(launch, stream_dev, reps, warm=2):
@@ -48,10 +62,10 @@ def _time(launch, stream_dev, reps, warm=2):
 
 def _pick(key, measure, candidates):
     if key in _cache:
-        return _cache[key]["tile"]
+        return _cache[key]["choice"]
     ms = {t: measure(t) for t in candidates}
     best = min(ms, key=ms.get)
-    _cache[key] = {"tile": best, "ms": ms}
+    _cache[key] = {"choice": best, "ms": ms}
     return best
 
 
@@ -62,7 +76,7 @@ def tune_sim(sim, candidates=CANDIDATES, reps=5):
     solver, lim = sim._solver_lim()
     key = ("sim", tuple(sim.shape), sim.precision, solver, lim)
     if key in _cache:
-        return _cache[key]["tile"]
+        return _cache[key]["choice"]
     code = PRECISION_CODES[sim.precision]
     scratch = torch.zeros(1, dtype=torch.float32, device=sim.device)
     U = [t.data_ptr() for t in sim.U]
@@ -86,7 +100,7 @@ def tune_batch(batch, pulses, candidates=CANDIDATES, reps=3):
     launch (it writes only the members' scratch blocks)."""
     key = ("batch", tuple(tuple(m.shape) for m in batch.members), batch.precision, batch.solver, batch.lim)
     if key in _cache:
-        return _cache[key]["tile"]
+        return _cache[key]["choice"]
 
     def measure(tile):
         batch.tile = tile
@@ -98,3 +112,20 @@ def tune_batch(batch, pulses, candidates=CANDIDATES, reps=3):
     best = _pick(key, measure, candidates)
     batch._stage_key = None
     return best
+
+
+def tune_rays(sim, nstars, casts, reps=2):
+    """True when the brick-sampled casts (brick build included) beat the direct ones for this grid, ray grid and
+    number of active stars. casts(use_bricks) launches one ionization's casts; both forms write the same fronts."""
+    key = ("rays", tuple(sim.shape), tuple(sim.rfronts.shape[1:]), nstars)
+    return _pick(key, lambda use: _time(lambda: casts(use), sim.device, reps), (False, True))
+
+
+def tune_batch_rays(batch, plans, reps=2):
+    """True when the batch's brick-sampled ray plan beats the direct one; plans = {False: plan, True: plan}"""
+    key = ("batch_rays", tuple(tuple(m.shape) for m in batch.members), tuple(len(batch._on(m)) for m in batch.members))
+
+    def measure(use):
+        plan = plans[use]
+        return _time(lambda: _lib.call("nv_cast_rays_batch", *plan.args, batch._stream), batch.device, reps)
+    return _pick(key, measure, (False, True))

@@ -56,6 +56,7 @@ class SimBatch:
         # or 1 / 2 / 3 (64x8, 32x16, 64x16); no shape changes a bit of the result
         self.tune = tile == "tune"
         self.tile = 0 if self.tune else tile
+        self.ray_bricks = "auto"  # ray casts through density bricks: True, False or measured ("auto")
         self.members = [Sim(setup=st, verbose=verbose, precision=precision, device=device, rays=rays,
                             ion_every=ion_every) for st in setups]
         m0 = self.members[0]
@@ -208,6 +209,9 @@ class SimBatch:
                 j.rfront = m.rfronts[slot].data_ptr()
                 j.nu, j.nphi = nu, nphi
                 j.wide = 1 if m.long_rays == "fp64" else 0
+                # the member's float32 RK stage block is free while ionize runs: a brick plan builds the density
+                # bricks there and the casts sample them (the binary16-velocity form samples rho directly)
+                j.bricks = m.stage.data_ptr() if (m.ray_bricks is not False and not m.vel_state) else None
                 jobs.append(j)
             xyz = np.array([m.stars[i][:3] for i in on], dtype=np.float32).reshape(-1, 3)
             if len(on):
@@ -230,11 +234,28 @@ class SimBatch:
             d.smax_out = self.smax.data_ptr() + 4 * s
         self._plan_rays = None
         if jobs:
-            arr = (_lib.NvRayDesc * len(jobs))(*jobs)
+            def ray_plan(use_bricks):
+                descs = []
+                for j in jobs:
+                    d = _lib.NvRayDesc.from_buffer_copy(j)
+                    if not use_bricks:
+                        d.bricks = None
+                    descs.append(d)
+                arr = (_lib.NvRayDesc * len(descs))(*descs)
 
-            def build_rays(buf, nbytes, arr=arr, n=len(jobs)):
-                _lib.call("nv_cast_rays_batch_plan", ctypes.cast(arr, P), n, buf, nbytes)
-            self._plan_rays = _Plan("nv_cast_rays_batch_plan_size", build_rays, len(jobs), self.device)
+                def build_rays(buf, nbytes, arr=arr, n=len(descs)):
+                    _lib.call("nv_cast_rays_batch_plan", ctypes.cast(arr, P), n, buf, nbytes)
+                return _Plan("nv_cast_rays_batch_plan_size", build_rays, len(descs), self.device)
+
+            use = self.ray_bricks if any(j.bricks for j in jobs) else False
+            if use == "auto":
+                from . import autotune
+                plans = {False: ray_plan(False), True: ray_plan(True)}
+                use = autotune.tune_batch_rays(self, plans)
+                self._plan_rays = plans[use]
+            else:
+                self._plan_rays = ray_plan(bool(use))
+            self.ray_bricks_used = bool(use)
 
         def build_ion(buf, nbytes):
             _lib.call("nv_ionize_apply_batch_plan", ctypes.cast(ions, P), self.n, vel, buf, nbytes)

@@ -36,7 +36,8 @@ inline float pow2_reciprocal(float x) {
 }
 
 // ------------------------------------------------------------------------------------ device helpers
-__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+// v clamped to [lo, hi] (lo <= hi at every call); one v_med3_i32
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
 
 // C-order [nx][ny][nz] offset; 64-bit so 1024^3 grids (2^30 cells) index safely
 __device__ __forceinline__ long long cell(int i, int j, int k, int ny, int nz) {

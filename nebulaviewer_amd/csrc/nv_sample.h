@@ -20,7 +20,15 @@ struct ScalePow2 {
 };
 
 // Loads at a cell index: OFF = uint32_t forms 32-bit byte offsets from the field's base (grids up to 2^30
-// cells, so that 4 * index < 2^32), uint64_t 64-bit ones.
+// cells, so that 4 * index < 2^32, and with nx ny < 2^24), uint64_t 64-bit ones.
+//
+// The cell index (i ny + j) nz + k in 32 bits: on the uint32_t path the row i ny + j is below 2^24, so both
+// products are exact 24-bit multiply-adds (full rate; a 32-bit multiply issues at a quarter of it).
+template <class OFF>
+__device__ __forceinline__ unsigned cell_index(unsigned i, unsigned j, unsigned k, unsigned ny, unsigned nz) {
+  if constexpr (sizeof(OFF) == 4) return __umul24(__umul24(i, ny) + j, nz) + k;
+  else return (i * ny + j) * nz + k;
+}
 template <class OFF>
 __device__ __forceinline__ float ld1(const float* __restrict__ f, unsigned i) {
   return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(f) + (OFF)i * 4u);
@@ -44,12 +52,14 @@ __device__ __forceinline__ float sample_r(const float* __restrict__ f, int nx, i
   const float gx = cs(px) - 0.5f, gy = cs(py) - 0.5f, gz = cs(pz) - 0.5f;
   if (gx < -0.5f || gy < -0.5f || gz < -0.5f) return 0.0f;
   if (gx > (float)nx - 0.5f || gy > (float)ny - 0.5f || gz > (float)nz - 0.5f) return 0.0f;
-  const int i0 = (int)floorf(gx), j0 = (int)floorf(gy), k0 = (int)floorf(gz);
-  const float fx = gx - (float)i0, fy = gy - (float)j0, fz = gz - (float)k0;
+  // (float)(int)floor(g) is floor(g) itself (|g| < 2^24 here), so the fractions come from the float floors
+  const float flx = floorf(gx), fly = floorf(gy), flz = floorf(gz);
+  const int i0 = (int)flx, j0 = (int)fly, k0 = (int)flz;
+  const float fx = gx - flx, fy = gy - fly, fz = gz - flz;
   const unsigned ia = clampi(i0, 0, nx - 1), ib = clampi(i0 + 1, 0, nx - 1);
   const unsigned ja = clampi(j0, 0, ny - 1), jb = clampi(j0 + 1, 0, ny - 1);
-  const unsigned r00 = (ia * ny + ja) * nz, r10 = (ib * ny + ja) * nz;
-  const unsigned r01 = (ia * ny + jb) * nz, r11 = (ib * ny + jb) * nz;
+  const unsigned r00 = cell_index<OFF>(ia, ja, 0, ny, nz), r10 = cell_index<OFF>(ib, ja, 0, ny, nz);
+  const unsigned r01 = cell_index<OFF>(ia, jb, 0, ny, nz), r11 = cell_index<OFF>(ib, jb, 0, ny, nz);
   float c000, c100, c010, c110, c001, c101, c011, c111;
   if constexpr (PAIR) {
     const unsigned kp = clampi(k0, 0, nz - 2);
@@ -97,12 +107,13 @@ __device__ __forceinline__ float sample_brick(const float4* __restrict__ B, int 
   const float gx = cs(px) - 0.5f, gy = cs(py) - 0.5f, gz = cs(pz) - 0.5f;
   if (gx < -0.5f || gy < -0.5f || gz < -0.5f) return 0.0f;
   if (gx > (float)nx - 0.5f || gy > (float)ny - 0.5f || gz > (float)nz - 0.5f) return 0.0f;
-  const int i0 = (int)floorf(gx), j0 = (int)floorf(gy), k0 = (int)floorf(gz);
-  const float fx = gx - (float)i0, fy = gy - (float)j0, fz = gz - (float)k0;
+  const float flx = floorf(gx), fly = floorf(gy), flz = floorf(gz);
+  const int i0 = (int)flx, j0 = (int)fly, k0 = (int)flz;
+  const float fx = gx - flx, fy = gy - fly, fz = gz - flz;
   const unsigned ia = clampi(i0, 0, nx - 1), ib = clampi(i0 + 1, 0, nx - 1);
   const unsigned jp = clampi(j0, 0, ny - 1), kp = clampi(k0, 0, nz - 1);
   const bool jlow = j0 < 0, klow = k0 < 0;
-  const float4 a = ld4<OFF>(B, (ia * ny + jp) * nz + kp), b = ld4<OFF>(B, (ib * ny + jp) * nz + kp);
+  const float4 a = ld4<OFF>(B, cell_index<OFF>(ia, jp, kp, ny, nz)), b = ld4<OFF>(B, cell_index<OFF>(ib, jp, kp, ny, nz));
   const float c000 = a.x, c100 = b.x;
   const float c001 = klow ? a.x : a.y, c101 = klow ? b.x : b.y;
   const float c010 = jlow ? a.x : a.z, c110 = jlow ? b.x : b.z;

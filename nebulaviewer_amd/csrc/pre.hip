@@ -122,7 +122,8 @@ extern "C" int nv_drive(float* rho, float* mx, float* my, float* mz, int nx, int
   NV_REQUIRE(rho && mx && my && mz && fx && fy && fz && gx && gy && gz, "nv_drive: null pointer");
   NV_REQUIRE(nx > 0 && ny > 0 && nz > 0 && sx > 0 && sy > 0 && sz >= 2,
              "nv_drive: bad sizes (the stirring grid needs sz >= 2)");
-  NV_REQUIRE((long long)sx * sy * sz <= (1LL << 30), "nv_drive: stirring grid above 2^30 cells");
+  NV_REQUIRE((long long)sx * sy * sz <= (1LL << 30) && (long long)sx * sy < (1LL << 24),
+             "nv_drive: stirring grid above 2^30 cells or 2^24 rows");
   NV_REQUIRE(dx > 0.0f && dxs >= 1.17549435e-38f && dxs <= 3.40282347e+38f, "nv_drive: dx, dxs must be > 0");
   DriveArgs A{rho, mx, my, mz, fx, fy, fz, gx, gy, gz, nx, ny, nz, sx, sy, sz, blend, amp, dt, dx};
   const long long n = (long long)nx * ny * nz;

@@ -172,17 +172,52 @@ __global__ __launch_bounds__(256) void cast_rays_batch_kernel(const RayJob* __re
   }
   const RayJob& J = jobs[lo];
   const long long t = (long long)(b - start[lo]) * 256 + threadIdx.x;
-  if (J.inv != 0.0f) {
-    ScalePow2 sc;
-    sc.inv = J.inv;
-    if (J.wide) cast_one<PAIR, true, OFF, ScalePow2, false>(J, sc, t);
-    else cast_one<PAIR, false, OFF, ScalePow2, false>(J, sc, t);
-  } else {
-    ScaleDiv sc;
-    sc.rdx = J.rdx;
-    if (J.wide) cast_one<PAIR, true, OFF, ScaleDiv, false>(J, sc, t);
-    else cast_one<PAIR, false, OFF, ScaleDiv, false>(J, sc, t);
+  auto by_scale = [&](auto brick) {
+    constexpr bool BR = decltype(brick)::value;
+    if (J.inv != 0.0f) {
+      ScalePow2 sc;
+      sc.inv = J.inv;
+      if (J.wide) cast_one<PAIR, true, OFF, ScalePow2, BR>(J, sc, t);
+      else cast_one<PAIR, false, OFF, ScalePow2, BR>(J, sc, t);
+    } else {
+      ScaleDiv sc;
+      sc.rdx = J.rdx;
+      if (J.wide) cast_one<PAIR, true, OFF, ScaleDiv, BR>(J, sc, t);
+      else cast_one<PAIR, false, OFF, ScaleDiv, BR>(J, sc, t);
+    }
+  };
+  if (J.bricks) by_scale(std::true_type{});
+  else by_scale(std::false_type{});
+}
+
+// The brick arrays of several grids in one launch: block b belongs to brick job s with bstart[s] <= b <
+// bstart[s+1]; each job's blocks cover its (i, j) rows, ceil(nz / 256) blocks per row (ray_bricks_kernel's work).
+struct BrickJob {
+  const float* f;
+  float4* B;
+  int nx, ny, nz, kblocks;
+};
+
+__global__ __launch_bounds__(256) void ray_bricks_batch_kernel(const BrickJob* __restrict__ jobs,
+                                                               const int* __restrict__ bstart, int n, int total) {
+  const int b = blockIdx.x;
+  if (b >= total) return;
+  int lo = 0, hi = n;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (bstart[mid] <= b) lo = mid;
+    else hi = mid;
   }
+  const BrickJob& J = jobs[lo];
+  const int l = b - bstart[lo];
+  const int row = l / J.kblocks, k = (l - row * J.kblocks) * 256 + threadIdx.x;
+  if (k >= J.nz) return;
+  const int i = row / J.ny, j = row - i * J.ny;
+  const long long c = ((long long)i * J.ny + j) * J.nz + k;
+  const int dk = k + 1 < J.nz ? 1 : 0;
+  const long long dj = j + 1 < J.ny ? J.nz : 0;
+  const float* f = J.f;
+  J.B[c] = make_float4(f[c], f[c + dk], f[c + dj], f[c + dj + dk]);
 }
 
 // cell centre seen from a star: r, u = cos(theta), phi in [0, 2pi) (nebula3d.py:284-290)
@@ -532,9 +567,10 @@ inline int blocks_for(long long n, int bs) { return (int)((n + bs - 1) / bs); }
 using namespace nv;
 
 // Calls f(pair, offset, scale) with the sampler's compile-time forms for this grid: paired z loads when nz >= 2,
-// 32-bit byte offsets up to 2^30 cells, and the power-of-two or shared-reciprocal cell scaling.
+// 32-bit byte offsets (and 24-bit index products) up to 2^30 cells with nx ny < 2^24, and the power-of-two or
+// shared-reciprocal cell scaling. cells: the grid's cells times the element size in floats.
 template <class F>
-static void dispatch_sampler(int nz, long long cells, float dx, F&& f) {
+static void dispatch_sampler(int nz, long long cells, long long rows, float dx, F&& f) {
   const float inv = pow2_reciprocal(dx);
   ScaleDiv sd;
   sd.rdx = Recip{1.0 / (double)dx, dx};   // correctly rounded: |r dx - 1| <= 2^-53, inside the bound div_r needs
@@ -545,7 +581,7 @@ static void dispatch_sampler(int nz, long long cells, float dx, F&& f) {
     else f(pair, off, sd);
   };
   auto by_off = [&](auto pair) {
-    if (cells <= (1LL << 30)) by_scale(pair, uint32_t{});
+    if (cells <= (1LL << 30) && rows < (1LL << 24)) by_scale(pair, uint32_t{});
     else by_scale(pair, uint64_t{});
   };
   if (nz >= 2) by_off(std::true_type{});
@@ -578,14 +614,14 @@ static int cast_rays_launch(const char* name, bool wide, const float* rho, const
   const long long cells = (long long)nx * ny * nz;
   if (bricks) {
     // 16-byte elements: 32-bit byte offsets up to 2^28 cells
-    dispatch_sampler(2, cells * 4, dx, [&](auto, auto off, auto sc) {
+    dispatch_sampler(2, cells * 4, (long long)nx * ny, dx, [&](auto, auto off, auto sc) {
       using O = decltype(off);
       using SC = decltype(sc);
       auto kern = wide ? cast_rays_kernel<true, true, O, SC, true> : cast_rays_kernel<true, false, O, SC, true>;
       hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(256), 0, as_stream(stream), J, sc);
     });
   } else {
-    dispatch_sampler(nz, cells, dx, [&](auto pair, auto off, auto sc) {
+    dispatch_sampler(nz, cells, (long long)nx * ny, dx, [&](auto pair, auto off, auto sc) {
       constexpr bool P = decltype(pair)::value;
       using O = decltype(off);
       using SC = decltype(sc);
@@ -735,7 +771,7 @@ extern "C" int nv_cast_fuv(const float* rho, const float* xi, int nx, int ny, in
   NV_REQUIRE(dr > 0.0f && dx > 0.0f && binr > 0.0f, "nv_cast_fuv: dr, dx, binr must be > 0");
   NV_REQUIRE((long long)nx * ny * nz < (1LL << 32), "nv_cast_fuv: grids above 2^32 cells are not supported");
   const long long n = (long long)nu * nphi;
-  dispatch_sampler(nz, (long long)nx * ny * nz, dx, [&](auto pair, auto off, auto sc) {
+  dispatch_sampler(nz, (long long)nx * ny * nz, (long long)nx * ny, dx, [&](auto pair, auto off, auto sc) {
     constexpr bool P = decltype(pair)::value;
     hipLaunchKernelGGL((cast_fuv_kernel<P, decltype(off), decltype(sc)>), dim3(blocks_for(n, 256)), dim3(256), 0,
                        as_stream(stream), rho, xi, nx, ny, nz, sx, sy, sz, kappa, dr, nsteps, sc, binr, bx, by, bz,
@@ -771,12 +807,14 @@ extern "C" int nv_fuv_apply(const float* atts, int nr, int nu, int nphi, const f
 }
 
 // ---------------------------------------------------------------------------------------- batch plans
-// Plan layout (host blob, uploaded once): a 64-byte header, the jobs' kernel arguments, then start[0..n].
+// Plan layout (host blob, uploaded once): a 64-byte header, the jobs' kernel arguments, then start[0..n]; a ray
+// plan continues with its brick jobs (at most one per job) and their block prefix bstart[0..nb].
 namespace nv {
 struct RayBatchHeader {
   uint32_t magic;
   int n, total, pair, off32;
-  int pad[11];
+  int nb, btotal;          // ray plans: brick jobs and their blocks
+  int pad[9];
 };
 static_assert(sizeof(RayBatchHeader) == 64, "plan header layout");
 constexpr uint32_t kRayMagic = 0x4e565231u;   // "NVR1"
@@ -788,17 +826,23 @@ static long long batch_bytes(int n) {
 }
 }  // namespace nv
 
-extern "C" long long nv_cast_rays_batch_plan_size(int n) { return n > 0 ? batch_bytes<RayJob>(n) : -1; }
+static long long ray_plan_bytes(int n) {
+  return batch_bytes<RayJob>(n) + (long long)n * (long long)sizeof(BrickJob) + 4LL * (n + 1);
+}
+
+extern "C" long long nv_cast_rays_batch_plan_size(int n) { return n > 0 ? ray_plan_bytes(n) : -1; }
 
 extern "C" int nv_cast_rays_batch_plan(const NvRayDesc* jobs, int n, void* plan, long long plan_bytes) {
   clear_error();
   NV_REQUIRE(jobs && plan && n > 0, "nv_cast_rays_batch_plan: need n > 0 jobs and a plan buffer");
-  NV_REQUIRE(plan_bytes >= batch_bytes<RayJob>(n), "nv_cast_rays_batch_plan: plan needs %lld bytes",
-             batch_bytes<RayJob>(n));
+  NV_REQUIRE(plan_bytes >= ray_plan_bytes(n), "nv_cast_rays_batch_plan: plan needs %lld bytes", ray_plan_bytes(n));
   RayBatchHeader H = {};
   RayJob* out = reinterpret_cast<RayJob*>(static_cast<char*>(plan) + sizeof(H));
   int* start = reinterpret_cast<int*>(out + n);
-  long long total = 0;
+  BrickJob* bj = reinterpret_cast<BrickJob*>(start + n + 1);
+  int* bstart = reinterpret_cast<int*>(bj + n);
+  long long total = 0, btotal = 0;
+  int nb = 0;
   bool pair = true, off32 = true;
   for (int s = 0; s < n; ++s) {
     const NvRayDesc& d = jobs[s];
@@ -809,21 +853,53 @@ extern "C" int nv_cast_rays_batch_plan(const NvRayDesc* jobs, int n, void* plan,
     const long long cells = (long long)d.nx * d.ny * d.nz;
     NV_REQUIRE(cells < (1LL << 32), "nv_cast_rays_batch_plan: job %d: grids above 2^32 cells are not supported", s);
     pair = pair && d.nz >= 2;
-    off32 = off32 && cells <= (1LL << 30);
+    // 32-bit byte offsets: 4-byte elements up to 2^30 cells, 16-byte bricks up to 2^28
+    off32 = off32 && cells <= (d.bricks ? (1LL << 28) : (1LL << 30)) && (long long)d.nx * d.ny < (1LL << 24);
     RayJob J;
     fill_ray_job(d.rho, d.nx, d.ny, d.nz, d.sx, d.sy, d.sz, d.Q, d.dr, d.nsteps, d.dx, d.rcap, d.bx, d.by, d.bz,
                  d.rfront, d.nu, d.nphi, d.wide != 0, J);
+    if (d.bricks) {
+      const char* r = reinterpret_cast<const char*>(d.rho);
+      const char* b = reinterpret_cast<const char*>(d.bricks);
+      NV_REQUIRE(((uintptr_t)d.bricks & 15) == 0, "nv_cast_rays_batch_plan: job %d: bricks must be 16-byte aligned", s);
+      NV_REQUIRE(b + 16 * cells <= r || r + 4 * cells <= b, "nv_cast_rays_batch_plan: job %d: bricks overlap rho", s);
+      NV_REQUIRE(d.ny <= 65535 && d.nx <= 65535, "nv_cast_rays_batch_plan: job %d: bad sizes", s);
+      J.bricks = reinterpret_cast<const float4*>(d.bricks);
+      // one brick job per distinct brick array (the jobs of one grid's stars share it)
+      bool seen = false;
+      for (int q = 0; q < nb; ++q) {
+        if (bj[q].B == reinterpret_cast<float4*>(d.bricks)) {
+          NV_REQUIRE(bj[q].f == d.rho && bj[q].nx == d.nx && bj[q].ny == d.ny && bj[q].nz == d.nz,
+                     "nv_cast_rays_batch_plan: job %d: one brick array for two different grids", s);
+          seen = true;
+        }
+      }
+      if (!seen) {
+        BrickJob B;
+        B.f = d.rho;
+        B.B = reinterpret_cast<float4*>(d.bricks);
+        B.nx = d.nx; B.ny = d.ny; B.nz = d.nz;
+        B.kblocks = (d.nz + 255) / 256;
+        memcpy(&bj[nb], &B, sizeof(B));
+        bstart[nb++] = (int)btotal;
+        btotal += (long long)d.nx * d.ny * B.kblocks;
+        NV_REQUIRE(btotal < (1LL << 31) - 1, "nv_cast_rays_batch_plan: too many brick rows");
+      }
+    }
     memcpy(&out[s], &J, sizeof(J));
     start[s] = (int)total;
     total += ray_blocks(d.nu, d.nphi);
     NV_REQUIRE(total < (1LL << 31) - 1, "nv_cast_rays_batch_plan: too many rays");
   }
   start[n] = (int)total;
+  bstart[nb] = (int)btotal;
   H.magic = kRayMagic;
   H.n = n;
   H.total = (int)total;
   H.pair = pair ? 1 : 0;
   H.off32 = off32 ? 1 : 0;
+  H.nb = nb;
+  H.btotal = (int)btotal;
   memcpy(plan, &H, sizeof(H));
   return NV_OK;
 }
@@ -836,6 +912,12 @@ extern "C" int nv_cast_rays_batch(const void* plan_host, const void* plan_dev, v
   NV_REQUIRE(H.magic == kRayMagic && H.n > 0, "nv_cast_rays_batch: not a ray batch plan");
   const RayJob* jobs = reinterpret_cast<const RayJob*>(static_cast<const char*>(plan_dev) + sizeof(H));
   const int* start = reinterpret_cast<const int*>(jobs + H.n);
+  if (H.nb > 0) {
+    const BrickJob* bj = reinterpret_cast<const BrickJob*>(start + H.n + 1);
+    const int* bstart = reinterpret_cast<const int*>(bj + H.n);
+    hipLaunchKernelGGL(ray_bricks_batch_kernel, dim3(H.btotal), dim3(256), 0, as_stream(stream), bj, bstart, H.nb,
+                       H.btotal);
+  }
   auto kern = H.pair ? (H.off32 ? cast_rays_batch_kernel<true, uint32_t> : cast_rays_batch_kernel<true, uint64_t>)
                      : (H.off32 ? cast_rays_batch_kernel<false, uint32_t> : cast_rays_batch_kernel<false, uint64_t>);
   hipLaunchKernelGGL(kern, dim3(H.total), dim3(256), 0, as_stream(stream), jobs, start, H.n, H.total);

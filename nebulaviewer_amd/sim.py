@@ -19,6 +19,10 @@ precision: "fp32" reproduces the reference's float32 step bit for bit; "fp32-fas
 hardware reciprocals / square roots in the RK stage; "fp16-vel" stores float32 density with binary16 velocity
 (58 instead of 88 bytes of state traffic per cell and step) and then mx, my, mz are computed on access.
 
+tile, ray_bricks: launch choices that change no bit: the RK stage's workgroup tile ("tune": measured on the first
+stage, or an NV_TILE_* code) and whether the ray casts sample the density through its brick array ("auto":
+measured on the first ionization, True, False); nebulaviewer_amd.autotune.
+
 long_rays: "fp64" (default) accumulates the recombination column of a ray in FP64 when the ray's chord through
 the box exceeds 512 cells, where float summation error moves the Stromgren front (nv_cast_rays_wide); shorter
 rays, and every ray of grids up to 294 cells on a side, are bit-identical to the float column. "fp32" keeps the
@@ -48,7 +52,7 @@ class Sim:
 
     def __init__(self, scenario="torus", res=None, params=None, rays=None, fuv_grid=None, ion_every=None,
                  device="cuda:0", verbose=True, fast=False, setup=None, fused=True, seed=None, precision="fp32",
-                 long_rays="fp64", tile="tune"):
+                 long_rays="fp64", tile="tune", ray_bricks="auto"):
         if precision not in self.PRECISIONS:
             raise ValueError(f"precision must be one of {self.PRECISIONS}")
         if long_rays not in ("fp64", "fp32"):
@@ -58,7 +62,11 @@ class Sim:
         self.precision = precision
         self.long_rays = long_rays
         self.tile = tile
-        self.ray_bricks = True      # sample the density from its brick array in the ray casts (same bits)
+        # ray casts sample the density through its brick array (True), directly (False) or by measurement ("auto");
+        # the fronts are the same bits either way
+        if ray_bricks not in ("auto", True, False):
+            raise ValueError("ray_bricks must be 'auto', True or False")
+        self.ray_bricks = ray_bricks
         self.device = torch.device(device)
         self.fast = fast
         self.verbose = verbose
@@ -352,23 +360,34 @@ class Sim:
         strm = self._stream
         on = [i for i, s in enumerate(self.stars) if self.t >= s[4]]
         Lx, Ly, Lz = self.bmax
-        # the casts sample the density from its brick array (two 16-byte loads per sample), built in the float32
-        # RK stage block, which is free while ionize runs; the binary16-velocity form has no such block and
-        # samples the density field directly. Both give the same fronts bit for bit.
-        bricks = None
-        if self.ray_bricks and on and not self.vel_state:
-            bricks = self._p(self.stage)
-            tk = self._t0("ray_bricks")
-            _lib.call("nv_ray_bricks", self._p(self.U[0]), nx, ny, nz, bricks, strm)
+        # The casts sample the density either directly or from its brick array (two 16-byte loads per sample),
+        # built in the float32 RK stage block, which is free while ionize runs; the same fronts bit for bit.
+        # ray_bricks="auto" measures both on the first ionization (nebulaviewer_amd.autotune); the
+        # binary16-velocity form has no float32 stage block and samples directly.
+        def casts(use_bricks, timed=False):
+            bricks = None
+            if use_bricks:
+                bricks = self._p(self.stage)
+                tk = self._t0("ray_bricks") if timed else None
+                _lib.call("nv_ray_bricks", self._p(self.U[0]), nx, ny, nz, bricks, strm)
+                self._t1(tk)
+            tk = self._t0("cast_rays") if timed else None
+            for slot, i in enumerate(on):
+                sx, sy, sz = self.stars[i][:3]
+                _lib.call("nv_cast_rays_ex", self._p(self.U[0]), bricks, nx, ny, nz,
+                          _f(sx), _f(sy), _f(sz), _f(self.q_eff(i)),
+                          _f(self.dr), self.nsteps, _f(self.dx), _f(self.diag), _f(Lx), _f(Ly), _f(Lz),
+                          self._p(self.rfronts[slot]), nu, nphi, 1 if self.long_rays == "fp64" else 0, strm)
             self._t1(tk)
-        tk = self._t0("cast_rays")
-        for slot, i in enumerate(on):
-            sx, sy, sz = self.stars[i][:3]
-            _lib.call("nv_cast_rays_ex", self._p(self.U[0]), bricks, nx, ny, nz,
-                      _f(sx), _f(sy), _f(sz), _f(self.q_eff(i)),
-                      _f(self.dr), self.nsteps, _f(self.dx), _f(self.diag), _f(Lx), _f(Ly), _f(Lz),
-                      self._p(self.rfronts[slot]), nu, nphi, 1 if self.long_rays == "fp64" else 0, strm)
-        self._t1(tk)
+
+        use = self.ray_bricks
+        if self.vel_state or not on:
+            use = False
+        elif use == "auto":
+            from . import autotune
+            use = autotune.tune_rays(self, len(on), casts)
+        self.ray_bricks_used = bool(use)
+        casts(bool(use), timed=True)
         width = 1.2 * self.dx
         half = self.dfront.dtype == torch.float16
         if self.fused:

@@ -19,6 +19,7 @@ int main(void) {
   OFF(NvRayDesc, sy); OFF(NvRayDesc, sz); OFF(NvRayDesc, Q); OFF(NvRayDesc, dr); OFF(NvRayDesc, nsteps);
   OFF(NvRayDesc, dx); OFF(NvRayDesc, rcap); OFF(NvRayDesc, bx); OFF(NvRayDesc, by); OFF(NvRayDesc, bz);
   OFF(NvRayDesc, rfront); OFF(NvRayDesc, nu); OFF(NvRayDesc, nphi); OFF(NvRayDesc, wide);
+  OFF(NvRayDesc, bricks);
   printf("NvIonDesc %zu\n", sizeof(NvIonDesc));
   OFF(NvIonDesc, rfronts); OFF(NvIonDesc, nu); OFF(NvIonDesc, nphi); OFF(NvIonDesc, star_xyz);
   OFF(NvIonDesc, nstars); OFF(NvIonDesc, dx); OFF(NvIonDesc, width); OFF(NvIonDesc, xi); OFF(NvIonDesc, dfront_h);

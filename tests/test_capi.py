@@ -40,7 +40,7 @@ def test_every_declared_symbol_is_exported_and_bound(lib):
 
 
 def test_abi_version(lib):
-    assert lib.nv_abi_version() == 1
+    assert lib.nv_abi_version() == 2
 
 
 def struct_fields(name):
@@ -113,6 +113,17 @@ def test_batch_plans_validate(lib):
     rays = (_lib.NvRayDesc * 1)()
     rbuf = ctypes.create_string_buffer(int(lib.nv_cast_rays_batch_plan_size(1)))
     assert lib.nv_cast_rays_batch_plan(rays, 1, rbuf, len(rbuf)) == 1001
+    # a ray job with a brick array: aligned, not overlapping its density field
+    r = rays[0]
+    r.rho, r.rfront = 4096, 1 << 20
+    r.nx = r.ny = r.nz = 8
+    r.nu, r.nphi, r.dr, r.dx = 4, 4, 0.1, 0.1
+    r.bricks = 4096 + 1024
+    assert lib.nv_cast_rays_batch_plan(rays, 1, rbuf, len(rbuf)) == 1001 and b"overlap" in lib.nv_last_error()
+    r.bricks = (1 << 16) + 8
+    assert lib.nv_cast_rays_batch_plan(rays, 1, rbuf, len(rbuf)) == 1001 and b"aligned" in lib.nv_last_error()
+    r.bricks = 1 << 16
+    assert lib.nv_cast_rays_batch_plan(rays, 1, rbuf, len(rbuf)) == 0
     ions = (_lib.NvIonDesc * 1)()
     ibuf = ctypes.create_string_buffer(int(lib.nv_ionize_apply_batch_plan_size(1)))
     assert lib.nv_ionize_apply_batch_plan(ions, 1, 0, ibuf, len(ibuf)) == 1001

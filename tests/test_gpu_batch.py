@@ -118,3 +118,16 @@ def test_autotuned_tiles_change_no_bit(torch_dev):
     b.step(3)
     assert b.tile in (1, 2, 3)
     check_members(b, 3, "fp32")
+
+
+def test_batch_mixed_brick_jobs_match_single(torch_dev):
+    """Ray jobs with and without density bricks in one batched cast (member 1 samples rho directly; the sheets'
+    four stars share their member's brick array, built once): every member equals its single Sim."""
+    from nebulaviewer_amd.batch import SimBatch
+    say("[batch] mixed brick / direct ray jobs")
+    b = SimBatch(setups(), rays=RAYS)
+    b.ray_bricks = True
+    b.members[1].ray_bricks = False
+    b.step(3)
+    assert b.ray_bricks_used
+    check_members(b, 3, "fp32")

@@ -467,13 +467,13 @@ def test_fuv_bitwise(env, okern32, shape):
 
 
 # ------------------------------------------------------------------ the whole Sim step against OracleSim
-def run_pair(env, scenario, n, steps, rays, fuv, fused=True, params=None):
+def run_pair(env, scenario, n, steps, rays, fuv, fused=True, params=None, ray_bricks="auto"):
     from nebulaviewer_amd import scenarios as S
     from nebulaviewer_amd.sim import Sim
     from oracle.oracle import OracleSim
-    say(f"[sim] {scenario} n={n} steps={steps} rays={rays} fused={fused}")
+    say(f"[sim] {scenario} n={n} steps={steps} rays={rays} fused={fused} ray_bricks={ray_bricks}")
     setup = S.make_setup(scenario, n, params=params)
-    sim = Sim(setup=setup, rays=rays, fuv_grid=fuv, verbose=False, fused=fused)
+    sim = Sim(setup=setup, rays=rays, fuv_grid=fuv, verbose=False, fused=fused, ray_bricks=ray_bricks)
     ref = OracleSim(S.make_setup(scenario, n, params=params), S.constants(setup), rays, fuv, np.float32, threads=16)
     sim.step(steps)
     ref.step(steps)
@@ -487,13 +487,15 @@ def assert_same_state(sim, ref):
     np.testing.assert_array_equal(N(sim.dfront).view(np.uint16), ref.dfront.view(np.uint16), err_msg="dfront")
 
 
-@pytest.mark.parametrize("scenario,n,steps,rays,fuv", [
-    ("torus", 40, 12, (64, 128), (24, 32, 64)),
-    ("sheet", 32, 9, (32, 64), (16, 16, 32)),
-    ("wall", 48, 8, (64, 128), (24, 32, 64)),
+@pytest.mark.parametrize("scenario,n,steps,rays,fuv,bricks", [
+    ("torus", 40, 12, (64, 128), (24, 32, 64), True),
+    ("sheet", 32, 9, (32, 64), (16, 16, 32), False),
+    ("wall", 48, 8, (64, 128), (24, 32, 64), True),
+    ("wall", 48, 8, (64, 128), (24, 32, 64), False),
 ])
-def test_sim_matches_oracle(env, scenario, n, steps, rays, fuv):
-    sim, ref = run_pair(env, scenario, n, steps, rays, fuv)
+def test_sim_matches_oracle(env, scenario, n, steps, rays, fuv, bricks):
+    sim, ref = run_pair(env, scenario, n, steps, rays, fuv, ray_bricks=bricks)
+    assert sim.ray_bricks_used == bricks
     assert_same_state(sim, ref)
     sim.update_render_fields(force=True)
     ref.fuv_field()
@@ -512,7 +514,7 @@ def test_sim_jet_and_rusanov(env):
 
 def test_sim_sheet_256_matches_oracle(env):
     """The config-1 grid (256^3, 4 stars, wind, Mach-5 velocity), 2 steps with one ionization, bitwise."""
-    sim, ref = run_pair(env, "sheet", 256, 2, (512, 1024), (160, 160, 320))
+    sim, ref = run_pair(env, "sheet", 256, 2, (512, 1024), (160, 160, 320), ray_bricks=True)
     assert_same_state(sim, ref)
 
 
