@@ -209,9 +209,9 @@ class SimBatch:
                 j.rfront = m.rfronts[slot].data_ptr()
                 j.nu, j.nphi = nu, nphi
                 j.wide = 1 if m.long_rays == "fp64" else 0
-                # the member's float32 RK stage block is free while ionize runs: a brick plan builds the density
-                # bricks there and the casts sample them (the binary16-velocity form samples rho directly)
-                j.bricks = m.stage.data_ptr() if (m.ray_bricks is not False and not m.vel_state) else None
+                # the member's 16 B/cell stage buffer is free while ionize runs: a brick plan builds the density
+                # bricks there and the casts sample them
+                j.bricks = m._brick_buf.data_ptr() if m.ray_bricks is not False else None
                 jobs.append(j)
             xyz = np.array([m.stars[i][:3] for i in on], dtype=np.float32).reshape(-1, 3)
             if len(on):

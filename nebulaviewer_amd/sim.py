@@ -145,7 +145,7 @@ class Sim:
     # ------------------------------------------------------------------ setup
     def reset(self, scenario=None, res=None, params=None, setup=None):
         t_setup = time.perf_counter()
-        for k in ("U", "xi", "dfront", "g0", "stage", "rfronts", "atts"):
+        for k in ("U", "xi", "dfront", "g0", "stage", "_brick_buf", "rfronts", "atts"):
             self.__dict__.pop(k, None)
         if setup is None:
             setup = S.make_setup(scenario or "torus", res, self._seed, params)
@@ -181,11 +181,17 @@ class Sim:
             r = rho.clamp_min(P["floor"])
             self.U = [rho] + [(c / r).half() for c in m]
             del r, m
-            h16 = dict(dtype=torch.float16, device=dev)
-            self.stage = [torch.zeros(self.shape, **f32)] + [torch.zeros(self.shape, **h16) for _ in range(3)]
+            # the stage block (float32 rho, binary16 velocity: 10 B/cell) is carved from one 16 B/cell buffer, which
+            # also holds the density bricks of the ray casts while ionize runs (the stage is free then)
+            n = math.prod(self.shape)
+            self._brick_buf = torch.zeros(4 * n, **f32)
+            halves = self._brick_buf.view(torch.float16)
+            self.stage = [self._brick_buf[:n].view(self.shape)] + [halves[(2 + c) * n:(3 + c) * n].view(self.shape)
+                                                                    for c in range(3)]
         else:
             self.U = [rho] + m
             self.stage = torch.zeros((4,) + self.shape, **f32)
+            self._brick_buf = self.stage
         self.xi = torch.zeros(self.shape, **f32)
         hdt = torch.float32 if P.get("fp32_fields") else torch.float16
         self.dfront = torch.full(self.shape, 1e3, dtype=hdt, device=dev)
@@ -361,13 +367,12 @@ class Sim:
         on = [i for i, s in enumerate(self.stars) if self.t >= s[4]]
         Lx, Ly, Lz = self.bmax
         # The casts sample the density either directly or from its brick array (two 16-byte loads per sample),
-        # built in the float32 RK stage block, which is free while ionize runs; the same fronts bit for bit.
-        # ray_bricks="auto" measures both on the first ionization (nebulaviewer_amd.autotune); the
-        # binary16-velocity form has no float32 stage block and samples directly.
+        # built in the 16 B/cell buffer of the RK stage block, which is free while ionize runs; the same fronts bit
+        # for bit. ray_bricks="auto" measures both on the first ionization (nebulaviewer_amd.autotune).
         def casts(use_bricks, timed=False):
             bricks = None
             if use_bricks:
-                bricks = self._p(self.stage)
+                bricks = self._p(self._brick_buf)
                 tk = self._t0("ray_bricks") if timed else None
                 _lib.call("nv_ray_bricks", self._p(self.U[0]), nx, ny, nz, bricks, strm)
                 self._t1(tk)
@@ -381,7 +386,7 @@ class Sim:
             self._t1(tk)
 
         use = self.ray_bricks
-        if self.vel_state or not on:
+        if not on:
             use = False
         elif use == "auto":
             from . import autotune
