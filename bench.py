@@ -188,7 +188,8 @@ def make_measure_batch(a, S, KernelTimer):
                  hydro_only_cell_updates_per_s=cells / (2 * hyd_s),
                  hydro_achieved_GBps=BYTES_PER_CELL_STEP["fp32"] / 2 * cells / hyd_s / 1e9,
                  kernels_ms={k: round(v["avg_ms"], 4) for k, v in ks.items()}, graph_replay=True,
-                 tile={1: "64x8", 2: "32x16", 3: "64x16"}.get(b.tile), ray_bricks=getattr(b, "ray_bricks_used", None))
+                 tile={1: "64x8", 2: "32x16", 3: "64x16"}.get(b.tile), ray_bricks=getattr(b, "ray_bricks_used", None),
+                 dispatch={f"stage{k}": ("queue" if v else "hardware") for k, v in b.dispatch.items()})
         log(f"[bench] batch: {r['value'] / 1e9:.2f} G cell-updates/s ({cells / 1e6:.1f} M cells, {r['stars']} stars), "
             f"{r['ms_per_step']:.3f} ms/step, hydro {hyd_s * 1e3:.3f} ms/launch; kernels {r['kernels_ms']}")
         del b
@@ -200,6 +201,7 @@ def make_measure_batch(a, S, KernelTimer):
 
 def batch_only(a):
     """--config 4: the uneven batch as the measured workload (one JSON line of its own)"""
+    from nebulaviewer_amd import autotune
     from nebulaviewer_amd import scenarios as S
     from nebulaviewer_amd.timing import KernelTimer
     r = make_measure_batch(a, S, KernelTimer)(a.batch or 64)
@@ -215,6 +217,7 @@ def batch_only(a):
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": r["hydro_achieved_GBps"] / HBM_PEAK_GBS,
                      "traffic": None, "algorithmic_bytes_per_launch": BYTES_PER_CELL_STEP["fp32"] / 2 * r["cells"]},
         "hydro_only_cell_updates_per_s": r["hydro_only_cell_updates_per_s"], "kernels_ms": r["kernels_ms"],
+        "tile": r["tile"], "ray_bricks": r["ray_bricks"], "dispatch": r["dispatch"], "autotune": autotune.report(),
         "variants": {}, "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)

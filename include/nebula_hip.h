@@ -152,6 +152,11 @@ long long nv_hydro_stage_batch_plan_size(int n);
 int nv_hydro_stage_batch_plan(const NvStageDesc* members, int n, int precision, int solver, int lim, int tile,
                               void* plan, long long plan_bytes);
 int nv_hydro_stage_batch(const void* plan_host, const void* plan_dev, void* stream);
+/* The same stage from the same plan, as a persistent launch: as many workgroups as the device holds at once, fed by
+ * a tile work queue in the plan's device copy (per-XCD runs of tiles, claimed by atomic adds, stolen across XCDs
+ * when a run is exhausted; the last workgroup resets the counters). Bit-identical to nv_hydro_stage_batch; two
+ * launches of one plan must not overlap (one stream). No reference counterpart (the reference has no batching). */
+int nv_hydro_stage_batch_queue(const void* plan_host, const void* plan_dev, void* stream);
 
 /* Mass budget in FP64 on the GPU (the north star's global conservation sums): out[0] += sum of rho over the grid,
  * out[1] += the net outward mass flux through the six domain faces, each face's float flux (the one the exact

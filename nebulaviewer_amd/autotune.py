@@ -34,7 +34,8 @@ _cache = {}
 
 
 NAMES = {"sim": TILES.get, "batch": TILES.get, "rays": {False: "direct", True: "bricks"}.get,
-         "batch_rays": {False: "direct", True: "bricks"}.get}
+         "batch_rays": {False: "direct", True: "bricks"}.get,
+         "batch_dispatch": {False: "hardware", True: "queue"}.get}
 
 
 def report():
@@ -60,10 +61,15 @@ def _time(launch, stream_dev, reps, warm=2):
     return s.elapsed_time(e) / reps
 
 
-def _pick(key, measure, candidates):
+def _pick(key, measure, candidates, rounds=2):
+    """the candidate with the least mean time; candidates are measured in interleaved rounds, so that a drift of
+    the device's speed over the first launches of a process (clocks, first touches) does not favour the later ones"""
     if key in _cache:
         return _cache[key]["choice"]
-    ms = {t: measure(t) for t in candidates}
+    ms = {t: 0.0 for t in candidates}
+    for _ in range(rounds):
+        for t in candidates:
+            ms[t] += measure(t) / rounds
     best = min(ms, key=ms.get)
     _cache[key] = {"choice": best, "ms": ms}
     return best
@@ -129,3 +135,35 @@ def tune_batch_rays(batch, plans, reps=2):
         plan = plans[use]
         return _time(lambda: _lib.call("nv_cast_rays_batch", *plan.args, batch._stream), batch.device, reps)
     return _pick(key, measure, (False, True))
+
+
+def tune_batch_dispatch(batch, pulses, reps=3):
+    """{1: use_queue, 2: use_queue}: for each stage form, whether the persistent tile-queue launch
+    (nv_hydro_stage_batch_queue) beats one workgroup per tile. Timed on plans whose outputs go to a scratch buffer
+    (16 B per cell and member, freed afterwards), so the state is untouched; the queue form exists for HLL +
+    minmod only."""
+    if (batch.solver, batch.lim) != (1, 0):
+        return {1: False, 2: False}
+    shapes = tuple(tuple(m.shape) for m in batch.members)
+    out = {}
+    bufs = [torch.empty(4 * m.shape[0] * m.shape[1] * m.shape[2], dtype=torch.float32, device=batch.device)
+            for m in batch.members]
+    smax = torch.zeros(batch.n, dtype=torch.float32, device=batch.device)
+    scratch = []
+    for b, m in zip(bufs, batch.members):
+        n = b.numel() // 4
+        base = b.data_ptr()
+        # float32 fields at 4 B, binary16 velocities at 2 B per cell, packed in the 16 B/cell buffer
+        scratch.append([base, base + 4 * n, base + 8 * n, base + 12 * n] if not m.vel_state else
+                       [base, base + 4 * n, base + 6 * n, base + 8 * n])
+    for stage in (1, 2):
+        key = ("batch_dispatch", shapes, batch.precision, batch.tile, stage)
+        plan = batch._stage_plan(stage, pulses, scratch=scratch, smax=smax.data_ptr())
+
+        def measure(queue, plan=plan):
+            entry = "nv_hydro_stage_batch_queue" if queue else "nv_hydro_stage_batch"
+            return _time(lambda: _lib.call(entry, *plan.args, batch._stream), batch.device, reps)
+        out[stage] = _pick(key, measure, (False, True))
+    torch.cuda.synchronize(batch.device)
+    del bufs
+    return out

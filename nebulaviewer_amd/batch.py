@@ -47,7 +47,7 @@ class _Plan:
 
 class SimBatch:
     def __init__(self, setups, precision="fp32", device="cuda:0", rays=None, ion_every=None, graph=False,
-                 verbose=False, tile="tune"):
+                 verbose=False, tile="tune", dispatch="tune"):
         if not setups:
             raise ValueError("a batch needs at least one member")
         self.device = torch.device(device)
@@ -57,6 +57,11 @@ class SimBatch:
         self.tune = tile == "tune"
         self.tile = 0 if self.tune else tile
         self.ray_bricks = "auto"  # ray casts through density bricks: True, False or measured ("auto")
+        # how the stage's tiles reach the CUs, per stage form: "hardware" (one workgroup per tile), "queue" (a
+        # persistent grid fed by a tile work queue, HLL + minmod only) or measured ("tune")
+        if dispatch not in ("tune", "hardware", "queue"):
+            raise ValueError("dispatch must be 'tune', 'hardware' or 'queue'")
+        self.dispatch = dispatch if dispatch == "tune" else {1: dispatch == "queue", 2: dispatch == "queue"}
         self.members = [Sim(setup=st, verbose=verbose, precision=precision, device=device, rays=rays,
                             ion_every=ion_every) for st in setups]
         m0 = self.members[0]
@@ -135,50 +140,58 @@ class SimBatch:
         return max(0.0, 1.0 + m.P["jet_pulse"] * math.sin(2 * math.pi * m.t / m.P["jet_period"]))
 
     # ------------------------------------------------------------------ plans
+    def _stage_plan(self, stage, pulses, scratch=None, smax=None):
+        """the plan of RK stage 1 (U -> U1) or stage 2 (U1, U0 = U -> U, with the fused CFL maxima into smax);
+        scratch: per member, four output pointers that take the result instead (the dispatch tuner's timing runs:
+        stage 2 then reads U with U0 = U1, the same traffic and arithmetic on values of the state's own range)"""
+        P = _lib.P
+        code = PRECISION_CODES[self.precision]
+        descs = (_lib.NvStageDesc * self.n)()
+        for s, m in enumerate(self.members):
+            d = descs[s]
+            U = [t.data_ptr() for t in m.U]
+            U1 = [m.stage[c].data_ptr() for c in range(4)]
+            src, u0, dst = (U, U, U1) if stage == 1 else (U1, U, U)
+            if scratch is not None:
+                src, u0, dst = (U, U, scratch[s]) if stage == 1 else (U, U1, scratch[s])
+            for c in range(4):
+                d.u[c] = src[c]
+                d.u0[c] = u0[c]
+                d.out[c] = dst[c]
+            d.xi = m.xi.data_ptr()
+            d.stars = m.star_arr.data_ptr()
+            d.jets = m.jet_arr.data_ptr()
+            d.smax_out = (smax + 4 * s) if stage == 2 else None
+            d.dt_dev = self.dt_f.data_ptr() + 4 * s
+            d.nx, d.ny, d.nz = m.shape
+            d.nstars = len(m.stars)
+            d.wind = m.P["wind"]
+            d.jet_cos = m.C["jet_cos"]
+            d.jet_r = m.P["jet_r"]
+            d.pulse = pulses[s]
+            d.dt = 0.0
+            d.dx = m.dx
+            d.a, d.b = (0.0, 1.0) if stage == 1 else (0.5, 0.5)
+            d.floor_ = m.P["floor"]
+            d.cn2, d.ci2, d.vmax = m.cn2, m.ci2, 6.0
+
+        def build(buf, nbytes):
+            _lib.call("nv_hydro_stage_batch_plan", ctypes.cast(descs, P), self.n, code, self.solver, self.lim,
+                      self.tile, buf, nbytes)
+        return _Plan("nv_hydro_stage_batch_plan_size", build, self.n, self.device)
+
     def _stage_plans(self, pulses):
         if self.tune:
             from . import autotune
             self.tune = False
             self.tile = autotune.tune_batch(self, pulses)
+        if self.dispatch == "tune":
+            from . import autotune
+            self.dispatch = autotune.tune_batch_dispatch(self, pulses)
         key = tuple(pulses)
         if key == self._stage_key:
             return
-        P = _lib.P
-        code = PRECISION_CODES[self.precision]
-        plans = []
-        for stage in (1, 2):
-            descs = (_lib.NvStageDesc * self.n)()
-            for s, m in enumerate(self.members):
-                d = descs[s]
-                U = [t.data_ptr() for t in m.U]
-                U1 = [m.stage[c].data_ptr() for c in range(4)]
-                src, dst = (U, U1) if stage == 1 else (U1, U)
-                for c in range(4):
-                    d.u[c] = src[c]
-                    d.u0[c] = U[c]
-                    d.out[c] = dst[c]
-                d.xi = m.xi.data_ptr()
-                d.stars = m.star_arr.data_ptr()
-                d.jets = m.jet_arr.data_ptr()
-                d.smax_out = (self.smax.data_ptr() + 4 * s) if stage == 2 else None
-                d.dt_dev = self.dt_f.data_ptr() + 4 * s
-                d.nx, d.ny, d.nz = m.shape
-                d.nstars = len(m.stars)
-                d.wind = m.P["wind"]
-                d.jet_cos = m.C["jet_cos"]
-                d.jet_r = m.P["jet_r"]
-                d.pulse = pulses[s]
-                d.dt = 0.0
-                d.dx = m.dx
-                d.a, d.b = (0.0, 1.0) if stage == 1 else (0.5, 0.5)
-                d.floor_ = m.P["floor"]
-                d.cn2, d.ci2, d.vmax = m.cn2, m.ci2, 6.0
-
-            def build(buf, nbytes, descs=descs):
-                _lib.call("nv_hydro_stage_batch_plan", ctypes.cast(descs, P), self.n, code, self.solver, self.lim,
-                          self.tile, buf, nbytes)
-            plans.append(_Plan("nv_hydro_stage_batch_plan_size", build, self.n, self.device))
-        self._plans_stage = plans
+        self._plans_stage = [self._stage_plan(1, pulses), self._stage_plan(2, pulses, smax=self.smax.data_ptr())]
         self._stage_key = key
         self._graphs.clear()
 
@@ -277,12 +290,11 @@ class SimBatch:
 
     def _launch_hydro(self):
         strm = self._stream
-        tk = self._t0("hydro_stage")
-        _lib.call("nv_hydro_stage_batch", *self._plans_stage[0].args, strm)
-        self._t1(tk)
-        tk = self._t0("hydro_stage")
-        _lib.call("nv_hydro_stage_batch", *self._plans_stage[1].args, strm)
-        self._t1(tk)
+        for stage in (1, 2):
+            entry = "nv_hydro_stage_batch_queue" if self.dispatch[stage] else "nv_hydro_stage_batch"
+            tk = self._t0("hydro_stage")
+            _lib.call(entry, *self._plans_stage[stage - 1].args, strm)
+            self._t1(tk)
 
     def _device_step(self, ion):
         strm = self._stream

@@ -618,19 +618,68 @@ __global__ __launch_bounds__(TL::NT, 4) void hydro_stage_kernel(HydroArgs A) {
 
 // A batch of independent grids in one launch: tile t of the flat list belongs to the member s with
 // start[s] <= t < start[s+1] (start[0] = 0, start[n] = total), whose arguments are args[s].
-template <class TL, class ST, class AR, int SOLVER, int LIM, bool SMAX, bool STAGE1>
-__global__ __launch_bounds__(TL::NT, 4) void hydro_stage_batch_kernel(const HydroArgs* __restrict__ args,
-                                                                  const int* __restrict__ start, int n, int total,
-                                                                  int per_xcd) {
-  const int t = xcd_tile(per_xcd);
-  if (t >= total) return;
+__device__ __forceinline__ int member_of(const int* __restrict__ start, int n, int t) {
   int lo = 0, hi = n;
   while (hi - lo > 1) {
     const int mid = (lo + hi) >> 1;
     if (start[mid] <= t) lo = mid;
     else hi = mid;
   }
+  return lo;
+}
+
+// One workgroup per tile, dealt by the hardware dispatcher.
+template <class TL, class ST, class AR, int SOLVER, int LIM, bool SMAX, bool STAGE1>
+__global__ __launch_bounds__(TL::NT, 4) void hydro_stage_batch_kernel(const HydroArgs* __restrict__ args,
+                                                                  const int* __restrict__ start, int n, int total,
+                                                                  int per_xcd) {
+  const int t = xcd_tile(per_xcd);
+  if (t >= total) return;
+  const int lo = member_of(start, n, t);
   stage_tile<TL, ST, AR, SOLVER, LIM, SMAX, STAGE1>(args[lo], t - start[lo]);
+}
+
+// The persistent form: a grid of resident workgroups fed by a tile work queue. queue[x], x < 8, counts the tiles
+// handed out of XCD x's contiguous run [x per_xcd, (x + 1) per_xcd); a workgroup claims from its own XCD's run first
+// and then steals from the others in turn, so tiles stay XCD-local while they last and no CU idles while another
+// XCD still has work. Every claim is one atomic add, so each tile is claimed once, and a workgroup exits after
+// finding all eight runs exhausted (at most eight adds per claim). queue[8] counts the workgroups that have exited;
+// the last one resets all nine counters for the next launch of the plan (launches of one plan must not overlap).
+// Looping over tiles keeps more values live than one tile per workgroup does (the compiler hoists the tile body's
+// invariants), so this form spills a few registers; which form is faster is measured (SimBatch dispatch="tune").
+template <class TL, class ST, class AR, int SOLVER, int LIM, bool SMAX, bool STAGE1>
+__global__ __launch_bounds__(TL::NT, 4) void hydro_stage_queue_kernel(const HydroArgs* __restrict__ args,
+                                                                  const int* __restrict__ start, int n, int total,
+                                                                  int per_xcd, int* __restrict__ queue) {
+  __shared__ int claimed;
+  const bool lead = threadIdx.x == 0 && threadIdx.y == 0;
+  const int home = blockIdx.x & 7;
+  for (;;) {
+    if (lead) {
+      int t = -1;
+      for (int q = 0; q < 8 && t < 0; ++q) {
+        const int x = (home + q) & 7;
+        const int base = x * per_xcd, cnt = min(per_xcd, total - base);
+        if (cnt <= 0) continue;
+        const int c = atomicAdd(&queue[x], 1);
+        if (c < cnt) t = base + c;
+      }
+      claimed = t;
+    }
+    __syncthreads();
+    const int t = claimed;
+    __syncthreads();                  // every thread has read the claim before the lead writes the next one
+    if (t < 0) break;
+    const int lo = member_of(start, n, t);
+    stage_tile<TL, ST, AR, SOLVER, LIM, SMAX, STAGE1>(args[lo], t - start[lo]);
+  }
+  if (lead) {
+    __threadfence();
+    if (atomicAdd(&queue[8], 1) == (int)gridDim.x - 1) {
+      for (int x = 0; x < 9; ++x) atomicExch(&queue[x], 0);
+      __threadfence();
+    }
+  }
 }
 
 // ------------------------------------------------------------------------------------------ conservation sums
@@ -881,6 +930,7 @@ static int launch_stage(Mode mode, const char* name, const void* f0, const void*
 // ------------------------------------------------------------------------------------------ batch plans
 // A plan is one host-built blob, uploaded once and used by every launch: header, kernel arguments of each
 // member, then the tile prefix start[0..n] (start[n] = total tiles).
+// The tile queue's nine counters (zero in the host blob; the persistent launch leaves them zero) follow start[].
 struct BatchHeader {
   uint32_t magic;
   int n, mode, solver, lim, smax, stage1, total, per_xcd, tile;
@@ -888,12 +938,31 @@ struct BatchHeader {
 };
 static_assert(sizeof(BatchHeader) == 64, "plan header layout");
 constexpr uint32_t kBatchMagic = 0x4e564231u;   // "NVB1"
+constexpr int kQueueInts = 16;                   // nine counters, padded
 
 static long long plan_bytes(int n) {
-  return (long long)sizeof(BatchHeader) + (long long)n * (long long)sizeof(HydroArgs) + 4LL * (n + 1);
+  return (long long)sizeof(BatchHeader) + (long long)n * (long long)sizeof(HydroArgs) + 4LL * (n + 1) +
+         4LL * kQueueInts;
 }
 
 typedef void (*BatchKernel)(const HydroArgs*, const int*, int, int, int);
+typedef void (*QueueKernel)(const HydroArgs*, const int*, int, int, int, int*);
+
+// the persistent form is built for HLL + minmod (the benchmark workloads' scheme), both stage forms
+template <class TL, class ST, class AR>
+static QueueKernel pick_queue_st(bool sm, bool stage1) {
+  if (sm) return stage1 ? hydro_stage_queue_kernel<TL, ST, AR, NV_SOLVER_HLL, NV_LIM_MINMOD, true, true>
+                        : hydro_stage_queue_kernel<TL, ST, AR, NV_SOLVER_HLL, NV_LIM_MINMOD, true, false>;
+  return stage1 ? hydro_stage_queue_kernel<TL, ST, AR, NV_SOLVER_HLL, NV_LIM_MINMOD, false, true>
+                : hydro_stage_queue_kernel<TL, ST, AR, NV_SOLVER_HLL, NV_LIM_MINMOD, false, false>;
+}
+
+template <class TL>
+static QueueKernel pick_queue_tl(int mode, bool sm, bool stage1) {
+  return mode == NV_PREC_F32        ? pick_queue_st<TL, F32, Exact>(sm, stage1)
+         : mode == NV_PREC_F32_FAST ? pick_queue_st<TL, F32, Fast>(sm, stage1)
+                                    : pick_queue_st<TL, F16V, Fast>(sm, stage1);
+}
 
 template <class TL, class ST, class AR, bool SMAX, bool STAGE1>
 static BatchKernel pick_batch_lim(int solver, int lim) {
@@ -1018,6 +1087,7 @@ extern "C" int nv_hydro_stage_batch_plan(const NvStageDesc* members, int n, int 
     memcpy(&args[s], &A, sizeof(A));
   }
   start[n] = (int)total;
+  memset(start + n + 1, 0, 4 * kQueueInts);
   H.magic = kBatchMagic;
   H.n = n;
   H.mode = precision;
@@ -1032,22 +1102,53 @@ extern "C" int nv_hydro_stage_batch_plan(const NvStageDesc* members, int n, int 
   return NV_OK;
 }
 
-extern "C" int nv_hydro_stage_batch(const void* plan_host, const void* plan_dev, void* stream) {
-  using namespace nv;
+namespace nv {
+// Hardware-dispatched (one workgroup per tile) or persistent (a queue-fed grid of resident workgroups) launch.
+static int batch_launch(const char* name, const void* plan_host, const void* plan_dev, bool persistent, void* stream) {
   clear_error();
-  NV_REQUIRE(plan_host && plan_dev, "nv_hydro_stage_batch: null plan");
+  NV_REQUIRE(plan_host && plan_dev, "%s: null plan", name);
   BatchHeader H;
   memcpy(&H, plan_host, sizeof(H));
-  NV_REQUIRE(H.magic == kBatchMagic && H.n > 0, "nv_hydro_stage_batch: not a stage batch plan");
+  NV_REQUIRE(H.magic == kBatchMagic && H.n > 0, "%s: not a stage batch plan", name);
   const HydroArgs* args = reinterpret_cast<const HydroArgs*>(static_cast<const char*>(plan_dev) + sizeof(BatchHeader));
   const int* start = reinterpret_cast<const int*>(args + H.n);
   const bool sm = H.smax != 0, st1 = H.stage1 != 0;
-  BatchKernel kern = H.tile == TILE_32x16   ? pick_batch_tl<T32>(H.mode, H.solver, H.lim, sm, st1)
-                     : H.tile == TILE_64x16 ? pick_batch_tl<T6416>(H.mode, H.solver, H.lim, sm, st1)
-                                            : pick_batch_tl<T64>(H.mode, H.solver, H.lim, sm, st1);
-  hipLaunchKernelGGL(kern, dim3(8 * H.per_xcd), dim3(tile_tk(H.tile), tile_tj(H.tile), 1), 0, as_stream(stream), args,
-                     start, H.n, H.total, H.per_xcd);
-  return check_launch("nv_hydro_stage_batch");
+  const dim3 block(tile_tk(H.tile), tile_tj(H.tile), 1);
+  if (!persistent) {
+    BatchKernel kern = H.tile == TILE_32x16   ? pick_batch_tl<T32>(H.mode, H.solver, H.lim, sm, st1)
+                       : H.tile == TILE_64x16 ? pick_batch_tl<T6416>(H.mode, H.solver, H.lim, sm, st1)
+                                              : pick_batch_tl<T64>(H.mode, H.solver, H.lim, sm, st1);
+    hipLaunchKernelGGL(kern, dim3(8 * H.per_xcd), block, 0, as_stream(stream), args, start, H.n, H.total, H.per_xcd);
+    return check_launch(name);
+  }
+  NV_REQUIRE(H.solver == NV_SOLVER_HLL && H.lim == NV_LIM_MINMOD, "%s: the persistent form is built for HLL + minmod",
+             name);
+  int* queue = const_cast<int*>(start + H.n + 1);
+  QueueKernel kern = H.tile == TILE_32x16   ? pick_queue_tl<T32>(H.mode, sm, st1)
+                     : H.tile == TILE_64x16 ? pick_queue_tl<T6416>(H.mode, sm, st1)
+                                            : pick_queue_tl<T64>(H.mode, sm, st1);
+  // as many workgroups as the device holds at once (a multiple of 8, one share per XCD), never more than tiles
+  int dev = 0, cus = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) !=
+      hipSuccess || hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, (int)(block.x * block.y), 0) !=
+      hipSuccess) {
+    set_error("%s: cannot query the device's occupancy", name);
+    return NV_ERR_LAUNCH;
+  }
+  const long long resident = std::max<long long>(8, ((long long)cus * std::max(per_cu, 1)) / 8 * 8);
+  const long long grid = std::min<long long>(resident, ((long long)H.total + 7) / 8 * 8);
+  hipLaunchKernelGGL(kern, dim3((unsigned)grid), block, 0, as_stream(stream), args, start, H.n, H.total, H.per_xcd,
+                     queue);
+  return check_launch(name);
+}
+}  // namespace nv
+
+extern "C" int nv_hydro_stage_batch(const void* plan_host, const void* plan_dev, void* stream) {
+  return nv::batch_launch("nv_hydro_stage_batch", plan_host, plan_dev, false, stream);
+}
+
+extern "C" int nv_hydro_stage_batch_queue(const void* plan_host, const void* plan_dev, void* stream) {
+  return nv::batch_launch("nv_hydro_stage_batch_queue", plan_host, plan_dev, true, stream);
 }
 
 extern "C" int nv_conservation_sums(const float* rho, const float* mx, const float* my, const float* mz,

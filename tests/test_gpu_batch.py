@@ -131,3 +131,27 @@ def test_batch_mixed_brick_jobs_match_single(torch_dev):
     b.step(3)
     assert b.ray_bricks_used
     check_members(b, 3, "fp32")
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp16-vel"])
+def test_batch_queue_dispatch_matches_single(torch_dev, precision):
+    """The persistent tile-queue launch of both stages (nv_hydro_stage_batch_queue) against each member stepped
+    alone, over several steps and two kinds of captured graph (the queue's counters reset after every launch)."""
+    from nebulaviewer_amd.batch import SimBatch
+    say(f"[batch] persistent tile-queue dispatch, {precision}")
+    b = SimBatch(setups(), precision=precision, rays=RAYS, dispatch="queue", graph=True)
+    assert b.dispatch == {1: True, 2: True}
+    b.step(6)
+    check_members(b, 6, precision)
+
+
+def test_batch_dispatch_is_tuned(torch_dev):
+    from nebulaviewer_amd.batch import SimBatch
+    from nebulaviewer_amd import autotune
+    say("[batch] dispatch tuning")
+    b = SimBatch(setups(), rays=RAYS)
+    b.step(2)
+    assert set(b.dispatch) == {1, 2}
+    keys = [k for k in autotune.report() if "batch_dispatch" in k]
+    assert len(keys) >= 2
+    check_members(b, 2, "fp32")
