@@ -56,8 +56,9 @@ typedef struct NvStageDesc {
 } NvStageDesc;
 
 /* One ray-cast job of a batch: the arguments of nv_cast_rays (wide != 0: those of nv_cast_rays_wide). bricks:
- * null, or a 16-byte aligned scratch of 16 bytes per cell that nv_cast_rays_batch fills with rho's brick array
- * (as nv_ray_bricks) before the casts sample it; jobs on one grid pass the same rho and bricks, built once. */
+ * null, or a 16-byte aligned scratch of nv_ray_bricks_size(nx, ny, nz) bytes that nv_cast_rays_batch fills with
+ * rho's brick array (as nv_ray_bricks) before the casts sample it; jobs on one grid pass the same rho and bricks,
+ * built once. */
 typedef struct NvRayDesc {
   const float* rho;
   int nx, ny, nz;
@@ -200,10 +201,14 @@ int nv_cast_rays_wide(const float* rho, int nx, int ny, int nz, float sx, float 
                       int nsteps, float dx, float rcap, float bx, float by, float bz, float* rfront, int nu,
                       int nphi, void* stream);
 
-/* The brick array of a density field for nv_cast_rays_ex: bricks[i][j][k] = (rho(i,j,k), rho(i,j,k+1),
- * rho(i,j+1,k), rho(i,j+1,k+1)) as 4 floats (16 bytes per cell, 16-byte aligned; the +1 neighbours clamped to
- * the grid). A caller with a free 16 B/cell scratch block (Sim's RK stage block is free while ionize runs) builds
- * it once per ionization. No reference counterpart: it is the sampler's data layout, not a reference step. */
+/* The brick array of a density field for nv_cast_rays_ex, with one ghost layer: for p in [0, nx+2), q in
+ * [0, ny+1), r in [0, nz+1), bricks[p][q][r] = (rho(i, ja, ka), rho(i, ja, kb), rho(i, jb, ka), rho(i, jb, kb)) as
+ * 4 floats, i = clamp(p-1), ja = clamp(q-1), jb = clamp(q), ka = clamp(r-1), kb = clamp(r) (clamped to the grid):
+ * the eight corners of any trilinear sample are the bricks at (i0+1, j0+1, k0+1) and (i0+2, j0+1, k0+1), with no
+ * clamping in the sampler. nv_ray_bricks_size(nx, ny, nz) bytes, 16-byte aligned (about 16 B per cell: Sim's RK
+ * stage block, which is free while ionize runs, is allocated that large). Built once per ionization. No reference
+ * counterpart: it is the sampler's data layout, not a reference step. */
+long long nv_ray_bricks_size(int nx, int ny, int nz);
 int nv_ray_bricks(const float* rho, int nx, int ny, int nz, float* bricks, void* stream);
 
 /* nv_cast_rays (wide == 0) or nv_cast_rays_wide (wide != 0), sampling the density from its brick array when

@@ -26,6 +26,7 @@ SIGNATURES = {
     "nv_max_speed": [P] * 5 + [I, I, I, F, F, P, P],
     "nv_cast_rays": [P, I, I, I, F, F, F, F, F, I, F, F, F, F, F, P, I, I, P],
     "nv_cast_rays_wide": [P, I, I, I, F, F, F, F, F, I, F, F, F, F, F, P, I, I, P],
+    "nv_ray_bricks_size": [I, I, I],
     "nv_ray_bricks": [P, I, I, I, P, P],
     "nv_cast_rays_ex": [P, P, I, I, I, F, F, F, F, F, I, F, F, F, F, F, P, I, I, I, P],
     "nv_apply_front": [P, I, I, F, F, F, F, F, P, P, I, I, I, P],
@@ -62,7 +63,7 @@ SIGNATURES = {
 }
 
 RESTYPES = {"nv_last_error": ctypes.c_char_p, "nv_hydro_stage_batch_plan_size": LL,
-            "nv_cast_rays_batch_plan_size": LL, "nv_ionize_apply_batch_plan_size": LL}
+            "nv_cast_rays_batch_plan_size": LL, "nv_ionize_apply_batch_plan_size": LL, "nv_ray_bricks_size": LL}
 
 
 class NvStageDesc(ctypes.Structure):
@@ -112,7 +113,7 @@ def load(path=LIB_PATH):
         fn = getattr(lib, name)
         fn.argtypes = args
         fn.restype = RESTYPES.get(name, ctypes.c_int)
-    if lib.nv_abi_version() != 2:
+    if lib.nv_abi_version() != 3:
         raise NebulaHipError("libnebula_hip.so ABI version mismatch")
     _lib = lib
     return lib

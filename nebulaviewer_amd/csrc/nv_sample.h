@@ -89,13 +89,12 @@ __device__ __forceinline__ float sample_r(const float* __restrict__ f, int nx, i
   return c0 + (c1 - c0) * fz;
 }
 
-// The same trilinear sample from a brick array: B[i][j][k] = (f(i,j,k), f(i,j,k+1), f(i,j+1,k), f(i,j+1,k+1)) with
-// the +1 neighbours clamped to the grid (nv_ray_bricks builds it). A sample's eight corners are then two
-// 16-byte loads, the bricks at (ia, jp, kp) and (ib, jp, kp) with jp = clamp(j0), kp = clamp(k0), instead of four
-// paired loads: half the load instructions, each touching no more cache lines. The clamped corners of sample_r
-// follow: jb = clamp(j0 + 1) is the brick's second row, except for j0 = -1, where jb = ja = 0 (the brick's first
-// row); j0 = ny - 1 gives jb = ny - 1, which the build already clamped into the second row. Likewise in k. Every
-// corner is the value sample_r reads and the interpolation is the same expression, so the result is identical.
+// The same trilinear sample from the ghost-layered brick array that nv_ray_bricks builds: brick (p, q, r) holds
+// (f(i, ja, ka), f(i, ja, kb), f(i, jb, ka), f(i, jb, kb)) with i = clamp(p-1), ja = clamp(q-1), jb = clamp(q),
+// ka = clamp(r-1), kb = clamp(r). A sample inside [-1/2, n-1/2] has i0, j0, k0 in [-1, n-1], and its eight corners
+// (ia = clamp(i0), ib = clamp(i0+1), likewise j and k: what sample_r reads) are the bricks at (i0+1, j0+1, k0+1)
+// and (i0+2, j0+1, k0+1): two 16-byte loads, no clamping and no selects. Every corner is the value sample_r reads
+// and the interpolation is the same expression, so the result is identical.
 template <class OFF>
 __device__ __forceinline__ float4 ld4(const float4* __restrict__ B, unsigned i) {
   return *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(B) + (OFF)i * 16u);
@@ -110,14 +109,11 @@ __device__ __forceinline__ float sample_brick(const float4* __restrict__ B, int 
   const float flx = floorf(gx), fly = floorf(gy), flz = floorf(gz);
   const int i0 = (int)flx, j0 = (int)fly, k0 = (int)flz;
   const float fx = gx - flx, fy = gy - fly, fz = gz - flz;
-  const unsigned ia = clampi(i0, 0, nx - 1), ib = clampi(i0 + 1, 0, nx - 1);
-  const unsigned jp = clampi(j0, 0, ny - 1), kp = clampi(k0, 0, nz - 1);
-  const bool jlow = j0 < 0, klow = k0 < 0;
-  const float4 a = ld4<OFF>(B, cell_index<OFF>(ia, jp, kp, ny, nz)), b = ld4<OFF>(B, cell_index<OFF>(ib, jp, kp, ny, nz));
-  const float c000 = a.x, c100 = b.x;
-  const float c001 = klow ? a.x : a.y, c101 = klow ? b.x : b.y;
-  const float c010 = jlow ? a.x : a.z, c110 = jlow ? b.x : b.z;
-  const float c011 = jlow ? c001 : (klow ? a.z : a.w), c111 = jlow ? c101 : (klow ? b.z : b.w);
+  const unsigned qy = ny + 1, qz = nz + 1;
+  const unsigned ia = cell_index<OFF>(i0 + 1, j0 + 1, k0 + 1, qy, qz);
+  const float4 a = ld4<OFF>(B, ia), b = ld4<OFF>(B, ia + qy * qz);
+  const float c000 = a.x, c100 = b.x, c001 = a.y, c101 = b.y;
+  const float c010 = a.z, c110 = b.z, c011 = a.w, c111 = b.w;
   const float c00 = c000 + (c100 - c000) * fx;
   const float c10 = c010 + (c110 - c010) * fx;
   const float c01 = c001 + (c101 - c001) * fx;

@@ -139,25 +139,22 @@ __global__ __launch_bounds__(256) void cast_rays_kernel(RayJob J, SC cs) {
   cast_one<PAIR, WIDE, OFF, SC, BRICK>(J, cs, (long long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
-// The brick array of a density field for the two-load sampler (sample_brick): B[i][j][k] = (f(i,j,k), f(i,j,k+1),
-// f(i,j+1,k), f(i,j+1,k+1)), the +1 neighbours clamped to the grid. One thread per cell of a z row; the grid is
-// (z chunks, y, x), so neighbouring rows run side by side on all XCDs. (Dealing runs of rows to one XCD, so that
-// row j+1 is fetched into one L2 only, halves the reads but took 0.60 ms against 0.49 ms at 512^3: the kernel
-// already moves its measured traffic at 6.8 TB/s.)
-__device__ __forceinline__ void brick_cell(const float* __restrict__ f, float4* __restrict__ B, int ny, int nz,
-                                           long long row, int j, int k) {
-  const long long c = row * nz + k;
-  const int dk = k + 1 < nz ? 1 : 0;
-  const long long dj = j + 1 < ny ? nz : 0;
-  B[c] = make_float4(f[c], f[c + dk], f[c + dj], f[c + dj + dk]);
+// The ghost-layered brick array of a density field for sample_brick: brick (p, q, r), p in [0, nx+2), q in
+// [0, ny+1), r in [0, nz+1), holds (f(i, ja, ka), f(i, ja, kb), f(i, jb, ka), f(i, jb, kb)) with i = clamp(p-1),
+// ja = clamp(q-1), jb = clamp(q), ka = clamp(r-1), kb = clamp(r). One thread per brick of a padded z row; the grid
+// is (z chunks, padded y, padded x), so neighbouring rows run side by side on all XCDs.
+__device__ __forceinline__ void brick_cell(const float* __restrict__ f, float4* __restrict__ B, int nx, int ny, int nz,
+                                           int p, int q, int r) {
+  const long long i = clampi(p - 1, 0, nx - 1);
+  const long long ra = (i * ny + clampi(q - 1, 0, ny - 1)) * nz, rb = (i * ny + clampi(q, 0, ny - 1)) * nz;
+  const int ka = clampi(r - 1, 0, nz - 1), kb = clampi(r, 0, nz - 1);
+  B[((long long)p * (ny + 1) + q) * (nz + 1) + r] = make_float4(f[ra + ka], f[ra + kb], f[rb + ka], f[rb + kb]);
 }
 
-__global__ __launch_bounds__(256) void ray_bricks_kernel(const float* __restrict__ f, float4* __restrict__ B, int ny,
-                                                         int nz) {
-  const int k = blockIdx.x * 256 + threadIdx.x;
-  if (k >= nz) return;
-  const int j = blockIdx.y;
-  brick_cell(f, B, ny, nz, (long long)blockIdx.z * ny + j, j, k);
+__global__ __launch_bounds__(256) void ray_bricks_kernel(const float* __restrict__ f, float4* __restrict__ B, int nx,
+                                                         int ny, int nz) {
+  // one padded row (nz + 1 bricks) per block: no block is left with a single brick of a row's tail
+  for (int r = threadIdx.x; r <= nz; r += 256) brick_cell(f, B, nx, ny, nz, blockIdx.y, blockIdx.x, r);
 }
 
 // blocks of 256 rays each job needs (64-ray direction patches, nebula ray_of_thread)
@@ -199,11 +196,11 @@ __global__ __launch_bounds__(256) void cast_rays_batch_kernel(const RayJob* __re
 }
 
 // The brick arrays of several grids in one launch: block b belongs to brick job s with bstart[s] <= b <
-// bstart[s+1]; each job's blocks cover its (i, j) rows, ceil(nz / 256) blocks per row (ray_bricks_kernel's work).
+// bstart[s+1]; each job's blocks are its padded rows, (nx + 2)(ny + 1) of them (ray_bricks_kernel's work).
 struct BrickJob {
   const float* f;
   float4* B;
-  int nx, ny, nz, kblocks;
+  int nx, ny, nz, pad;
 };
 
 __global__ __launch_bounds__(256) void ray_bricks_batch_kernel(const BrickJob* __restrict__ jobs,
@@ -217,10 +214,9 @@ __global__ __launch_bounds__(256) void ray_bricks_batch_kernel(const BrickJob* _
     else hi = mid;
   }
   const BrickJob& J = jobs[lo];
-  const int l = b - bstart[lo];
-  const int row = l / J.kblocks, k = (l - row * J.kblocks) * 256 + threadIdx.x;
-  if (k >= J.nz) return;
-  brick_cell(J.f, J.B, J.ny, J.nz, row, row % J.ny, k);
+  const int row = b - bstart[lo];   // row = p (ny + 1) + q, one padded row per block
+  const int p = row / (J.ny + 1), q = row - p * (J.ny + 1);
+  for (int r = threadIdx.x; r <= J.nz; r += 256) brick_cell(J.f, J.B, J.nx, J.ny, J.nz, p, q, r);
 }
 
 // cell centre seen from a star: r, u = cos(theta), phi in [0, 2pi) (nebula3d.py:284-290)
@@ -601,6 +597,8 @@ static void fill_ray_job(const float* rho, int nx, int ny, int nz, float sx, flo
   J.inv = pow2_reciprocal(dx);
 }
 
+extern "C" long long nv_ray_bricks_size(int nx, int ny, int nz);
+
 static int cast_rays_launch(const char* name, bool wide, const float* rho, const float* bricks, int nx, int ny,
                             int nz, float sx, float sy, float sz, float Q, float dr, int nsteps, float dx, float rcap,
                             float bx, float by, float bz, float* rfront, int nu, int nphi, void* stream) {
@@ -610,14 +608,16 @@ static int cast_rays_launch(const char* name, bool wide, const float* rho, const
   NV_REQUIRE(dr > 0.0f && dx > 0.0f, "%s: dr and dx must be > 0", name);
   NV_REQUIRE((long long)nx * ny * nz < (1LL << 32), "%s: grids above 2^32 cells are not supported", name);
   NV_REQUIRE(!bricks || ((uintptr_t)bricks & 15) == 0, "%s: the brick array must be 16-byte aligned", name);
+  NV_REQUIRE(!bricks || nv_ray_bricks_size(nx, ny, nz) / 16 < (1LL << 32), "%s: brick grid above 2^32 bricks", name);
   RayJob J;
   fill_ray_job(rho, nx, ny, nz, sx, sy, sz, Q, dr, nsteps, dx, rcap, bx, by, bz, rfront, nu, nphi, wide, J);
   J.bricks = reinterpret_cast<const float4*>(bricks);
   const long long blocks = ray_blocks(nu, nphi);
   const long long cells = (long long)nx * ny * nz;
   if (bricks) {
-    // 16-byte elements: 32-bit byte offsets up to 2^28 cells
-    dispatch_sampler(2, cells * 4, (long long)nx * ny, dx, [&](auto, auto off, auto sc) {
+    // 16-byte bricks of the padded grid: 32-bit byte offsets up to 2^28 bricks, 24-bit products below 2^24 rows
+    const long long prow = (long long)(nx + 2) * (ny + 1);
+    dispatch_sampler(2, prow * (nz + 1) * 4, prow, dx, [&](auto, auto off, auto sc) {
       using O = decltype(off);
       using SC = decltype(sc);
       auto kern = wide ? cast_rays_kernel<true, true, O, SC, true> : cast_rays_kernel<true, false, O, SC, true>;
@@ -649,17 +649,23 @@ extern "C" int nv_cast_rays_wide(const float* rho, int nx, int ny, int nz, float
                           bx, by, bz, rfront, nu, nphi, stream);
 }
 
+extern "C" long long nv_ray_bricks_size(int nx, int ny, int nz) {
+  if (nx <= 0 || ny <= 0 || nz <= 0) return -1;
+  return 16LL * (nx + 2) * (ny + 1) * (nz + 1);
+}
+
 extern "C" int nv_ray_bricks(const float* rho, int nx, int ny, int nz, float* bricks, void* stream) {
   clear_error();
   NV_REQUIRE(rho && bricks, "nv_ray_bricks: null pointer");
   NV_REQUIRE(((uintptr_t)bricks & 15) == 0, "nv_ray_bricks: the brick array must be 16-byte aligned");
-  NV_REQUIRE(nx > 0 && ny > 0 && nz > 0 && nx <= 65535 && ny <= 65535, "nv_ray_bricks: bad sizes");
+  NV_REQUIRE(nx > 0 && ny > 0 && nz > 0 && nx <= 65533 && ny <= 65534, "nv_ray_bricks: bad sizes");
   const long long cells = (long long)nx * ny * nz;
   const char* r = reinterpret_cast<const char*>(rho);
   const char* b = reinterpret_cast<const char*>(bricks);
-  NV_REQUIRE(b + 16 * cells <= r || r + 4 * cells <= b, "nv_ray_bricks: the brick array overlaps the field");
-  hipLaunchKernelGGL(ray_bricks_kernel, dim3((unsigned)((nz + 255) / 256), (unsigned)ny, (unsigned)nx), dim3(256), 0,
-                     as_stream(stream), rho, reinterpret_cast<float4*>(bricks), ny, nz);
+  NV_REQUIRE(b + nv_ray_bricks_size(nx, ny, nz) <= r || r + 4 * cells <= b,
+             "nv_ray_bricks: the brick array overlaps the field");
+  hipLaunchKernelGGL(ray_bricks_kernel, dim3((unsigned)(ny + 1), (unsigned)(nx + 2)), dim3(256), 0, as_stream(stream),
+                     rho, reinterpret_cast<float4*>(bricks), nx, ny, nz);
   return check_launch("nv_ray_bricks");
 }
 
@@ -857,7 +863,12 @@ extern "C" int nv_cast_rays_batch_plan(const NvRayDesc* jobs, int n, void* plan,
     NV_REQUIRE(cells < (1LL << 32), "nv_cast_rays_batch_plan: job %d: grids above 2^32 cells are not supported", s);
     pair = pair && d.nz >= 2;
     // 32-bit byte offsets: 4-byte elements up to 2^30 cells, 16-byte bricks up to 2^28
-    off32 = off32 && cells <= (d.bricks ? (1LL << 28) : (1LL << 30)) && (long long)d.nx * d.ny < (1LL << 24);
+    if (d.bricks) {   // the padded brick grid: 32-bit offsets up to 2^28 bricks, 24-bit products below 2^24 rows
+      const long long prow = (long long)(d.nx + 2) * (d.ny + 1);
+      off32 = off32 && prow * (d.nz + 1) <= (1LL << 28) && prow < (1LL << 24);
+    } else {
+      off32 = off32 && cells <= (1LL << 30) && (long long)d.nx * d.ny < (1LL << 24);
+    }
     RayJob J;
     fill_ray_job(d.rho, d.nx, d.ny, d.nz, d.sx, d.sy, d.sz, d.Q, d.dr, d.nsteps, d.dx, d.rcap, d.bx, d.by, d.bz,
                  d.rfront, d.nu, d.nphi, d.wide != 0, J);
@@ -865,8 +876,10 @@ extern "C" int nv_cast_rays_batch_plan(const NvRayDesc* jobs, int n, void* plan,
       const char* r = reinterpret_cast<const char*>(d.rho);
       const char* b = reinterpret_cast<const char*>(d.bricks);
       NV_REQUIRE(((uintptr_t)d.bricks & 15) == 0, "nv_cast_rays_batch_plan: job %d: bricks must be 16-byte aligned", s);
-      NV_REQUIRE(b + 16 * cells <= r || r + 4 * cells <= b, "nv_cast_rays_batch_plan: job %d: bricks overlap rho", s);
-      NV_REQUIRE(d.ny <= 65535 && d.nx <= 65535, "nv_cast_rays_batch_plan: job %d: bad sizes", s);
+      NV_REQUIRE(b + nv_ray_bricks_size(d.nx, d.ny, d.nz) <= r || r + 4 * cells <= b,
+                 "nv_cast_rays_batch_plan: job %d: bricks overlap rho", s);
+      NV_REQUIRE(d.ny <= 65534 && d.nx <= 65533 && nv_ray_bricks_size(d.nx, d.ny, d.nz) / 16 < (1LL << 32),
+                 "nv_cast_rays_batch_plan: job %d: bad sizes", s);
       J.bricks = reinterpret_cast<const float4*>(d.bricks);
       // one brick job per distinct brick array (the jobs of one grid's stars share it)
       bool seen = false;
@@ -882,10 +895,10 @@ extern "C" int nv_cast_rays_batch_plan(const NvRayDesc* jobs, int n, void* plan,
         B.f = d.rho;
         B.B = reinterpret_cast<float4*>(d.bricks);
         B.nx = d.nx; B.ny = d.ny; B.nz = d.nz;
-        B.kblocks = (d.nz + 255) / 256;
+        B.pad = 0;
         memcpy(&bj[nb], &B, sizeof(B));
         bstart[nb++] = (int)btotal;
-        btotal += (long long)d.nx * d.ny * B.kblocks;
+        btotal += (long long)(d.nx + 2) * (d.ny + 1);
         NV_REQUIRE(btotal < (1LL << 31) - 1, "nv_cast_rays_batch_plan: too many brick rows");
       }
     }

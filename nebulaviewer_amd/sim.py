@@ -181,17 +181,18 @@ class Sim:
             r = rho.clamp_min(P["floor"])
             self.U = [rho] + [(c / r).half() for c in m]
             del r, m
-            # the stage block (float32 rho, binary16 velocity: 10 B/cell) is carved from one 16 B/cell buffer, which
-            # also holds the density bricks of the ray casts while ionize runs (the stage is free then)
+            # the stage block (float32 rho, binary16 velocity: 10 B/cell) is carved from the buffer that also holds
+            # the density bricks of the ray casts while ionize runs (the stage is free then)
             n = math.prod(self.shape)
-            self._brick_buf = torch.zeros(4 * n, **f32)
+            self._brick_buf = torch.zeros(self._brick_floats(), **f32)
             halves = self._brick_buf.view(torch.float16)
             self.stage = [self._brick_buf[:n].view(self.shape)] + [halves[(2 + c) * n:(3 + c) * n].view(self.shape)
                                                                     for c in range(3)]
         else:
             self.U = [rho] + m
-            self.stage = torch.zeros((4,) + self.shape, **f32)
-            self._brick_buf = self.stage
+            # the float32 stage block, at the head of the (slightly larger) buffer of the ray casts' density bricks
+            self._brick_buf = torch.zeros(self._brick_floats(), **f32)
+            self.stage = self._brick_buf[:4 * math.prod(self.shape)].view((4,) + self.shape)
         self.xi = torch.zeros(self.shape, **f32)
         hdt = torch.float32 if P.get("fp32_fields") else torch.float16
         self.dfront = torch.full(self.shape, 1e3, dtype=hdt, device=dev)
@@ -492,6 +493,12 @@ class Sim:
         smax = float(self.smax.item())
         self._smax_ok = False
         return self.P["cfl"] * self.dx / smax
+
+    def _brick_floats(self):
+        """floats of the buffer that holds the RK stage block and, while ionize runs, the density bricks (16 B per
+        cell plus one ghost layer: nv_ray_bricks_size)"""
+        nx, ny, nz = self.shape
+        return max(4 * nx * ny * nz, int(_lib.load().nv_ray_bricks_size(nx, ny, nz)) // 4)
 
     def _solver_lim(self):
         solver = S.SOLVERS[self.P.get("solver", "rusanov")]

@@ -40,7 +40,7 @@ def test_every_declared_symbol_is_exported_and_bound(lib):
 
 
 def test_abi_version(lib):
-    assert lib.nv_abi_version() == 2
+    assert lib.nv_abi_version() == 3
 
 
 def struct_fields(name):
@@ -163,6 +163,7 @@ def test_rejects_bad_enums_and_sizes(lib):
     assert rc == 1001 and b"aligned" in lib.nv_last_error()
     rc = lib.nv_ray_bricks(ctypes.c_void_p(4096), 8, 8, 8, ctypes.c_void_p(4096 + 16 * 64), None)
     assert rc == 1001 and b"overlaps" in lib.nv_last_error()
+    assert lib.nv_ray_bricks_size(8, 8, 8) == 16 * 10 * 9 * 9 and lib.nv_ray_bricks_size(0, 8, 8) == -1
     rc = lib.nv_cast_rays_ex(p, ctypes.c_void_p(24), 8, 8, 8, f(0), f(0), f(0), f(1), f(0.1), 10, f(0.1), f(1), f(1),
                              f(1), f(1), p, 4, 4, 0, None)
     assert rc == 1001 and b"aligned" in lib.nv_last_error()

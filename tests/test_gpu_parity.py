@@ -289,10 +289,14 @@ def test_cast_rays_sampler_forms_bitwise(env, okern32, shape, dx):
 
 
 def bricks_numpy(rho):
-    """(rho(i,j,k), rho(i,j,k+1), rho(i,j+1,k), rho(i,j+1,k+1)), the +1 neighbours clamped"""
-    jn = np.minimum(np.arange(rho.shape[1]) + 1, rho.shape[1] - 1)
-    kn = np.minimum(np.arange(rho.shape[2]) + 1, rho.shape[2] - 1)
-    return np.stack([rho, rho[:, :, kn], rho[:, jn, :], rho[:, jn][:, :, kn]], axis=-1)
+    """the ghost-layered bricks: brick (p, q, r) = (rho(i, ja, ka), rho(i, ja, kb), rho(i, jb, ka), rho(i, jb, kb)),
+    i = clamp(p-1), ja = clamp(q-1), jb = clamp(q), ka = clamp(r-1), kb = clamp(r); shape (nx+2, ny+1, nz+1, 4)"""
+    nx, ny, nz = rho.shape
+    i = np.clip(np.arange(nx + 2) - 1, 0, nx - 1)
+    ja, jb = np.clip(np.arange(ny + 1) - 1, 0, ny - 1), np.clip(np.arange(ny + 1), 0, ny - 1)
+    ka, kb = np.clip(np.arange(nz + 1) - 1, 0, nz - 1), np.clip(np.arange(nz + 1), 0, nz - 1)
+    r = rho[i]
+    return np.stack([r[:, ja][:, :, ka], r[:, ja][:, :, kb], r[:, jb][:, :, ka], r[:, jb][:, :, kb]], axis=-1)
 
 
 @pytest.mark.parametrize("shape,dx", [((40, 36, 44), 1 / 44), ((32, 16, 64), 1 / 64), ((20, 24, 1), 1 / 24),
@@ -305,7 +309,9 @@ def test_cast_rays_bricks_bitwise(env, okern32, shape, dx):
     torch, dev, L = env
     rho = np.random.default_rng(41).lognormal(0, 1.0, shape).astype(F32)
     grho = T(env, rho)
-    gb = torch.full(shape + (4,), -1.0, dtype=torch.float32, device=dev)
+    pshape = (shape[0] + 2, shape[1] + 1, shape[2] + 1, 4)
+    assert L.load().nv_ray_bricks_size(*shape) == 4 * math.prod(pshape)
+    gb = torch.full(pshape, -1.0, dtype=torch.float32, device=dev)
     L.call("nv_ray_bricks", grho.data_ptr(), *shape, gb.data_ptr(), stream(env))
     np.testing.assert_array_equal(N(gb), bricks_numpy(rho))
     Lb = tuple(n * dx for n in shape)
