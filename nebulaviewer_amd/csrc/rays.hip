@@ -223,7 +223,7 @@ __global__ __launch_bounds__(256) void ray_bricks_batch_kernel(const BrickJob* _
 __device__ __forceinline__ void cell_sph(int i, int j, int k, float sx, float sy, float sz, float dx, float* r_o,
                                          float* u_o, float* phi_o) {
   const float r0 = ((float)i + 0.5f) * dx - sx, r1 = ((float)j + 0.5f) * dx - sy, r2 = ((float)k + 0.5f) * dx - sz;
-  const float r = fmaxf(sqrtf((r0 * r0 + r1 * r1) + r2 * r2), 1e-6f);
+  const float r = fmaxf(sqrt_rn((r0 * r0 + r1 * r1) + r2 * r2), 1e-6f);
   float phi = nvm::atan2f_det(r1, r0);
   if (phi < 0.0f) phi = phi + 2.0f * PI_F;
   *r_o = r;
@@ -343,6 +343,7 @@ __global__ __launch_bounds__(256) void fill_kernel(float* __restrict__ dst, floa
 constexpr int IK = 32, IJ = 16, INT_ = IK * IJ;    // tile: 32 z x 16 y, marching along x
 constexpr int XK = IK + 2, XJ = IJ + 2, XN = XK * XJ;
 constexpr int XR = 4;                               // ring of unsmoothed-xi planes (i-1, i, i+1 read; i+2 written)
+#define CONST_AS __attribute__((address_space(4)))   // AMDGPU constant memory: uniform loads become scalar loads
 
 struct IonArgs {
   const float* rfronts;
@@ -371,6 +372,9 @@ __device__ __forceinline__ void ionize_tile(const IonArgs& A, int bk, int bj, in
   const int k = k0 + tx, j = j0 + ty;
   const bool active = k < nz && j < ny;
   const long long tab = (long long)A.nu * A.nphi;
+  // the star table is read-only for the whole launch: through the constant address space its loads (at a
+  // wave-uniform index) are scalar, instead of a vector load and wait per star and cell
+  const CONST_AS float* stars = (const CONST_AS float*)A.star_xyz;
 
   // unsmoothed xi and fp32 dfront of plane p (clamped cell positions, as ldc in smooth_xi)
   auto make_plane = [&](int p) {
@@ -381,7 +385,7 @@ __device__ __forceinline__ void ionize_tile(const IonArgs& A, int bk, int bj, in
       const int jc = clampi(j0 + jj - 1, 0, ny - 1), kc = clampi(k0 + kk - 1, 0, nz - 1);
       float x = 0.0f, df = 1e3f;   // xi.zero_(), df.fill_(1e3) (sim.py:930-931)
       for (int st = 0; st < A.nstars; ++st) {
-        const float ssx = A.star_xyz[3 * st], ssy = A.star_xyz[3 * st + 1], ssz = A.star_xyz[3 * st + 2];
+        const float ssx = stars[3 * st], ssy = stars[3 * st + 1], ssz = stars[3 * st + 2];
         float r, u, phi;
         cell_sph(ic, jc, kc, ssx, ssy, ssz, A.dx, &r, &u, &phi);
         const float rf = front_at(A.rfronts + st * tab, A.nu, A.nphi, u, phi);
@@ -398,6 +402,21 @@ __device__ __forceinline__ void ionize_tile(const IonArgs& A, int bk, int bj, in
   make_plane(i0);
   float tmax = 0.0f;
   for (int i = i0; i < i1; ++i) {
+    // the state of plane i for the CFL max, issued before the fronts of plane i+1 so that its loads overlap them
+    const long long o = cell(i, j, k, ny, nz);
+    float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f, f3 = 0.0f;
+    if (SMAX && active) {
+      if constexpr (VEL) {
+        f1 = ldf((const uint16_t*)A.f1, o);
+        f2 = ldf((const uint16_t*)A.f2, o);
+        f3 = ldf((const uint16_t*)A.f3, o);
+      } else {
+        f0 = ((const float*)A.f0)[o];
+        f1 = ((const float*)A.f1)[o];
+        f2 = ((const float*)A.f2)[o];
+        f3 = ((const float*)A.f3)[o];
+      }
+    }
     make_plane(i + 1);
     __syncthreads();
     if (active) {
@@ -408,7 +427,6 @@ __device__ __forceinline__ void ionize_tile(const IonArgs& A, int bk, int bj, in
       // neighbours in the order of nebula3d.py:313-314
       const float m = div_r(pm[c] + pp[c] + pc[c - XK] + pc[c + XK] + pc[c - 1] + pc[c + 1], R6);
       const float xs = 0.5f * pc[c] + 0.5f * m;
-      const long long o = cell(i, j, k, ny, nz);
       A.xi[o] = xs;
       const float df = sd_[((i % XR) + XR) % XR][c];
       if (A.df_h) A.df_h[o] = f32_to_f16_bits(df);
@@ -416,13 +434,11 @@ __device__ __forceinline__ void ionize_tile(const IonArgs& A, int bk, int bj, in
       if constexpr (SMAX) {
         float s;
         if constexpr (VEL) {
-          s = ((fabsf(ldf((const uint16_t*)A.f1, o)) + fabsf(ldf((const uint16_t*)A.f2, o))) +
-               fabsf(ldf((const uint16_t*)A.f3, o))) + sqrtf(A.cn2 + xs * (A.ci2 - A.cn2));
+          s = ((fabsf(f1) + fabsf(f2)) + fabsf(f3)) + sqrt_rn(A.cn2 + xs * (A.ci2 - A.cn2));
         } else {
           float v0, v1, v2;
-          div3(((const float*)A.f1)[o], ((const float*)A.f2)[o], ((const float*)A.f3)[o], ((const float*)A.f0)[o],
-               &v0, &v1, &v2);
-          s = ((fabsf(v0) + fabsf(v1)) + fabsf(v2)) + sqrtf(A.cn2 + xs * (A.ci2 - A.cn2));
+          div3(f1, f2, f3, f0, &v0, &v1, &v2);
+          s = ((fabsf(v0) + fabsf(v1)) + fabsf(v2)) + sqrt_rn(A.cn2 + xs * (A.ci2 - A.cn2));
         }
         tmax = fmaxf(tmax, s);
       }
