@@ -36,6 +36,10 @@ inline float pow2_reciprocal(float x) {
 }
 
 // ------------------------------------------------------------------------------------ device helpers
+// AMDGPU constant address space: for a table that no thread writes during the launch, loads at a wave-uniform
+// index through such a pointer are scalar loads (no vector load and wait per lane)
+#define CONST_AS __attribute__((address_space(4)))
+
 // v clamped to [lo, hi] (lo <= hi at every call); one v_med3_i32
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
 

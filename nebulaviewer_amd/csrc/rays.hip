@@ -343,7 +343,6 @@ __global__ __launch_bounds__(256) void fill_kernel(float* __restrict__ dst, floa
 constexpr int IK = 32, IJ = 16, INT_ = IK * IJ;    // tile: 32 z x 16 y, marching along x
 constexpr int XK = IK + 2, XJ = IJ + 2, XN = XK * XJ;
 constexpr int XR = 4;                               // ring of unsmoothed-xi planes (i-1, i, i+1 read; i+2 written)
-#define CONST_AS __attribute__((address_space(4)))   // AMDGPU constant memory: uniform loads become scalar loads
 
 struct IonArgs {
   const float* rfronts;
