@@ -261,22 +261,63 @@ __device__ __forceinline__ void phi_pair(const float* __restrict__ row, int pa, 
   *vb = pb == q + 1 ? v.y : row[pb];
 }
 
-// bilinear front radius at direction (u, phi) (nebula3d.py:291-301, rf_at 273-278)
-__device__ __forceinline__ float front_at(const float* __restrict__ rfront, int nu, int nphi, float u, float phi) {
+// The bilinear front radius at direction (u, phi) (nebula3d.py:291-301, rf_at 273-278) in two halves, so that a
+// caller can issue the rfront gathers early and use them later: front_taps issues one 8-byte load per table row,
+// (row[q], row[q+1]) with q = min(pa, nphi - 2), and front_lerp combines them. The two phi taps are (pa, pa + 1)
+// except at the wrap pa = nphi - 1, where they are (nphi - 1, 0): front_lerp then loads row[0] itself (a rare,
+// late load), so front_taps has nothing to wait for.
+struct FrontTaps {
+  float tu, tp;
+  float a0, a1, b0, b1;   // (row[q], row[q+1]) of rows ua and ub
+  int ua, ub;             // row offsets (ua nphi, ub nphi)
+  bool wrap;
+};
+
+__device__ __forceinline__ void front_taps(const float* __restrict__ rfront, int nu, int nphi, float u, float phi,
+                                           FrontTaps& T) {
   const float fu = (u + 1.0f) * 0.5f * (float)nu - 0.5f;
   const float fp = div_r(phi, R2PI) * (float)nphi - 0.5f;
   const int iu = (int)floorf(fu), ip = (int)floorf(fp);
-  const float tu = fu - (float)iu, tp = fp - (float)ip;
-  const int ua = clampi(iu, 0, nu - 1), ub = clampi(iu + 1, 0, nu - 1);
-  const int pa = wrap_phi(ip, nphi), pb = wrap_phi(ip + 1, nphi);
-  const float* ra = rfront + ua * nphi;
-  const float* rb = rfront + ub * nphi;
-  float ra0, ra1, rb0, rb1;
-  phi_pair(ra, pa, pb, nphi, &ra0, &ra1);
-  phi_pair(rb, pa, pb, nphi, &rb0, &rb1);
-  const float av = ra0 * (1.0f - tp) + ra1 * tp;
-  const float bv = rb0 * (1.0f - tp) + rb1 * tp;
-  return av * (1.0f - tu) + bv * tu;
+  T.tu = fu - (float)iu;
+  T.tp = fp - (float)ip;
+  T.ua = clampi(iu, 0, nu - 1) * nphi;
+  T.ub = clampi(iu + 1, 0, nu - 1) * nphi;
+  const int pa = wrap_phi(ip, nphi);
+  if (nphi < 2) {   // one phi bin: both taps are row[0]
+    T.a0 = T.a1 = rfront[T.ua];
+    T.b0 = T.b1 = rfront[T.ub];
+    T.wrap = false;
+    return;
+  }
+  const int q = min(pa, nphi - 2);
+  typedef float f2u __attribute__((ext_vector_type(2), aligned(4)));
+  const f2u va = *reinterpret_cast<const f2u*>(rfront + T.ua + q);
+  const f2u vb = *reinterpret_cast<const f2u*>(rfront + T.ub + q);
+  T.a0 = va.x;
+  T.a1 = va.y;
+  T.b0 = vb.x;
+  T.b1 = vb.y;
+  T.wrap = pa != q;
+}
+
+__device__ __forceinline__ float front_lerp(const float* __restrict__ rfront, const FrontTaps& T) {
+  float ra0 = T.a0, ra1 = T.a1, rb0 = T.b0, rb1 = T.b1;
+  if (T.wrap) {   // taps (nphi - 1, 0): the loaded pair is (nphi - 2, nphi - 1)
+    ra0 = T.a1;
+    rb0 = T.b1;
+    ra1 = rfront[T.ua];
+    rb1 = rfront[T.ub];
+  }
+  const float av = ra0 * (1.0f - T.tp) + ra1 * T.tp;
+  const float bv = rb0 * (1.0f - T.tp) + rb1 * T.tp;
+  return av * (1.0f - T.tu) + bv * T.tu;
+}
+
+// bilinear front radius at direction (u, phi)
+__device__ __forceinline__ float front_at(const float* __restrict__ rfront, int nu, int nphi, float u, float phi) {
+  FrontTaps T;
+  front_taps(rfront, nu, nphi, u, phi, T);
+  return front_lerp(rfront, T);
 }
 
 // x / d for one divisor and three numerators: the shared reciprocal where d is a positive normal float (the
@@ -343,6 +384,7 @@ __global__ __launch_bounds__(256) void fill_kernel(float* __restrict__ dst, floa
 constexpr int IK = 32, IJ = 16, INT_ = IK * IJ;    // tile: 32 z x 16 y, marching along x
 constexpr int XK = IK + 2, XJ = IJ + 2, XN = XK * XJ;
 constexpr int XR = 4;                               // ring of unsmoothed-xi planes (i-1, i, i+1 read; i+2 written)
+static_assert(INT_ <= XN, "every thread owns a halo cell of the first pass");
 
 struct IonArgs {
   const float* rfronts;
@@ -358,7 +400,12 @@ struct IonArgs {
   Recip rwidth;                    // the ramp width and 1 / width, correctly rounded (its shared reciprocal)
 };
 
-// One 64 z x 8 y tile of the fused ionize pass, marching over x chunk bi.
+// One 32 z x 16 y tile of the fused ionize pass, marching over x chunk bi.
+//
+// Per plane i: the unsmoothed xi and dfront of the tile plus a one-cell y/z halo (XN cells: 512 threads take the
+// first XN - 512 of them twice) go to an LDS ring; plane i is smoothed from planes i-1, i, i+1. The rfront
+// gathers of plane i+2's first pass (its first star) are issued before plane i is smoothed and stored, and used
+// after: the smoothing, the stores and the CFL max run while the gathers are in flight.
 template <bool SMAX, bool VEL>
 __device__ __forceinline__ void ionize_tile(const IonArgs& A, int bk, int bj, int bi) {
   __shared__ float sx_[XR][XN];   // unsmoothed xi (max over stars of the front ramp)
@@ -375,33 +422,45 @@ __device__ __forceinline__ void ionize_tile(const IonArgs& A, int bk, int bj, in
   // wave-uniform index) are scalar, instead of a vector load and wait per star and cell
   const CONST_AS float* stars = (const CONST_AS float*)A.star_xyz;
 
-  // unsmoothed xi and fp32 dfront of plane p (clamped cell positions, as ldc in smooth_xi)
-  auto make_plane = [&](int p) {
-    const int s = ((p % XR) + XR) % XR;
-    const int ic = clampi(p, 0, nx - 1);
-    for (int t = tid; t < XN; t += INT_) {
-      const int jj = t / XK, kk = t - jj * XK;
-      const int jc = clampi(j0 + jj - 1, 0, ny - 1), kc = clampi(k0 + kk - 1, 0, nz - 1);
-      float x = 0.0f, df = 1e3f;   // xi.zero_(), df.fill_(1e3) (sim.py:930-931)
-      for (int st = 0; st < A.nstars; ++st) {
-        const float ssx = stars[3 * st], ssy = stars[3 * st + 1], ssz = stars[3 * st + 2];
-        float r, u, phi;
-        cell_sph(ic, jc, kc, ssx, ssy, ssz, A.dx, &r, &u, &phi);
-        const float rf = front_at(A.rfronts + st * tab, A.nu, A.nphi, u, phi);
-        const float val = fminf(fmaxf(div_r(rf - r, A.rwidth) + 0.5f, 0.0f), 1.0f);
-        if (val > x) x = val;
-        df = fminf(df, r - rf);
-      }
-      sx_[s][t] = x;
-      sd_[s][t] = df;
+  // halo-plane cell t of plane p (clamped cell positions, as ldc in smooth_xi): its distance, direction and the
+  // front taps of star st
+  auto star_taps = [&](int p, int t, int st, float& r, FrontTaps& T) {
+    const int jj = t / XK, kk = t - jj * XK;
+    const int ic = clampi(p, 0, nx - 1), jc = clampi(j0 + jj - 1, 0, ny - 1), kc = clampi(k0 + kk - 1, 0, nz - 1);
+    float u, phi;
+    cell_sph(ic, jc, kc, stars[3 * st], stars[3 * st + 1], stars[3 * st + 2], A.dx, &r, &u, &phi);
+    front_taps(A.rfronts + st * tab, A.nu, A.nphi, u, phi, T);
+  };
+  // one star's ramp into the running max (xi) and min (dfront)
+  auto accumulate = [&](float r, float rf, float& x, float& df) {
+    const float val = fminf(fmaxf(div_r(rf - r, A.rwidth) + 0.5f, 0.0f), 1.0f);
+    if (val > x) x = val;
+    df = fminf(df, r - rf);
+  };
+  // stars s0.. of halo cell t of plane p, written to the ring; x, df carry the stars before s0
+  auto finish_cell = [&](int p, int t, int s0, float x, float df) {
+    for (int st = s0; st < A.nstars; ++st) {
+      float r;
+      FrontTaps T;
+      star_taps(p, t, st, r, T);
+      accumulate(r, front_lerp(A.rfronts + st * tab, T), x, df);
     }
+    const int sl = ((p % XR) + XR) % XR;
+    sx_[sl][t] = x;
+    sd_[sl][t] = df;
+  };
+  // whole plane p: every halo cell, every star (xi.zero_(), df.fill_(1e3): sim.py:930-931)
+  auto make_plane = [&](int p) {
+    for (int t = tid; t < XN; t += INT_) finish_cell(p, t, 0, 0.0f, 1e3f);
   };
 
   make_plane(i0 - 1);
   make_plane(i0);
+  make_plane(i0 + 1);
+  __syncthreads();
   float tmax = 0.0f;
   for (int i = i0; i < i1; ++i) {
-    // the state of plane i for the CFL max, issued before the fronts of plane i+1 so that its loads overlap them
+    // the state of plane i for the CFL max, then the first star's front gathers of plane i+2 (first pass)
     const long long o = cell(i, j, k, ny, nz);
     float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f, f3 = 0.0f;
     if (SMAX && active) {
@@ -416,8 +475,11 @@ __device__ __forceinline__ void ionize_tile(const IonArgs& A, int bk, int bj, in
         f3 = ((const float*)A.f3)[o];
       }
     }
-    make_plane(i + 1);
-    __syncthreads();
+    const bool next = i + 2 <= i1;   // plane i1 is the last one a smoothing of this chunk reads
+    float r0 = 0.0f;
+    FrontTaps T0;
+    if (next && A.nstars > 0) star_taps(i + 2, tid, 0, r0, T0);
+
     if (active) {
       const int c = (ty + 1) * XK + tx + 1;
       const float* pc = sx_[((i % XR) + XR) % XR];
@@ -442,6 +504,15 @@ __device__ __forceinline__ void ionize_tile(const IonArgs& A, int bk, int bj, in
         tmax = fmaxf(tmax, s);
       }
     }
+
+    if (next) {
+      // plane i+2 goes to the ring slot of plane i-2, which no thread reads after the last barrier
+      float x = 0.0f, df = 1e3f;
+      if (A.nstars > 0) accumulate(r0, front_lerp(A.rfronts, T0), x, df);
+      finish_cell(i + 2, tid, 1, x, df);
+      for (int t = tid + INT_; t < XN; t += INT_) finish_cell(i + 2, t, 0, 0.0f, 1e3f);
+    }
+    __syncthreads();
   }
   if constexpr (SMAX) block_max_atomic(tmax, red, A.smax);
 }

@@ -389,13 +389,17 @@ def test_apply_front_smooth_half_fill_bitwise(env, okern32):
     assert bool((gout == 1e3).all())
 
 
-@pytest.mark.parametrize("nst,half,shape", [(1, True, (40, 36, 44)), (3, False, (23, 70, 130)), (0, True, (8, 8, 8))])
-def test_ionize_apply_fused_bitwise(env, okern32, nst, half, shape):
-    say(f"[ionize_apply] nstars={nst} half={half} shape={shape}")
+@pytest.mark.parametrize("nst,half,shape,rays", [(1, True, (40, 36, 44), (32, 64)), (3, False, (23, 70, 130), (32, 64)),
+                                                (0, True, (8, 8, 8), (32, 64)), (2, True, (9, 20, 40), (16, 1)),
+                                                (1, False, (13, 17, 33), (8, 2))])
+def test_ionize_apply_fused_bitwise(env, okern32, nst, half, shape, rays):
+    """the fused pass against the reference's launch sequence; x chunks of 4 planes leave one-plane tails at nx = 9
+    and 13; a single phi bin and two phi bins (every lookup at the wrap) exercise the front lookup's edge forms"""
+    say(f"[ionize_apply] nstars={nst} half={half} shape={shape} rays={rays}")
     torch, dev, L = env
     rho, dx, Lb, diag = ray_case(shape, seed=13)
     U = [rho] + [(rho * np.random.default_rng(s).normal(0, 0.5, shape)).astype(F32) for s in range(3)]
-    nu, nphi = 32, 64
+    nu, nphi = rays
     dr = 0.5 * dx
     nsteps = int(diag / dr) + 1
     stars = [(0.4 * Lb[0], 0.35 * Lb[1], 0.5 * Lb[2]), (0.9 * Lb[0], 0.1 * Lb[1], 0.2 * Lb[2]),
