@@ -384,7 +384,7 @@ __global__ __launch_bounds__(256) void fill_kernel(float* __restrict__ dst, floa
 constexpr int IK = 32, IJ = 16, INT_ = IK * IJ;    // tile: 32 z x 16 y, marching along x
 constexpr int XK = IK + 2, XJ = IJ + 2, XN = XK * XJ;
 constexpr int XR = 4;                               // ring of unsmoothed-xi planes (i-1, i, i+1 read; i+2 written)
-static_assert(INT_ <= XN, "every thread owns a halo cell of the first pass");
+static_assert(INT_ <= XN && XN <= 2 * INT_, "every thread owns one or two halo cells of a plane");
 
 struct IonArgs {
   const float* rfronts;
@@ -476,9 +476,13 @@ __device__ __forceinline__ void ionize_tile(const IonArgs& A, int bk, int bj, in
       }
     }
     const bool next = i + 2 <= i1;   // plane i1 is the last one a smoothing of this chunk reads
-    float r0 = 0.0f;
-    FrontTaps T0;
-    if (next && A.nstars > 0) star_taps(i + 2, tid, 0, r0, T0);
+    const bool second = tid + INT_ < XN;   // this thread also owns a cell of the second pass
+    float r0 = 0.0f, r1 = 0.0f;
+    FrontTaps T0, T1;
+    if (next && A.nstars > 0) {
+      star_taps(i + 2, tid, 0, r0, T0);
+      if (second) star_taps(i + 2, tid + INT_, 0, r1, T1);
+    }
 
     if (active) {
       const int c = (ty + 1) * XK + tx + 1;
@@ -510,7 +514,12 @@ __device__ __forceinline__ void ionize_tile(const IonArgs& A, int bk, int bj, in
       float x = 0.0f, df = 1e3f;
       if (A.nstars > 0) accumulate(r0, front_lerp(A.rfronts, T0), x, df);
       finish_cell(i + 2, tid, 1, x, df);
-      for (int t = tid + INT_; t < XN; t += INT_) finish_cell(i + 2, t, 0, 0.0f, 1e3f);
+      if (second) {
+        x = 0.0f;
+        df = 1e3f;
+        if (A.nstars > 0) accumulate(r1, front_lerp(A.rfronts, T1), x, df);
+        finish_cell(i + 2, tid + INT_, 1, x, df);
+      }
     }
     __syncthreads();
   }
