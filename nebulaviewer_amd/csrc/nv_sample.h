@@ -93,7 +93,7 @@ __device__ __forceinline__ float sample_r(const float* __restrict__ f, int nx, i
 // (f(i, ja, ka), f(i, ja, kb), f(i, jb, ka), f(i, jb, kb)) with i = clamp(p-1), ja = clamp(q-1), jb = clamp(q),
 // ka = clamp(r-1), kb = clamp(r). A sample inside [-1/2, n-1/2] has i0, j0, k0 in [-1, n-1], and its eight corners
 // (ia = clamp(i0), ib = clamp(i0+1), likewise j and k: what sample_r reads) are the bricks at (i0+1, j0+1, k0+1)
-// and (i0+2, j0+1, k0+1): two 16-byte loads, no clamping and no selects. Every corner is the value sample_r reads
+// and (i0+2, j0+1, k0+1): two 16-byte loads, no index clamps or selects. Every corner is the value sample_r reads
 // and the interpolation is the same expression, so the result is identical.
 template <class OFF>
 __device__ __forceinline__ float4 ld4(const float4* __restrict__ B, unsigned i) {
@@ -103,14 +103,26 @@ __device__ __forceinline__ float4 ld4(const float4* __restrict__ B, unsigned i) 
 template <class OFF, class SC>
 __device__ __forceinline__ float sample_brick(const float4* __restrict__ B, int nx, int ny, int nz, float px, float py,
                                               float pz, const SC& cs) {
-  const float gx = cs(px) - 0.5f, gy = cs(py) - 0.5f, gz = cs(pz) - 0.5f;
-  if (gx < -0.5f || gy < -0.5f || gz < -0.5f) return 0.0f;
-  if (gx > (float)nx - 0.5f || gy > (float)ny - 0.5f || gz > (float)nz - 0.5f) return 0.0f;
+  const float ux = cs(px) - 0.5f, uy = cs(py) - 0.5f, uz = cs(pz) - 0.5f;
+  const float hx = (float)nx - 0.5f, hy = (float)ny - 0.5f, hz = (float)nz - 0.5f;
+  const bool outside = ux < -0.5f || uy < -0.5f || uz < -0.5f || ux > hx || uy > hy || uz > hz;
+  // No branch: a sample outside the box reads the bricks of its clamped position and returns 0, so the loads of
+  // consecutive samples can be in flight together. Inside, the clamp leaves the coordinates unchanged.
+  const float gx = fminf(fmaxf(ux, -0.5f), hx), gy = fminf(fmaxf(uy, -0.5f), hy), gz = fminf(fmaxf(uz, -0.5f), hz);
   const float flx = floorf(gx), fly = floorf(gy), flz = floorf(gz);
   const int i0 = (int)flx, j0 = (int)fly, k0 = (int)flz;
   const float fx = gx - flx, fy = gy - fly, fz = gz - flz;
   const unsigned qy = ny + 1, qz = nz + 1;
-  const unsigned ia = cell_index<OFF>(i0 + 1, j0 + 1, k0 + 1, qy, qz);
+  unsigned ia;
+  if constexpr (sizeof(OFF) == 4) {
+    // the padded row (i0 + 1) qy + j0 + 1 is below 2^24 on this path: one exact fma of integer-valued floats,
+    // then a 24-bit multiply-add (an integer (i0 + 1) qy, rewritten by the compiler as i0 qy + qy, becomes a
+    // quarter-rate 32-bit multiply)
+    const unsigned row = (unsigned)fmaf(flx, (float)qy, fly + (float)(qy + 1));
+    ia = __umul24(row, qz) + (unsigned)(k0 + 1);
+  } else {
+    ia = cell_index<OFF>(i0 + 1, j0 + 1, k0 + 1, qy, qz);
+  }
   const float4 a = ld4<OFF>(B, ia), b = ld4<OFF>(B, ia + qy * qz);
   const float c000 = a.x, c100 = b.x, c001 = a.y, c101 = b.y;
   const float c010 = a.z, c110 = b.z, c011 = a.w, c111 = b.w;
@@ -120,7 +132,9 @@ __device__ __forceinline__ float sample_brick(const float4* __restrict__ B, int 
   const float c11 = c011 + (c111 - c011) * fx;
   const float c0 = c00 + (c10 - c00) * fy;
   const float c1 = c01 + (c11 - c01) * fy;
-  return c0 + (c1 - c0) * fz;
+  // +0 outside by a bit mask, not a select: the compiler turns a select over loaded values back into a branch
+  const unsigned keep = outside ? 0u : ~0u;
+  return __uint_as_float(__float_as_uint(c0 + (c1 - c0) * fz) & keep);
 }
 
 }  // namespace nv

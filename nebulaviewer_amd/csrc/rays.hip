@@ -164,7 +164,7 @@ __host__ __device__ inline long long ray_blocks(int nu, int nphi) {
 
 // Many ray-cast jobs in one launch: block b belongs to the job s with start[s] <= b < start[s+1]. The cell
 // scaling and the FP64 column are per job (block-uniform branches); paired z loads and 32-bit offsets are shared.
-template <bool PAIR, class OFF>
+template <bool PAIR, class OFF, bool ANY_BRICK>
 __global__ __launch_bounds__(256) void cast_rays_batch_kernel(const RayJob* __restrict__ jobs,
                                                               const int* __restrict__ start, int n, int total) {
   const int b = blockIdx.x;
@@ -191,7 +191,9 @@ __global__ __launch_bounds__(256) void cast_rays_batch_kernel(const RayJob* __re
       else cast_one<PAIR, false, OFF, ScaleDiv, BR>(J, sc, t);
     }
   };
-  if (J.bricks) by_scale(std::true_type{});
+  // without brick jobs the kernel holds the direct sampler alone (the brick sampler's registers would lower the
+  // occupancy of every job)
+  if (ANY_BRICK && J.bricks) by_scale(std::true_type{});
   else by_scale(std::false_type{});
 }
 
@@ -1029,8 +1031,12 @@ extern "C" int nv_cast_rays_batch(const void* plan_host, const void* plan_dev, v
     hipLaunchKernelGGL(ray_bricks_batch_kernel, dim3(H.btotal), dim3(256), 0, as_stream(stream), bj, bstart, H.nb,
                        H.btotal);
   }
-  auto kern = H.pair ? (H.off32 ? cast_rays_batch_kernel<true, uint32_t> : cast_rays_batch_kernel<true, uint64_t>)
-                     : (H.off32 ? cast_rays_batch_kernel<false, uint32_t> : cast_rays_batch_kernel<false, uint64_t>);
+  auto pick = [&](auto brick) {
+    constexpr bool BR = decltype(brick)::value;
+    return H.pair ? (H.off32 ? cast_rays_batch_kernel<true, uint32_t, BR> : cast_rays_batch_kernel<true, uint64_t, BR>)
+                  : (H.off32 ? cast_rays_batch_kernel<false, uint32_t, BR> : cast_rays_batch_kernel<false, uint64_t, BR>);
+  };
+  auto kern = H.nb > 0 ? pick(std::true_type{}) : pick(std::false_type{});
   hipLaunchKernelGGL(kern, dim3(H.total), dim3(256), 0, as_stream(stream), jobs, start, H.n, H.total);
   return check_launch("nv_cast_rays_batch");
 }
