@@ -280,6 +280,12 @@ __device__ __forceinline__ float4 face_flux(const float4& Lq, const float4& Rq, 
   return make_float4(F[0], F[1], F[2], F[3]);
 }
 
+// Which stage forms load the halo of the next plane one plane ahead (a register cost of four per thread).
+template <bool STAGE1>
+struct HALO_PREFETCH {
+  static constexpr bool value = STAGE1;
+};
+
 // One tile (64 z x 8 y columns, marching along x over ci planes) of one RK stage.
 // STAGE1: a = 0, b = 1 and U0 = U (sim.py:1025), so out = 0*U0 + 1*Un is Un (up to the sign of zero).
 template <class TL, class ST, class AR, int SOLVER, int LIM, bool SMAX, bool STAGE1>
@@ -384,6 +390,20 @@ __device__ __forceinline__ void stage_tile(const HydroArgs& A, int tile) {
   const float dx = A.dx;
   float tmax = 0.0f;
 
+  // The halo of plane p (at most one cell per thread: the primitive halo on the last NHALO threads, the c^2 halo
+  // on the first NCHALO): with PF it is loaded into registers one plane ahead, so its latency is not exposed
+  // between the start of a plane and its first barrier.
+  static_assert(NHALO <= NT && NCHALO <= NT && NCHALO <= NT - NHALO, "one halo cell per thread");
+  constexpr bool PF = HALO_PREFETCH<STAGE1>::value;
+  const bool hq = tid >= NT - NHALO, hc = tid < NCHALO;
+  auto halo_load = [&](int p, float4& raw) {
+    const long long pb = (long long)p * plane;
+    if (hq) raw = ST::load(A.f0, A.f1, A.f2, A.f3, pb + htab[NT - 1 - tid].y);
+    else if (hc) raw.x = A.xi[pb + ctab[tid].y];
+  };
+  float4 hraw = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if constexpr (PF) halo_load(i0, hraw);
+
   for (int i = i0; i < i1; ++i) {
     // next window plane (i+3): issued now, consumed at the end of this iteration
     float4 rnext;
@@ -398,21 +418,33 @@ __device__ __forceinline__ void stage_tile(const HydroArgs& A, int tile) {
     const int cc = (ty + 1) * CK + tx + 1;
     sq[cq] = qw[0];
     sc[cc] = make_float2(c2w[0], cw[0]);
-    const long long pbase = (long long)i * plane;
     // halo cells from the last wavefronts up (the first ones take the c^2 halo here and the tile-boundary edges
     // and faces later), so that no wavefront carries two of these jobs between two barriers
-    for (int h = NT - 1 - tid; h < NHALO; h += NT) {
-      const int2 e = htab[h];
-      float4 qh, Uh;
-      decode<ST, AR>(ST::load(A.f0, A.f1, A.f2, A.f3, pbase + e.y), floor_, qh, Uh);
-      sq[e.x] = qh;
-    }
-    for (int h = tid; h < NCHALO; h += NT) {
-      const int2 e = ctab[h];
-      const float c2 = AR::mad(A.xi[pbase + e.y], dci, cn2);
-      sc[e.x] = make_float2(c2, AR::sqrt(c2));
+    if constexpr (PF) {
+      if (hq) {
+        float4 qh, Uh;
+        decode<ST, AR>(hraw, floor_, qh, Uh);
+        sq[htab[NT - 1 - tid].x] = qh;
+      } else if (hc) {
+        const float c2 = AR::mad(hraw.x, dci, cn2);
+        sc[ctab[tid].x] = make_float2(c2, AR::sqrt(c2));
+      }
+    } else {
+      const long long pbase = (long long)i * plane;
+      for (int h = NT - 1 - tid; h < NHALO; h += NT) {
+        const int2 e = htab[h];
+        float4 qh, Uh;
+        decode<ST, AR>(ST::load(A.f0, A.f1, A.f2, A.f3, pbase + e.y), floor_, qh, Uh);
+        sq[e.x] = qh;
+      }
+      for (int h = tid; h < NCHALO; h += NT) {
+        const int2 e = ctab[h];
+        const float c2 = AR::mad(A.xi[pbase + e.y], dci, cn2);
+        sc[e.x] = make_float2(c2, AR::sqrt(c2));
+      }
     }
     __syncthreads();
+    if (PF && i + 1 < i1) halo_load(i + 1, hraw);
 
     // ---- B: edge states. Own cell in y and z; the tile's outer neighbours for the boundary faces.
     float4 loy, loz, hi;
