@@ -404,10 +404,10 @@ struct IonArgs {
 
 // One 32 z x 16 y tile of the fused ionize pass, marching over x chunk bi.
 //
-// Per plane i: the unsmoothed xi and dfront of the tile plus a one-cell y/z halo (XN cells: 512 threads take the
-// first XN - 512 of them twice) go to an LDS ring; plane i is smoothed from planes i-1, i, i+1. The rfront
-// gathers of plane i+2's first pass (its first star) are issued before plane i is smoothed and stored, and used
-// after: the smoothing, the stores and the CFL max run while the gathers are in flight.
+// Per plane i: the unsmoothed xi and dfront of the tile plus a one-cell y/z halo (XN cells: thread t owns cell t,
+// and cell t + 512 where that exists) go to an LDS ring; plane i is smoothed from planes i-1, i, i+1. The first
+// star's rfront gathers for a thread's cells of plane i+2 are issued before plane i is smoothed and stored, and
+// used after: the smoothing, the stores and the CFL max run while the gathers are in flight.
 template <bool SMAX, bool VEL>
 __device__ __forceinline__ void ionize_tile(const IonArgs& A, int bk, int bj, int bi) {
   __shared__ float sx_[XR][XN];   // unsmoothed xi (max over stars of the front ramp)
