@@ -56,10 +56,26 @@ __device__ __forceinline__ float sample_r(const float* __restrict__ f, int nx, i
   const float flx = floorf(gx), fly = floorf(gy), flz = floorf(gz);
   const int i0 = (int)flx, j0 = (int)fly, k0 = (int)flz;
   const float fx = gx - flx, fy = gy - fly, fz = gz - flz;
-  const unsigned ia = clampi(i0, 0, nx - 1), ib = clampi(i0 + 1, 0, nx - 1);
-  const unsigned ja = clampi(j0, 0, ny - 1), jb = clampi(j0 + 1, 0, ny - 1);
-  const unsigned r00 = cell_index<OFF>(ia, ja, 0, ny, nz), r10 = cell_index<OFF>(ib, ja, 0, ny, nz);
-  const unsigned r01 = cell_index<OFF>(ia, jb, 0, ny, nz), r11 = cell_index<OFF>(ib, jb, 0, ny, nz);
+  unsigned r00, r10, r01, r11;
+  if constexpr (sizeof(OFF) == 4) {
+    // rows i ny + j below 2^24 on this path: clamped in float (one v_med3 each) and formed by one exact fma of
+    // integer-valued floats, then a 24-bit multiply (integer clamps of i0 + 1 are rewritten by the compiler into
+    // a quarter-rate 32-bit multiply)
+    const float xm = (float)(nx - 1), ym = (float)(ny - 1), fny = (float)ny;
+    const float fia = fminf(fmaxf(flx, 0.0f), xm), fib = fminf(fmaxf(flx + 1.0f, 0.0f), xm);
+    const float fja = fminf(fmaxf(fly, 0.0f), ym), fjb = fminf(fmaxf(fly + 1.0f, 0.0f), ym);
+    r00 = __umul24((unsigned)fmaf(fia, fny, fja), nz);
+    r10 = __umul24((unsigned)fmaf(fib, fny, fja), nz);
+    r01 = __umul24((unsigned)fmaf(fia, fny, fjb), nz);
+    r11 = __umul24((unsigned)fmaf(fib, fny, fjb), nz);
+  } else {
+    const unsigned ia = clampi(i0, 0, nx - 1), ib = clampi(i0 + 1, 0, nx - 1);
+    const unsigned ja = clampi(j0, 0, ny - 1), jb = clampi(j0 + 1, 0, ny - 1);
+    r00 = cell_index<OFF>(ia, ja, 0, ny, nz);
+    r10 = cell_index<OFF>(ib, ja, 0, ny, nz);
+    r01 = cell_index<OFF>(ia, jb, 0, ny, nz);
+    r11 = cell_index<OFF>(ib, jb, 0, ny, nz);
+  }
   float c000, c100, c010, c110, c001, c101, c011, c111;
   if constexpr (PAIR) {
     const unsigned kp = clampi(k0, 0, nz - 2);
