@@ -280,7 +280,8 @@ __device__ __forceinline__ float4 face_flux(const float4& Lq, const float4& Rq, 
   return make_float4(F[0], F[1], F[2], F[3]);
 }
 
-// Which stage forms load the halo of the next plane one plane ahead (a register cost of four per thread).
+// Which stage forms load the halo of the next plane one plane ahead (four more registers per thread): stage 1;
+// stage 2 already holds U0 and the CFL max and would spill at its 128-register limit.
 template <bool STAGE1>
 struct HALO_PREFETCH {
   static constexpr bool value = STAGE1;
