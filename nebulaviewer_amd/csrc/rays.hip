@@ -462,7 +462,7 @@ __device__ __forceinline__ void ionize_tile(const IonArgs& A, int bk, int bj, in
   __syncthreads();
   float tmax = 0.0f;
   for (int i = i0; i < i1; ++i) {
-    // the state of plane i for the CFL max, then the first star's front gathers of plane i+2 (first pass)
+    // the state of plane i for the CFL max, then the first star's front gathers of this thread's cells of plane i+2
     const long long o = cell(i, j, k, ny, nz);
     float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f, f3 = 0.0f;
     if (SMAX && active) {
