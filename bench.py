@@ -36,26 +36,46 @@ HBM_PEAK_GBS = 8000.0
 # algorithmic bytes per cell per RK2 step: 5 S_U + 2 S_xi (S_xi = 4): S_U = 16 for the float32 state,
 # 10 for float32 density with binary16 velocity
 BYTES_PER_CELL_STEP = {"fp32": 88.0, "fp32-fast": 88.0, "fp16-vel": 58.0}
+HEADLINE = "fp32-fast"
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def measured_traffic(kernel="hydro_stage_kernel"):
-    """HBM bytes per launch of the kernel from the committed rocprofv3 --pmc summary (null if absent)."""
+def measured_traffic(precision):
+    """HBM bytes per launch of the stage kernel of this precision form on the 512^3 wall, from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE runs scaled by a plain copy's counts); null if absent."""
     path = os.path.join(ROOT, "profiles", "hydro_stage_wall512_traffic.md")
     if not os.path.exists(path):
         return None
-    m = re.search(r"hbm_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read())
+    m = re.search(r"hbm_bytes_per_launch\[" + re.escape(precision) + r"\]:\s*([0-9.eE+]+)", open(path).read())
     return float(m.group(1)) if m else None
+
+
+def host_cores():
+    """the host's CPU inventory: os.cpu_count(), the CPUs this process may run on, the OpenMP thread count the
+    environment grants, the CPU model; `use` is the thread count the CPU baseline runs with (all granted CPUs)"""
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    omp = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return dict(os_cpu_count=os.cpu_count(), affinity=aff, omp_num_threads=omp or None, model=model,
+                use=min(aff, omp) if omp > 0 else aff)
 
 
 def cpu_baseline(setup, ion_every):
     import numpy as np
     from nebulaviewer_amd import scenarios as S
     from oracle.oracle import OracleSim
-    threads = min(16, os.cpu_count() or 1)
+    host = host_cores()
+    threads = host["use"]
     C = S.constants(setup)
     t0 = time.perf_counter()
     ref = OracleSim(setup, C, S.ray_grid(setup.shape[2]), S.fuv_grid(setup.shape[2]), np.float32, threads=threads)
@@ -86,7 +106,7 @@ def cpu_baseline(setup, ion_every):
     k1.hydro_stage(U1, xi, U, U, a=0.5, b=0.5, **common)
     t1 = time.perf_counter() - t0
     single = math.prod(sl) / t1
-    return dict(value=n / per_step, unit="cell-updates/s", cores=threads, kind="port",
+    return dict(value=n / per_step, unit="cell-updates/s", cores=threads, kind="port", host=host,
                 sample=f"C++ oracle (float, -O3, OpenMP {threads} threads) on the same {setup.shape[0]}^3 grid: one "
                        f"ionization ({t_ion:.2f} s) and one CFL + two-stage hydro step ({t_hydro:.2f} s), "
                        f"per step = hydro + ionization/{ion_every}",
@@ -167,7 +187,7 @@ def make_measure_batch(a, S, KernelTimer):
         t0 = time.perf_counter()
         setups = S.batch_setups(count)
         log(f"[bench] batch of {count}: initial conditions in {time.perf_counter() - t0:.1f} s")
-        b = SimBatch(setups, precision="fp32", graph=True)
+        b = SimBatch(setups, precision=a.precision, graph=True)
         cells = b.cells
         b.step(a.warmup)                       # captures the two kinds of step
         torch.cuda.synchronize()
@@ -186,7 +206,7 @@ def make_measure_batch(a, S, KernelTimer):
                  sizes=sorted({m.shape[0] for m in b.members}), value=cells * a.steps / wall,
                  ms_per_step=wall / a.steps * 1e3, hydro_launch_ms=hyd_s * 1e3,
                  hydro_only_cell_updates_per_s=cells / (2 * hyd_s),
-                 hydro_achieved_GBps=BYTES_PER_CELL_STEP["fp32"] / 2 * cells / hyd_s / 1e9,
+                 hydro_achieved_GBps=BYTES_PER_CELL_STEP[a.precision] / 2 * cells / hyd_s / 1e9,
                  kernels_ms={k: round(v["avg_ms"], 4) for k, v in ks.items()}, graph_replay=True,
                  tile={1: "64x8", 2: "32x16", 3: "64x16"}.get(b.tile), ray_bricks=getattr(b, "ray_bricks_used", None),
                  dispatch={f"stage{k}": ("queue" if v else "hardware") for k, v in b.dispatch.items()})
@@ -206,16 +226,16 @@ def batch_only(a):
     from nebulaviewer_amd.timing import KernelTimer
     r = make_measure_batch(a, S, KernelTimer)(a.batch or 64)
     out = {
-        "metric": f"cell-updates/s, batch of {r['members']} uneven grids (FP32 step)",
+        "metric": f"cell-updates/s, batch of {r['members']} uneven grids ({a.precision} step)",
         "value": r["value"], "unit": "cell-updates/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
         "ms_per_step": r["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32", "precision": "fp32", "data": "synthetic (seeded initial conditions generated on the host)",
+        "dtype": "f32", "precision": a.precision, "data": "synthetic (seeded initial conditions generated on the host)",
         "config": {"workload": f"batch of {r['members']} members (torus / sheet, N in {r['sizes']}), HLL+minmod "
                                "RK2, CFL 0.3, ionization every 2 steps with 512x1024 rays, seeds 100+i",
                    "cells": r["cells"], "stars": r["stars"]},
         "roofline": {"bound": "hbm", "kernel": "hydro_stage_batch_kernel", "achieved": r["hydro_achieved_GBps"],
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": r["hydro_achieved_GBps"] / HBM_PEAK_GBS,
-                     "traffic": None, "algorithmic_bytes_per_launch": BYTES_PER_CELL_STEP["fp32"] / 2 * r["cells"]},
+                     "traffic": None, "algorithmic_bytes_per_launch": BYTES_PER_CELL_STEP[a.precision] / 2 * r["cells"]},
         "hydro_only_cell_updates_per_s": r["hydro_only_cell_updates_per_s"], "kernels_ms": r["kernels_ms"],
         "tile": r["tile"], "ray_bricks": r["ray_bricks"], "dispatch": r["dispatch"], "autotune": autotune.report(),
         "variants": {}, "cpu_baseline": None,
@@ -231,8 +251,10 @@ def main():
     ap.add_argument("--config", type=int, default=2, help="0-3: one grid of scenarios.CONFIGS; 4: the uneven batch")
     ap.add_argument("--n", type=int, default=None, help="override the grid size (testing only)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp32-fast", "fp16-vel"],
-                    help="the form whose throughput is `value` (fp32 = the bit-exact drop-in)")
+    ap.add_argument("--precision", default=HEADLINE, choices=["fp32", "fp32-fast", "fp16-vel"],
+                    help="the form whose throughput is `value`: fp32-fast (FP32 arithmetic with fused multiply-adds "
+                         "and hardware reciprocals, within the drop-in tolerance of the FP64 oracle), fp32 (the "
+                         "reference's operation order, bit-exact against the CPU oracle) or fp16-vel")
     ap.add_argument("--no-variants", action="store_true", help="do not also time the other two forms")
     ap.add_argument("--batch", type=int, default=64,
                     help="members of the uneven-batch variant (0: skip it; timed with the variants)")
@@ -262,7 +284,7 @@ def main():
     setup = S.make_setup(scenario, n, seed)
     log(f"[bench] {scenario} {n}^3 initial conditions in {time.perf_counter() - t0:.1f} s")
     def measure(precision):
-        sim = Sim(setup=setup, verbose=False, precision=precision)
+        sim = Sim(fast=False, setup=setup, verbose=False, precision=precision)
         log(f"[bench] {precision}: warmup {a.warmup} steps, timing {a.steps} steps")
         sim.step(a.warmup)
         torch.cuda.synchronize()
@@ -304,7 +326,7 @@ def main():
                 variants[p] = measure(p)
         if a.batch > 0:
             br = measure_batch(a.batch)
-            ref = main_r if a.precision == "fp32" else variants["fp32"]
+            ref = main_r
             br["vs_single_grid"] = {"step": br["value"] / ref["value"],
                                     "hydro_only": br["hydro_only_cell_updates_per_s"] /
                                     ref["hydro_only_cell_updates_per_s"]}
@@ -313,7 +335,7 @@ def main():
     value = main_r["value"]
     achieved = main_r["hydro_achieved_GBps"]
     # the committed counter profile covers the FP32 stage on the 512^3 wall only
-    traffic = measured_traffic() if (a.precision == "fp32" and a.config == 2 and n == 512) else None
+    traffic = measured_traffic(a.precision) if (a.config == 2 and n == 512) else None
 
     cpu = None
     if not a.no_cpu:
@@ -322,8 +344,8 @@ def main():
         log(f"[bench] CPU baseline {cpu['value'] / 1e6:.2f} M cell-updates/s")
 
     out = {
-        "metric": (f"cell-updates/s at {n}^3 (FP32 step) and % of HBM peak" if a.precision == "fp32"
-                   else f"cell-updates/s at {n}^3 ({a.precision} step) and % of HBM peak"),
+        "metric": (f"cell-updates/s at {n}^3 (FP32 step) and % of HBM peak" if a.precision != "fp16-vel"
+                   else f"cell-updates/s at {n}^3 (FP16-storage step) and % of HBM peak"),
         "value": value,
         "unit": "cell-updates/s",
         "n_gpus": 1,
@@ -346,7 +368,7 @@ def main():
         "hydro_only_cell_updates_per_s": main_r["hydro_only_cell_updates_per_s"],
         "kernels_ms": main_r["kernels_ms"],
         "ionization_roofline": main_r["ionization_roofline"],
-        "mass_budget": main_r.get("mass_budget"),
+        "mass_budget": main_r.get("mass_budget") or variants.get("fp32", {}).get("mass_budget"),
         "autotune": autotune.report(),
         "variants": variants,
         "cpu_baseline": cpu,

@@ -109,10 +109,10 @@ def tune_batch(batch, pulses, candidates=CANDIDATES, reps=3):
         return _cache[key]["choice"]
 
     def measure(tile):
+        # the stage-1 plan of this tile alone (not SimBatch._stage_plans, which would also tune the dispatch form
+        # for a tile that may not win)
         batch.tile = tile
-        batch._stage_key = None
-        batch._stage_plans(pulses)
-        plan = batch._plans_stage[0]
+        plan = batch._stage_plan(1, pulses)
         return _time(lambda: _lib.call("nv_hydro_stage_batch", *plan.args, batch._stream), batch.device, reps)
 
     best = _pick(key, measure, candidates)

@@ -62,9 +62,10 @@ class SimBatch:
         if dispatch not in ("tune", "hardware", "queue"):
             raise ValueError("dispatch must be 'tune', 'hardware' or 'queue'")
         self.dispatch = dispatch if dispatch == "tune" else {1: dispatch == "queue", 2: dispatch == "queue"}
-        self.members = [Sim(setup=st, verbose=verbose, precision=precision, device=device, rays=rays,
+        self.members = [Sim(fast=False, setup=st, verbose=verbose, precision=precision, device=device, rays=rays,
                             ion_every=ion_every) for st in setups]
         m0 = self.members[0]
+        self._check_members()
         for m in self.members:
             if (m.P["solver"], m.P["limiter"]) != (m0.P["solver"], m0.P["limiter"]):
                 raise ValueError("batch members must share the solver and limiter")
@@ -100,6 +101,16 @@ class SimBatch:
         torch.cuda.synchronize(dev)
 
     # ------------------------------------------------------------------ helpers
+    def _check_members(self):
+        """The batch runs the main-phase step only (Sim._main_step, sim.py:1099-1114): a member still in its
+        pre-turbulence phase (Sim._pre_step) or stirred in the main phase (drive_main > 0, sim.py:1109) would
+        silently diverge from Sim, so such members are refused."""
+        for s, m in enumerate(self.members):
+            if m.phase != "main":
+                raise ValueError(f"batch member {s} is in its '{m.phase}' phase: SimBatch steps main-phase members")
+            if m.P.get("drive_main", 0.0) > 0:
+                raise ValueError(f"batch member {s} stirs in the main phase (drive_main > 0): not batched")
+
     @property
     def _stream(self):
         return torch.cuda.current_stream(self.device).cuda_stream
@@ -181,14 +192,19 @@ class SimBatch:
         return _Plan("nv_hydro_stage_batch_plan_size", build, self.n, self.device)
 
     def _stage_plans(self, pulses):
+        # the tile first, then the dispatch form for that tile
         if self.tune:
             from . import autotune
             self.tune = False
             self.tile = autotune.tune_batch(self, pulses)
+            self._stage_key = None
         if self.dispatch == "tune":
             from . import autotune
             self.dispatch = autotune.tune_batch_dispatch(self, pulses)
-        key = tuple(pulses)
+        # everything a plan bakes in: the pulses, the tile, and per member its live wind, its star count and the
+        # addresses of its star and jet tables (Sim.replace_stars may reallocate them)
+        key = (tuple(pulses), self.tile) + tuple(
+            (m.P["wind"], len(m.stars), m.star_arr.data_ptr(), m.jet_arr.data_ptr()) for m in self.members)
         if key == self._stage_key:
             return
         self._plans_stage = [self._stage_plan(1, pulses), self._stage_plan(2, pulses, smax=self.smax.data_ptr())]
@@ -318,6 +334,7 @@ class SimBatch:
 
     def step(self, k=1):
         """advance every member k steps (Sim._main_step order, sim.py:1099-1114)"""
+        self._check_members()
         for m in self.members:
             m._set_star_weights()
         if self.device_clock_ok():
@@ -359,6 +376,7 @@ class SimBatch:
             m.dt = dts[s]
             m.it += 1
             m.version += 1
+            m._smax_ok = False             # the member's own cached CFL max predates this step
         self.it += 1
         self.t_dev.copy_(torch.tensor([m.t for m in self.members], dtype=torch.float64))
 

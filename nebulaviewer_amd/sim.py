@@ -50,9 +50,20 @@ class Sim:
 
     PRECISIONS = ("fp32", "fp32-fast", "fp16-vel")
 
-    def __init__(self, scenario="torus", res=None, params=None, rays=None, fuv_grid=None, ion_every=None,
-                 device="cuda:0", verbose=True, fast=False, setup=None, fused=True, seed=None, precision="fp32",
-                 long_rays="fp64", tile="tune", ray_bricks="auto"):
+    IC_MODES = ("nebula3d", "matched")
+
+    def __init__(self, scenario="torus", res=None, params=None, ic_mode="nebula3d", rays=None, fuv_grid=None,
+                 ion_every=None, device="cuda:0", verbose=True, fast=True, setup=None, fused=True, seed=None,
+                 precision="fp32", long_rays="fp64", tile="tune", ray_bricks="auto"):
+        # the reference's constructor (sim.py:553-565): positional order up to fast, fast=True as the viewer runs it
+        # (1-voxel rays, ionization every >= 3 steps, sim.py:743, 758); bake.py and the benchmark pass fast=False
+        if ic_mode not in self.IC_MODES:
+            raise ValueError(f"ic_mode must be one of {self.IC_MODES}")
+        if ic_mode == "matched":
+            raise ValueError("ic_mode='matched' (the reference's resolution-matched noise for its artistic scenes, "
+                             "sim.py:270-330) is not built: the scenarios here are the seeded nebula3d.grf3 workloads "
+                             "(ic_mode='nebula3d')")
+        self.ic_mode = ic_mode
         if precision not in self.PRECISIONS:
             raise ValueError(f"precision must be one of {self.PRECISIONS}")
         if long_rays not in ("fp64", "fp32"):
@@ -151,6 +162,10 @@ class Sim:
             setup = S.make_setup(scenario or "torus", res, self._seed, params)
         elif params:
             setup.P.update(params)
+        if setup.P.get("trigger", 0) > 0:
+            # the reference forms stars where dense gas collapses (Sim._check_triggers, sim.py:1061-1097, host
+            # NumPy/scipy); that decision is not built, so a run that asks for it is refused rather than run without
+            raise NotImplementedError("star formation (P['trigger'] > 0) is not built")
         self.setup = setup
         self.scenario = setup.name
         self.res = setup.res
@@ -579,6 +594,8 @@ class Sim:
         return time.perf_counter() - t0
 
     def set_param(self, name, value):
+        if name == "trigger" and value > 0:
+            raise NotImplementedError("star formation (P['trigger'] > 0) is not built")
         old = self.P.get(name)
         if isinstance(old, int) and not isinstance(old, bool):
             value = int(round(value))
@@ -596,6 +613,9 @@ class Sim:
         return self.t if self.phase == "main" else self.tp - self.P["pre_t"]
 
     def time_label(self):
+        """the viewer's clock label (sim.py:1213-1216)"""
+        if self.phase == "pre":
+            return f"pre-turb {self.tp:.3f}/{self.P['pre_t']:.2f}"
         return f"t={self.t:.4f}"
 
     def get_state_meta(self):

@@ -163,7 +163,7 @@ def test_timeline_capture_restore_and_branch(env):
     from nebulaviewer_amd.sim import Sim
     from nebulaviewer_amd.timeline import Timeline
     say("[timeline] capture / restore / branch on a torus 32^3")
-    sim = Sim(setup=S.make_setup("torus", 32), verbose=False, rays=(32, 64), fuv_grid=(16, 16, 32))
+    sim = Sim(fast=False, setup=S.make_setup("torus", 32), verbose=False, rays=(32, 64), fuv_grid=(16, 16, 32))
     tl = Timeline(budget_gb=1.0)
     tl.clear(sim)
     tl.dtau = 1e-9                    # a snapshot after every step
@@ -197,7 +197,7 @@ def test_bake_frames(env, tmp_path):
     from nebulaviewer_amd.sim import Sim
     from nebulaviewer_amd.bake import bake
     say("[bake] 3 frames of a torus 24^3")
-    sim = Sim(setup=S.make_setup("torus", 24), verbose=False, rays=(32, 64), fuv_grid=(16, 16, 32))
+    sim = Sim(fast=False, setup=S.make_setup("torus", 24), verbose=False, rays=(32, 64), fuv_grid=(16, 16, 32))
     t_end = 5 * 0.3 / 24 / 12         # a handful of CFL steps
     meta = bake(sim, str(tmp_path), t_end, 3)
     files = sorted(os.listdir(tmp_path / "frames"))

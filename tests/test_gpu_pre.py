@@ -87,7 +87,7 @@ def test_sim_through_pre_turbulence_matches_oracle(env):
     say("[pre] Sim torus 24^3 with a turbulent phase, 9 steps, vs OracleSim")
     params = dict(pre_t=0.16, drive_main=0.5)   # 5 pre-turbulence steps, then 4 ionized ones
     st = S.make_setup("torus", 24, params=params)
-    sim = Sim(setup=st, verbose=False, rays=(32, 64), fuv_grid=(16, 16, 32))
+    sim = Sim(fast=False, setup=st, verbose=False, rays=(32, 64), fuv_grid=(16, 16, 32))
     ref = OracleSim(S.make_setup("torus", 24, params=params), S.constants(st), (32, 64), (16, 16, 32), np.float32,
                     threads=8)
     sim.step(9)

@@ -110,7 +110,7 @@ def run_torus(env, precision, steps, n=64):
     from nebulaviewer_amd import scenarios as S
     from nebulaviewer_amd.sim import Sim
     say(f"[sim] torus {n}^3 {precision} {steps} steps")
-    sim = Sim(setup=S.make_setup("torus", n), verbose=False, precision=precision)
+    sim = Sim(fast=False, setup=S.make_setup("torus", n), verbose=False, precision=precision)
     sim.step(steps)
     return sim
 
@@ -163,7 +163,7 @@ def test_forms_against_fp64_per_config(env, scenario, n, steps):
     ref = OracleSim(setup, S.constants(setup), S.ray_grid(n), S.fuv_grid(n), np.float64, threads=16)
     ref.step(steps)
     for prec in ("fp32", "fp32-fast", "fp16-vel"):
-        sim = Sim(setup=S.make_setup(scenario, n), verbose=False, precision=prec)
+        sim = Sim(fast=False, setup=S.make_setup(scenario, n), verbose=False, precision=prec)
         sim.step(steps)
         errs = {name: rel_l1(getattr(sim, name).cpu().numpy(), getattr(ref, name)) for name in ("rho", "mx", "my", "mz")}
         frac, nf = front_disagreement(sim.xi.cpu().numpy(), ref.xi)
@@ -183,13 +183,13 @@ def test_half_velocity_wall_512_300_steps(env):
     torch = env[0]
     say("[fp16-vel vs fp32] wall 512^3, 300 steps")
     setup = S.make_setup("wall", 512)
-    ref = Sim(setup=setup, verbose=False, precision="fp32")
+    ref = Sim(fast=False, setup=setup, verbose=False, precision="fp32")
     ref.step(300)
     r_ref = ref.rho.cpu().numpy()
     xi_ref = ref.xi.cpu().numpy()
     del ref
     torch.cuda.empty_cache()
-    half = Sim(setup=setup, verbose=False, precision="fp16-vel")
+    half = Sim(fast=False, setup=setup, verbose=False, precision="fp16-vel")
     half.step(300)
     e = rel_l1(half.rho.cpu().numpy(), r_ref)
     say(f"  rel-L1 rho fp16-vel vs fp32 after 300 steps = {e:.3e}")
