@@ -5,7 +5,8 @@
  *     z fastest, the layout of sim.py:721-733), grid sizes as ints and a hipStream_t passed as void*;
  *   - launches asynchronously on that stream and returns NV_OK (0) or a nonzero code; nv_last_error()
  *     then holds a message for the calling thread. Nothing throws across this boundary;
- *   - is stateless (no allocations, no globals written) and reentrant per stream.
+ *   - is stateless (no allocations, no globals written) and reentrant per stream; the one exception is
+ *     nv_cast_rays_noalloc, which stages its data in a static device array of the library (see there).
  * Scalars follow the reference launch semantics: the caller computes them in double (Python floats) and
  * passes them rounded to float, exactly as a Warp launch rounds its float arguments.
  *
@@ -218,6 +219,17 @@ int nv_ray_bricks(const float* rho, int nx, int ny, int nz, float* bricks, void*
 int nv_cast_rays_ex(const float* rho, const float* bricks, int nx, int ny, int nz, float sx, float sy, float sz,
                     float Q, float dr, int nsteps, float dx, float rcap, float bx, float by, float bz, float* rfront,
                     int nu, int nphi, int wide, void* stream);
+
+/* nv_cast_rays_ex with bricks, allocating nothing: the brick array is built in a static device array of the library
+ * (128 MiB per GPU, held until the process ends; grids whose nv_ray_bricks_size exceeds it are rejected with
+ * NV_ERR_ARG) instead of a caller buffer, then sampled by the same casts, so rfront is bit-identical to
+ * nv_ray_bricks + nv_cast_rays_ex (and to nv_cast_rays / nv_cast_rays_wide). No hipMalloc, hipFree or allocating
+ * library call. The array is shared by every call on one GPU: calls that overlap there (other streams or threads)
+ * race. No counterpart in the reference: it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc). */
+int nv_cast_rays_noalloc(const float* rho, int nx, int ny, int nz, float sx, float sy, float sz, float Q, float dr,
+                         int nsteps, float dx, float rcap, float bx, float by, float bz, float* rfront, int nu,
+                         int nphi, int wide, void* stream);
 
 /* Batched ray casts: every (grid, star) job in one launch, each job's rfront bit-identical to nv_cast_rays
  * (nv_cast_rays_wide when wide != 0) on that job alone. Jobs with bricks get their grid's brick array built first,

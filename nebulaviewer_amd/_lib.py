@@ -29,6 +29,7 @@ SIGNATURES = {
     "nv_ray_bricks_size": [I, I, I],
     "nv_ray_bricks": [P, I, I, I, P, P],
     "nv_cast_rays_ex": [P, P, I, I, I, F, F, F, F, F, I, F, F, F, F, F, P, I, I, I, P],
+    "nv_cast_rays_noalloc": [P, I, I, I, F, F, F, F, F, I, F, F, F, F, F, P, I, I, I, P],
     "nv_apply_front": [P, I, I, F, F, F, F, F, P, P, I, I, I, P],
     "nv_smooth_xi": [P, P, I, I, I, P],
     "nv_to_half": [P, P, LL, P],

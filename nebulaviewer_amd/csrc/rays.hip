@@ -11,6 +11,7 @@
 // nv_fuv_apply accumulates every star into a register before the single store.
 // Transcendentals come from nv_math.h so the CPU oracle reproduces them bit for bit.
 #include <string.h>
+#include <mutex>
 #include <type_traits>
 #include "nv_common.h"
 #include "nv_math.h"
@@ -771,6 +772,65 @@ extern "C" int nv_cast_rays_ex(const float* rho, const float* bricks, int nx, in
                                float bz, float* rfront, int nu, int nphi, int wide, void* stream) {
   return cast_rays_launch("nv_cast_rays_ex", wide != 0, rho, bricks, nx, ny, nz, sx, sy, sz, Q, dr, nsteps, dx, rcap,
                           bx, by, bz, rfront, nu, nphi, stream);
+}
+
+// ------------------------------------------------------------------------------------------ no-allocation cast
+// nv_cast_rays_noalloc builds the density's brick array in `scratch`, a static device array of this library (one
+// per GPU, held until the process ends), instead of a caller buffer: for callers that cannot allocate device memory.
+// 128 MiB holds the bricks of grids up to about 200^3 (a 160^3 grid needs 64.1 MiB). Host code never passes the
+// array's name to a kernel: each device's address comes from hipGetSymbolAddress on the first call with that device
+// current, and is kept per device. Every brick the casts read is written by the same call's brick build, so nothing
+// needs clearing; two calls that overlap on one GPU (other streams or threads) share the array and race.
+namespace nv {
+constexpr long long kNoAllocBrickBytes = 128LL << 20;
+static __device__ __align__(256) float4 scratch[kNoAllocBrickBytes / 16];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static float4* g_scratch_ptr[kMaxDevices];
+
+// this device's address of `scratch`, looked up once per device (with the device current)
+static int scratch_of_current_device(const char* name, float4** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+    set_error("%s: no current device (or device id %d above %d)", name, dev, kMaxDevices - 1);
+    return NV_ERR_LAUNCH;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_error("%s: hipGetSymbolAddress(scratch) on device %d failed: %s", name, dev, hipGetErrorString(e));
+      return NV_ERR_LAUNCH;
+    }
+    g_scratch_ptr[dev] = static_cast<float4*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return NV_OK;
+}
+}  // namespace nv
+
+extern "C" int nv_cast_rays_noalloc(const float* rho, int nx, int ny, int nz, float sx, float sy, float sz, float Q,
+                                    float dr, int nsteps, float dx, float rcap, float bx, float by, float bz,
+                                    float* rfront, int nu, int nphi, int wide, void* stream) {
+  using namespace nv;
+  clear_error();
+  NV_REQUIRE(rho && rfront, "nv_cast_rays_noalloc: null pointer");
+  NV_REQUIRE(nx > 0 && ny > 0 && nz > 0 && nu > 0 && nphi > 0, "nv_cast_rays_noalloc: bad sizes");
+  const long long need = nv_ray_bricks_size(nx, ny, nz);
+  NV_REQUIRE(need <= kNoAllocBrickBytes,
+             "nv_cast_rays_noalloc: the %dx%dx%d grid's brick array needs %lld bytes, above the library's %lld-byte "
+             "static scratch (nv_ray_bricks + nv_cast_rays_ex take a caller buffer of any size)",
+             nx, ny, nz, need, kNoAllocBrickBytes);
+  float4* B = nullptr;
+  const int rc = scratch_of_current_device("nv_cast_rays_noalloc", &B);
+  if (rc != NV_OK) return rc;
+  hipLaunchKernelGGL(ray_bricks_kernel, dim3((unsigned)(ny + 1), (unsigned)(nx + 2)), dim3(256), 0, as_stream(stream),
+                     rho, B, nx, ny, nz);
+  if (check_launch("nv_cast_rays_noalloc") != NV_OK) return NV_ERR_LAUNCH;
+  return cast_rays_launch("nv_cast_rays_noalloc", wide != 0, rho, reinterpret_cast<const float*>(B), nx, ny, nz, sx,
+                          sy, sz, Q, dr, nsteps, dx, rcap, bx, by, bz, rfront, nu, nphi, stream);
 }
 
 extern "C" int nv_apply_front(const float* rfront, int nu, int nphi, float sx, float sy, float sz, float dx,
