@@ -248,17 +248,22 @@ __device__ __forceinline__ float4 face_flux(const float4& Lq, const float4& Rq, 
 #pragma unroll
     for (int c = 0; c < 4; ++c) F[c] = AR::mad(-ha, UR[c] - UL[c], 0.5f * (FL[c] + FR[c]));
   } else if constexpr (AR::FAST) {
-    // SL clamped to <= 0 and SR to >= 0: the one HLL formula then gives FL where SL >= 0 and FR where SR <= 0
-    // (to the rounding of SR FL / SR), with no three-way select. SR - SL is 0 only for c = 0 and unL = unR = 0,
-    // where every term of the numerator is 0 as well.
+    // The HLL flux (SR FL - SL FR + SL SR (UR - UL)) / (SR - SL) as wR FL + wL FR + s (UR - UL) with
+    // wR = SR / (SR - SL), wL = -SL / (SR - SL), s = SL SR / (SR - SL); since FL = UL unL (+ rL c^2 on the normal
+    // component) and UL = rL (1, vL), every component is F_c = L_c bL + R_c bR (L_0 = R_0 = 1) with
+    // bL = rL (wR unL - s) >= 0 and bR = rR (wL unR + s) <= 0, plus wR rL c^2_b + wL rR c^2_c on the normal one.
+    // SL clamped to <= 0 and SR to >= 0 make the one formula give FL where SL >= 0 (wR = 1, wL = s = 0) and FR
+    // where SR <= 0, with no three-way select; SR - SL > 0 since c > 0.
     const float SL = fminf(fminf(unL - cb, unR - cc), 0.0f);
     const float SR = fmaxf(fmaxf(unL + cb, unR + cc), 0.0f);
-    const float ss = SL * SR;
-    const typename AR::Rcp rden = AR::rcp(fmaxf(SR - SL, 1.17549435e-38f));
-    float N[4];
+    const float inv = AR::rcp(fmaxf(SR - SL, 1.17549435e-38f));
+    const float wR = SR * inv, wL = -SL * inv;
+    const float sd = SL * wR;
+    const float bL = rL * AR::mad(wR, unL, -sd), bR = rR * AR::mad(wL, unR, sd);
+    F[0] = bL + bR;
 #pragma unroll
-    for (int c = 0; c < 4; ++c) N[c] = AR::mad(ss, UR[c] - UL[c], AR::mad(SR, FL[c], -(SL * FR[c])));
-    AR::template divn<4>(N, rden, F);
+    for (int c = 1; c < 4; ++c) F[c] = AR::mad(L[c], bL, R[c] * bR);
+    F[AX + 1] = AR::mad(wR * rL, c2b, AR::mad(wL * rR, c2c, F[AX + 1]));
   } else {
     const float SL = fminf(unL - cb, unR - cc);
     const float SR = fmaxf(unL + cb, unR + cc);
@@ -319,6 +324,27 @@ __device__ __forceinline__ void stage_tile(const HydroArgs& A, int tile) {
   const long long col = (long long)min(j, ny - 1) * nz + min(k, nz - 1);
   const float floor_ = A.floor_, cn2 = A.cn2, dci = A.ci2 - A.cn2;
   const float dt = A.dt_dev ? *A.dt_dev : A.dt;
+  // The switched-on stars (weight > 0) and, among them, those with an active jet (strength > 0), one bit per star:
+  // lane l of each wavefront looks at star l once per tile, so a cell loops over these stars only, in the same
+  // ascending order, instead of loading and testing every star's table entries (stars that fail the test add
+  // nothing). Above 64 stars the cells test every star themselves.
+  bool on_l = false, jet_l = false;
+  {
+    const int l = __lane_id();
+    if (l < A.nstars) {
+      on_l = A.stars[l].w > 0.0f;
+      jet_l = on_l && A.jets[l].w > 0.0f;
+    }
+  }
+  const unsigned long long star_bits = __ballot(on_l), jet_bits = __ballot(jet_l);
+  const bool all_stars = A.nstars > 64;
+  auto each_star = [&](unsigned long long bits, auto&& fn) {
+    if (all_stars) {
+      for (int s = 0; s < A.nstars; ++s) fn(s);
+    } else {
+      for (unsigned long long m = bits; m; m &= m - 1) fn(__builtin_ctzll(m));
+    }
+  };
 
   auto load_col = [&](int p, float4& raw, float& x) {
     const long long o = (long long)clampi(p, 0, nx - 1) * plane + col;
@@ -526,7 +552,7 @@ __device__ __forceinline__ void stage_tile(const HydroArgs& A, int tile) {
       if (A.wind > 0.0f && x_ion > 0.01f) {
         // radiation pressure / wind on ionized gas (sim.py:463-473)
         float acc0 = 0.0f, acc1 = 0.0f, acc2 = 0.0f;
-        for (int s = 0; s < A.nstars; ++s) {
+        each_star(star_bits, [&](int s) {
           const float4 st = A.stars[s];
           if (st.w > 0.0f) {
             const float r0 = p0 - st.x, r1 = p1 - st.y, r2v = p2 - st.z;
@@ -542,7 +568,7 @@ __device__ __forceinline__ void stage_tile(const HydroArgs& A, int tile) {
             acc1 = acc1 + r1 * f;
             acc2 = acc2 + r2v * f;
           }
-        }
+        });
         const float wx = A.wind * x_ion;
         dU[1] = dU[1] + U[0] * (acc0 * wx);
         dU[2] = dU[2] + U[0] * (acc1 * wx);
@@ -551,7 +577,7 @@ __device__ __forceinline__ void stage_tile(const HydroArgs& A, int tile) {
       // bipolar jets (sim.py:474-485); where no cone contributes the term is +0 and adding it is skipped
       float aj0 = 0.0f, aj1 = 0.0f, aj2 = 0.0f;
       bool jet_hit = false;
-      for (int s = 0; s < A.nstars; ++s) {
+      each_star(jet_bits, [&](int s) {
         const float4 jt = A.jets[s];
         const float4 st = A.stars[s];
         if (jt.w > 0.0f && st.w > 0.0f) {
@@ -572,7 +598,7 @@ __device__ __forceinline__ void stage_tile(const HydroArgs& A, int tile) {
             }
           }
         }
-      }
+      });
       if (jet_hit) {
         dU[1] = dU[1] + U[0] * aj0;
         dU[2] = dU[2] + U[0] * aj1;
