@@ -475,9 +475,9 @@ __device__ __forceinline__ void stage_tile(const HydroArgs& A, int tile) {
 
     // ---- B: edge states. Own cell in y and z; the tile's outer neighbours for the boundary faces.
     float4 loy, loz, hi;
-    edges<AR, LM>(sq[cq - PK], sq[cq], sq[cq + PK], loy, hi);
+    edges<AR, LM>(sq[cq - PK], qw[0], sq[cq + PK], loy, hi);   // the own cell is qw[0], which sq[cq] holds
     ehy[(ty + 1) * TK + tx] = hi;
-    edges<AR, LM>(sq[cq - 1], sq[cq], sq[cq + 1], loz, hi);
+    edges<AR, LM>(sq[cq - 1], qw[0], sq[cq + 1], loz, hi);
     ehz[ty * (TK + 1) + tx + 1] = hi;
     {
       float4 lo_, hi_;
@@ -538,15 +538,33 @@ __device__ __forceinline__ void stage_tile(const HydroArgs& A, int tile) {
         div[c] = ((comp(fRx, c) - comp(fLx, c)) + (comp(fRy, c) - comp(fLy, c))) + (comp(fRz, c) - comp(fLz, c));
       const float4 Uc = Uraw[0];
       const float U[4] = {Uc.x, Uc.y, Uc.z, Uc.w};
-      float dU[4];
-      if (A.inv_dx != 0.0f) {
+      // T = Un (sim.py:486) being formed: the fast policy starts from U - (dt / dx) div with one scale and adds
+      // dt rho a for each source; the exact one forms dU = -div / dx plus rho a per source, then U + dt dU
+      float T[4];
+      if constexpr (AR::FAST) {
+        const float kdt = -dt * (A.inv_dx != 0.0f ? A.inv_dx : rdx);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) T[c] = AR::mad(div[c], kdt, U[c]);
+      } else if (A.inv_dx != 0.0f) {
         // dx = 2^e: -div * 2^-e is the same real number as -div / dx, so one rounding gives the same float
 #pragma unroll
-        for (int c = 0; c < 4; ++c) dU[c] = -div[c] * A.inv_dx;
+        for (int c = 0; c < 4; ++c) T[c] = -div[c] * A.inv_dx;
       } else {
         const float nd[4] = {-div[0], -div[1], -div[2], -div[3]};
-        AR::template divn<4>(nd, rdx, dU);
+        AR::template divn<4>(nd, rdx, T);
       }
+      auto add_source = [&](float a0, float a1, float a2) {   // a momentum source rho (a0, a1, a2)
+        if constexpr (AR::FAST) {
+          const float dtr = dt * U[0];
+          T[1] = AR::mad(dtr, a0, T[1]);
+          T[2] = AR::mad(dtr, a1, T[2]);
+          T[3] = AR::mad(dtr, a2, T[3]);
+        } else {
+          T[1] = T[1] + U[0] * a0;
+          T[2] = T[2] + U[0] * a1;
+          T[3] = T[3] + U[0] * a2;
+        }
+      };
       const float p0 = ((float)i + 0.5f) * dx, p1 = ((float)j + 0.5f) * dx, p2 = ((float)k + 0.5f) * dx;
       const float x_ion = xraw[0];
       if (A.wind > 0.0f && x_ion > 0.01f) {
@@ -570,9 +588,7 @@ __device__ __forceinline__ void stage_tile(const HydroArgs& A, int tile) {
           }
         });
         const float wx = A.wind * x_ion;
-        dU[1] = dU[1] + U[0] * (acc0 * wx);
-        dU[2] = dU[2] + U[0] * (acc1 * wx);
-        dU[3] = dU[3] + U[0] * (acc2 * wx);
+        add_source(acc0 * wx, acc1 * wx, acc2 * wx);
       }
       // bipolar jets (sim.py:474-485); where no cone contributes the term is +0 and adding it is skipped
       float aj0 = 0.0f, aj1 = 0.0f, aj2 = 0.0f;
@@ -599,24 +615,43 @@ __device__ __forceinline__ void stage_tile(const HydroArgs& A, int tile) {
           }
         }
       });
-      if (jet_hit) {
-        dU[1] = dU[1] + U[0] * aj0;
-        dU[2] = dU[2] + U[0] * aj1;
-        dU[3] = dU[3] + U[0] * aj2;
-      }
+      if (jet_hit) add_source(aj0, aj1, aj2);
+      float Un[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) Un[c] = AR::FAST ? T[c] : AR::mad(dt, T[c], U[c]);
 
       float out[4];
       if constexpr (STAGE1) {
 #pragma unroll
-        for (int c = 0; c < 4; ++c) out[c] = AR::mad(dt, dU[c], U[c]);
+        for (int c = 0; c < 4; ++c) out[c] = Un[c];
       } else {
         const float4 U0c = conserved<ST>(U0raw);
         const float U0v[4] = {A.u0_is_u ? U[0] : U0c.x, A.u0_is_u ? U[1] : U0c.y, A.u0_is_u ? U[2] : U0c.z,
                               A.u0_is_u ? U[3] : U0c.w};
 #pragma unroll
-        for (int c = 0; c < 4; ++c) out[c] = AR::mad(A.a, U0v[c], A.b * AR::mad(dt, dU[c], U[c]));
+        for (int c = 0; c < 4; ++c) out[c] = AR::mad(A.a, U0v[c], A.b * Un[c]);
       }
       const float r = fmaxf(out[0], floor_);
+      if constexpr (AR::FAST && !ST::VEL) {
+        // The stored momentum is r (m / r) = m, so m is stored as it is unless |m|^2 may exceed (vmax r)^2; the
+        // threshold sits a little below vmax^2 so that every cell the cap can touch takes the exact test.
+        float m0 = out[1], m1 = out[2], m2 = out[3];
+        const float s2m = AR::mad(m2, m2, AR::mad(m1, m1, m0 * m0));
+        if (s2m > (A.vmax2_lo * 0.99999f) * (r * r)) {
+          const Rcp rc = AR::rcp(r);
+          const float v0 = m0 * rc, v1 = m1 * rc, v2 = m2 * rc;
+          const float sp = AR::sqrt(AR::mad(v2, v2, AR::mad(v1, v1, v0 * v0)));
+          const float f = sp > A.vmax ? A.vmax * AR::rcp(sp) : 1.0f;   // the cap of sim.py:491-494
+          m0 = r * (v0 * f);
+          m1 = r * (v1 * f);
+          m2 = r * (v2 * f);
+        }
+        ((float*)A.o0)[o] = r;
+        ((float*)A.o1)[o] = m0;
+        ((float*)A.o2)[o] = m1;
+        ((float*)A.o3)[o] = m2;
+        if constexpr (SMAX) tmax = fmaxf(tmax, ((fabsf(m0) + fabsf(m1)) + fabsf(m2)) * AR::rcp(r) + cw[0]);
+      } else {
       const Rcp rr = AR::rcp(r);
       const float mo[3] = {out[1], out[2], out[3]};
       float vo[3];
@@ -639,6 +674,7 @@ __device__ __forceinline__ void stage_tile(const HydroArgs& A, int tile) {
         // sound speed sqrt(cn2 + xi (ci2 - cn2)), which is cw[0]
         const float s = ((fabsf(vs.x) + fabsf(vs.y)) + fabsf(vs.z)) + cw[0];
         tmax = fmaxf(tmax, s);
+      }
       }
     }
     fLx = fRx;
