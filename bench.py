@@ -303,7 +303,8 @@ def main():
         r = dict(value=cells * a.steps / wall, ms_per_step=wall / a.steps * 1e3, hydro_launch_ms=hyd_s * 1e3,
                  hydro_only_cell_updates_per_s=cells / (2 * hyd_s), hydro_achieved_GBps=achieved,
                  kernels_ms={k: round(v["avg_ms"], 4) for k, v in ks.items()}, ion_every=sim.ion_every,
-                 rays=tuple(sim.rfronts.shape[1:]), nstars=len(sim.stars), tile=autotune.TILES.get(sim.tile_shape),
+                 rays=tuple(sim.rfronts.shape[1:]), nstars=len(sim.stars),
+                 tile={f"stage{k + 1}": autotune.TILES.get(t) for k, t in enumerate(sim.tile_shape)},
                  ray_bricks=getattr(sim, "ray_bricks_used", None),
                  ionization_roofline=ionization_roofline(ks, cells, sum(1 for st in sim.stars if sim.t >= st[4]),
                                                          precision))

@@ -15,7 +15,7 @@ the casts are bound by the cache lines their gathers touch (512^3: 3.2 ms of cas
 build), and lose on small ones, whose density stays in the caches anyway (the 64-member batch of 48^3-160^3 grids:
 2.7 ms direct, 3.0 ms with bricks), so that choice is measured too, on the first ionization.
 
-    tile = tune_sim(sim)                 # Sim(tile="tune") does this on its first RK stage
+    t1, t2 = tune_sim(sim)               # Sim(tile="tune") does this on its first RK stage
     tile = tune_batch(batch, pulses)     # SimBatch(tile="tune") does this when it builds its stage plans
     use = tune_rays(sim, nstars, casts)  # Sim(ray_bricks="auto") on its first ionization
     report()                             # {signature: {"choice": name, "ms": {name: ms, ...}}}
@@ -76,29 +76,31 @@ def _pick(key, measure, candidates, rounds=2):
 
 
 def tune_sim(sim, candidates=CANDIDATES, reps=5):
-    """The fastest tile for this Sim's grid, precision, solver and limiter. Times the two stage forms the step uses
-    (the stage-1 form and the general a U0 + b Un form with the fused CFL reduction), both writing only the free
-    scratch block sim.stage and a scratch maximum, so the simulation state is untouched."""
+    """The fastest tile of each stage form for this Sim's grid, precision, solver and limiter: (stage-1 tile,
+    stage-2 tile). Stage 1 is timed as the step runs it (U -> the free scratch block sim.stage); stage 2 with the
+    step's traffic (reads U as U1 and the scratch block as U0, writes the scratch block in place, fused CFL
+    maximum into a scratch float), so the simulation state is untouched."""
     solver, lim = sim._solver_lim()
-    key = ("sim", tuple(sim.shape), sim.precision, solver, lim)
-    if key in _cache:
-        return _cache[key]["choice"]
+    keys = [("sim", tuple(sim.shape), sim.precision, solver, lim, st) for st in (1, 2)]
+    if all(k in _cache for k in keys):
+        return tuple(_cache[k]["choice"] for k in keys)
     code = PRECISION_CODES[sim.precision]
     scratch = torch.zeros(1, dtype=torch.float32, device=sim.device)
     U = [t.data_ptr() for t in sim.U]
     U1 = [sim.stage[c].data_ptr() for c in range(4)]
-    d1 = sim._stage_desc(U, U, U1, 0.0, 1.0, 0.0, len(sim.stars), sim.P["wind"], 1.0, None)
-    d2 = sim._stage_desc(U, U, U1, 0.5, 0.5, 0.0, len(sim.stars), sim.P["wind"], 1.0, scratch.data_ptr())
+    descs = {1: sim._stage_desc(U, U, U1, 0.0, 1.0, 0.0, len(sim.stars), sim.P["wind"], 1.0, None),
+             2: sim._stage_desc(U, U1, U1, 0.5, 0.5, 0.0, len(sim.stars), sim.P["wind"], 1.0, scratch.data_ptr())}
+    out = []
+    for st, key in zip((1, 2), keys):
+        d = descs[st]
 
-    def measure(tile):
-        strm = sim._stream
+        def measure(tile, d=d):
+            strm = sim._stream
+            return _time(lambda: _lib.call("nv_hydro_stage_ex", ctypes.byref(d), code, solver, lim, tile, strm),
+                         sim.device, reps)
 
-        def launch():
-            for d in (d1, d2):
-                _lib.call("nv_hydro_stage_ex", ctypes.byref(d), code, solver, lim, tile, strm)
-        return _time(launch, sim.device, reps) / 2
-
-    return _pick(key, measure, candidates)
+        out.append(_pick(key, measure, candidates))
+    return tuple(out)
 
 
 def tune_batch(batch, pulses, candidates=CANDIDATES, reps=3):

@@ -105,21 +105,42 @@ struct Fast {
 // ------------------------------------------------------------------------------------------ storage
 __device__ __forceinline__ float h2f(uint16_t h) { return (float)__builtin_bit_cast(_Float16, h); }
 
+// The stage's outputs and its U0 are touched by one cell only, so they are stored and loaded non-temporally: they
+// do not take L2 lines from the state and xi planes, which neighbouring tiles read again as their halos.
+#ifndef NV_NT
+#define NV_NT 1
+#endif
+template <class T>
+__device__ __forceinline__ void st_once(T* p, long long o, T v) {
+  if constexpr (NV_NT) __builtin_nontemporal_store(v, p + o);
+  else p[o] = v;
+}
+template <class T>
+__device__ __forceinline__ T ld_once(const T* p, long long o) {
+  if constexpr (NV_NT) return __builtin_nontemporal_load(p + o);
+  else return p[o];
+}
+
 struct F32 {
   static constexpr bool VEL = false;   // stores momentum
   static __device__ __forceinline__ float4 load(const void* f0, const void* f1, const void* f2, const void* f3,
                                                 long long o) {
     return make_float4(((const float*)f0)[o], ((const float*)f1)[o], ((const float*)f2)[o], ((const float*)f3)[o]);
   }
+  static __device__ __forceinline__ float4 load_once(const void* f0, const void* f1, const void* f2, const void* f3,
+                                                     long long o) {
+    return make_float4(ld_once((const float*)f0, o), ld_once((const float*)f1, o), ld_once((const float*)f2, o),
+                       ld_once((const float*)f3, o));
+  }
   // stores r and r*v; returns the velocity the CFL reduction sees (stored m / stored rho)
   template <class AR>
   static __device__ __forceinline__ float3 store(void* o0, void* o1, void* o2, void* o3, long long o, float r,
                                                  typename AR::Rcp rr, float v0, float v1, float v2, bool want_v) {
     const float m0 = r * v0, m1 = r * v1, m2 = r * v2;
-    ((float*)o0)[o] = r;
-    ((float*)o1)[o] = m0;
-    ((float*)o2)[o] = m1;
-    ((float*)o3)[o] = m2;
+    st_once((float*)o0, o, r);
+    st_once((float*)o1, o, m0);
+    st_once((float*)o2, o, m1);
+    st_once((float*)o3, o, m2);
     if (!want_v) return make_float3(0.0f, 0.0f, 0.0f);
     const float m[3] = {m0, m1, m2};
     float v[3];
@@ -135,14 +156,19 @@ struct F16V {
     return make_float4(((const float*)f0)[o], h2f(((const uint16_t*)f1)[o]), h2f(((const uint16_t*)f2)[o]),
                        h2f(((const uint16_t*)f3)[o]));
   }
+  static __device__ __forceinline__ float4 load_once(const void* f0, const void* f1, const void* f2, const void* f3,
+                                                     long long o) {
+    return make_float4(ld_once((const float*)f0, o), h2f(ld_once((const uint16_t*)f1, o)),
+                       h2f(ld_once((const uint16_t*)f2, o)), h2f(ld_once((const uint16_t*)f3, o)));
+  }
   template <class AR>
   static __device__ __forceinline__ float3 store(void* o0, void* o1, void* o2, void* o3, long long o, float r,
                                                  typename AR::Rcp, float v0, float v1, float v2, bool) {
     const uint16_t h0 = f32_to_f16_bits(v0), h1 = f32_to_f16_bits(v1), h2 = f32_to_f16_bits(v2);
-    ((float*)o0)[o] = r;
-    ((uint16_t*)o1)[o] = h0;
-    ((uint16_t*)o2)[o] = h1;
-    ((uint16_t*)o3)[o] = h2;
+    st_once((float*)o0, o, r);
+    st_once((uint16_t*)o1, o, h0);
+    st_once((uint16_t*)o2, o, h1);
+    st_once((uint16_t*)o3, o, h2);
     return make_float3(h2f(h0), h2f(h1), h2f(h2));   // the velocity the next step will read
   }
 };
@@ -438,7 +464,7 @@ __device__ __forceinline__ void stage_tile(const HydroArgs& A, int tile) {
     load_col(i + 3, rnext, xnext);
     float4 U0raw = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     const long long o = (long long)i * plane + col;
-    if (!STAGE1 && !A.u0_is_u && active) U0raw = ST::load(A.g0, A.g1, A.g2, A.g3, o);
+    if (!STAGE1 && !A.u0_is_u && active) U0raw = ST::load_once(A.g0, A.g1, A.g2, A.g3, o);
 
     // ---- A: plane i into LDS (own cell from the window, halo from global)
     const int cq = (ty + 2) * PK + tx + 2;
@@ -646,10 +672,10 @@ __device__ __forceinline__ void stage_tile(const HydroArgs& A, int tile) {
           m1 = r * (v1 * f);
           m2 = r * (v2 * f);
         }
-        ((float*)A.o0)[o] = r;
-        ((float*)A.o1)[o] = m0;
-        ((float*)A.o2)[o] = m1;
-        ((float*)A.o3)[o] = m2;
+        st_once((float*)A.o0, o, r);
+        st_once((float*)A.o1, o, m0);
+        st_once((float*)A.o2, o, m1);
+        st_once((float*)A.o3, o, m2);
         if constexpr (SMAX) tmax = fmaxf(tmax, ((fabsf(m0) + fabsf(m1)) + fabsf(m2)) * AR::rcp(r) + cw[0]);
       } else {
       const Rcp rr = AR::rcp(r);

@@ -19,8 +19,8 @@ precision: "fp32" reproduces the reference's float32 step bit for bit; "fp32-fas
 hardware reciprocals / square roots in the RK stage; "fp16-vel" stores float32 density with binary16 velocity
 (58 instead of 88 bytes of state traffic per cell and step) and then mx, my, mz are computed on access.
 
-tile, ray_bricks: launch choices that change no bit: the RK stage's workgroup tile ("tune": measured on the first
-stage, or an NV_TILE_* code) and whether the ray casts sample the density through its brick array ("auto":
+tile, ray_bricks: launch choices that change no bit: the RK stage's workgroup tile ("tune": measured per stage form
+on the first stage, or an NV_TILE_* code for both) and whether the ray casts sample the density through its brick array ("auto":
 measured on the first ionization, True, False); nebulaviewer_amd.autotune.
 
 long_rays: "fp64" (default) accumulates the recombination column of a ray in FP64 when the ray's chord through
@@ -545,9 +545,9 @@ class Sim:
         solver, lim = self._solver_lim()
         if self.tile == "tune":
             from . import autotune
-            self.tile_shape = autotune.tune_sim(self)
+            self.tile_shape = autotune.tune_sim(self)          # one tile per stage form
         else:
-            self.tile_shape = self.tile
+            self.tile_shape = (self.tile, self.tile)
         code = {"fp32": 0, "fp32-fast": 1, "fp16-vel": 2}[self.precision]
         strm = self._stream
         U = [self._p(t) for t in self.U]
@@ -555,11 +555,11 @@ class Sim:
         d1 = self._stage_desc(U, U, U1, 0.0, 1.0, dt, nstars, wind, pulse, None)
         d2 = self._stage_desc(U1, U, U, 0.5, 0.5, dt, nstars, wind, pulse, self._p(self.smax))
         tk = self._t0("hydro_stage")
-        _lib.call("nv_hydro_stage_ex", ctypes.byref(d1), code, solver, lim, self.tile_shape, strm)
+        _lib.call("nv_hydro_stage_ex", ctypes.byref(d1), code, solver, lim, self.tile_shape[0], strm)
         self._t1(tk)
         self.smax.zero_()
         tk = self._t0("hydro_stage")
-        _lib.call("nv_hydro_stage_ex", ctypes.byref(d2), code, solver, lim, self.tile_shape, strm)
+        _lib.call("nv_hydro_stage_ex", ctypes.byref(d2), code, solver, lim, self.tile_shape[1], strm)
         self._t1(tk)
         self._smax_ok = True
 
