@@ -112,7 +112,7 @@ def test_autotuned_tiles_change_no_bit(torch_dev):
         assert runs["tune"].t == runs[tile].t
     assert len(runs["tune"].tile_shape) == 2 and all(t in (1, 2, 3) for t in runs["tune"].tile_shape)
     rep = autotune.report()
-    keys = [k for k in rep if "'sim', (40, 40, 40)" in k]
+    keys = [k for k in rep if "'sim', (40, 40, 40), 'fp32', " in k]
     assert len(keys) == 2 and all(set(rep[k]["ms"]) == {"64x8", "32x16", "64x16"} for k in keys)
     b = SimBatch(setups(), rays=RAYS)
     b.step(3)
