@@ -181,6 +181,7 @@ def copy_calibration(nbytes):
 
 def make_measure_batch(a, S, KernelTimer):
     import torch
+    from nebulaviewer_amd import autotune
 
     def measure_batch(count):
         from nebulaviewer_amd.batch import SimBatch
@@ -208,7 +209,8 @@ def make_measure_batch(a, S, KernelTimer):
                  hydro_only_cell_updates_per_s=cells / (2 * hyd_s),
                  hydro_achieved_GBps=BYTES_PER_CELL_STEP[a.precision] / 2 * cells / hyd_s / 1e9,
                  kernels_ms={k: round(v["avg_ms"], 4) for k, v in ks.items()}, graph_replay=True,
-                 tile={1: "64x8", 2: "32x16", 3: "64x16"}.get(b.tile), ray_bricks=getattr(b, "ray_bricks_used", None),
+                 tile={f"stage{k}": autotune.TILES.get(t) for k, t in b.tile.items()},
+                 ray_bricks=getattr(b, "ray_bricks_used", None),
                  dispatch={f"stage{k}": ("queue" if v else "hardware") for k, v in b.dispatch.items()})
         log(f"[bench] batch: {r['value'] / 1e9:.2f} G cell-updates/s ({cells / 1e6:.1f} M cells, {r['stars']} stars), "
             f"{r['ms_per_step']:.3f} ms/step, hydro {hyd_s * 1e3:.3f} ms/launch; kernels {r['kernels_ms']}")

@@ -16,7 +16,7 @@ build), and lose on small ones, whose density stays in the caches anyway (the 64
 2.7 ms direct, 3.0 ms with bricks), so that choice is measured too, on the first ionization.
 
     t1, t2 = tune_sim(sim)               # Sim(tile="tune") does this on its first RK stage
-    tile = tune_batch(batch, pulses)     # SimBatch(tile="tune") does this when it builds its stage plans
+    tiles = tune_batch(batch, pulses)    # SimBatch(tile="tune") does this when it builds its stage plans
     use = tune_rays(sim, nstars, casts)  # Sim(ray_bricks="auto") on its first ionization
     report()                             # {signature: {"choice": name, "ms": {name: ms, ...}}}
 """
@@ -104,22 +104,42 @@ def tune_sim(sim, candidates=CANDIDATES, reps=5):
 
 
 def tune_batch(batch, pulses, candidates=CANDIDATES, reps=3):
-    """The fastest tile for a SimBatch's member grids: builds the stage-1 plan of each candidate and times its
-    launch (it writes only the members' scratch blocks)."""
-    key = ("batch", tuple(tuple(m.shape) for m in batch.members), batch.precision, batch.solver, batch.lim)
-    if key in _cache:
-        return _cache[key]["choice"]
+    """The fastest tile of each stage form for a SimBatch's member grids: {1: tile, 2: tile}. Stage 1 is timed on
+    its own plan (it writes only the members' scratch blocks); stage 2 on a plan with the step's traffic (U as U1,
+    the scratch blocks as U0) whose outputs go to a scratch buffer, so the state is untouched."""
+    shapes = tuple(tuple(m.shape) for m in batch.members)
+    keys = {st: ("batch", shapes, batch.precision, batch.solver, batch.lim, st) for st in (1, 2)}
+    if all(k in _cache for k in keys.values()):
+        return {st: _cache[k]["choice"] for st, k in keys.items()}
+    bufs, scratch = _scratch_outputs(batch)
+    smax = torch.zeros(batch.n, dtype=torch.float32, device=batch.device)
+    out = {}
+    for st, key in keys.items():
+        def measure(tile, st=st):
+            batch.tile = {1: tile, 2: tile}
+            plan = batch._stage_plan(st, pulses, scratch=scratch if st == 2 else None,
+                                     smax=smax.data_ptr() if st == 2 else None)
+            return _time(lambda: _lib.call("nv_hydro_stage_batch", *plan.args, batch._stream), batch.device, reps)
 
-    def measure(tile):
-        # the stage-1 plan of this tile alone (not SimBatch._stage_plans, which would also tune the dispatch form
-        # for a tile that may not win)
-        batch.tile = tile
-        plan = batch._stage_plan(1, pulses)
-        return _time(lambda: _lib.call("nv_hydro_stage_batch", *plan.args, batch._stream), batch.device, reps)
-
-    best = _pick(key, measure, candidates)
+        out[st] = _pick(key, measure, candidates)
+    torch.cuda.synchronize(batch.device)
+    del bufs
     batch._stage_key = None
-    return best
+    return out
+
+
+def _scratch_outputs(batch):
+    """per member, four output pointers into a scratch buffer of 16 B per cell (float32 fields at 4 B, binary16
+    velocities at 2 B per cell); the buffers are returned so the caller keeps them alive"""
+    bufs = [torch.empty(4 * m.shape[0] * m.shape[1] * m.shape[2], dtype=torch.float32, device=batch.device)
+            for m in batch.members]
+    scratch = []
+    for b, m in zip(bufs, batch.members):
+        n = b.numel() // 4
+        base = b.data_ptr()
+        scratch.append([base, base + 4 * n, base + 8 * n, base + 12 * n] if not m.vel_state else
+                       [base, base + 4 * n, base + 6 * n, base + 8 * n])
+    return bufs, scratch
 
 
 def tune_rays(sim, nstars, casts, reps=2):
@@ -148,18 +168,10 @@ def tune_batch_dispatch(batch, pulses, reps=3):
         return {1: False, 2: False}
     shapes = tuple(tuple(m.shape) for m in batch.members)
     out = {}
-    bufs = [torch.empty(4 * m.shape[0] * m.shape[1] * m.shape[2], dtype=torch.float32, device=batch.device)
-            for m in batch.members]
+    bufs, scratch = _scratch_outputs(batch)
     smax = torch.zeros(batch.n, dtype=torch.float32, device=batch.device)
-    scratch = []
-    for b, m in zip(bufs, batch.members):
-        n = b.numel() // 4
-        base = b.data_ptr()
-        # float32 fields at 4 B, binary16 velocities at 2 B per cell, packed in the 16 B/cell buffer
-        scratch.append([base, base + 4 * n, base + 8 * n, base + 12 * n] if not m.vel_state else
-                       [base, base + 4 * n, base + 6 * n, base + 8 * n])
     for stage in (1, 2):
-        key = ("batch_dispatch", shapes, batch.precision, batch.tile, stage)
+        key = ("batch_dispatch", shapes, batch.precision, batch.tile[stage], stage)
         plan = batch._stage_plan(stage, pulses, scratch=scratch, smax=smax.data_ptr())
 
         def measure(queue, plan=plan):

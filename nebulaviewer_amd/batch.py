@@ -52,10 +52,10 @@ class SimBatch:
             raise ValueError("a batch needs at least one member")
         self.device = torch.device(device)
         self.precision = precision
-        # workgroup tile shape of the stage: "tune" (measured, nebulaviewer_amd.autotune), 0 (the library's rule),
-        # or 1 / 2 / 3 (64x8, 32x16, 64x16); no shape changes a bit of the result
+        # workgroup tile shape of each stage form: "tune" (measured per stage, nebulaviewer_amd.autotune), 0 (the
+        # library's rule), or 1 / 2 / 3 (64x8, 32x16, 64x16) for both; no shape changes a bit of the result
         self.tune = tile == "tune"
-        self.tile = 0 if self.tune else tile
+        self.tile = {1: 0, 2: 0} if self.tune else {1: tile, 2: tile}
         self.ray_bricks = "auto"  # ray casts through density bricks: True, False or measured ("auto")
         # how the stage's tiles reach the CUs, per stage form: "hardware" (one workgroup per tile), "queue" (a
         # persistent grid fed by a tile work queue, HLL + minmod only) or measured ("tune")
@@ -188,7 +188,7 @@ class SimBatch:
 
         def build(buf, nbytes):
             _lib.call("nv_hydro_stage_batch_plan", ctypes.cast(descs, P), self.n, code, self.solver, self.lim,
-                      self.tile, buf, nbytes)
+                      self.tile[stage], buf, nbytes)
         return _Plan("nv_hydro_stage_batch_plan_size", build, self.n, self.device)
 
     def _stage_plans(self, pulses):
@@ -196,14 +196,14 @@ class SimBatch:
         if self.tune:
             from . import autotune
             self.tune = False
-            self.tile = autotune.tune_batch(self, pulses)
+            self.tile = autotune.tune_batch(self, pulses)     # {stage: tile}
             self._stage_key = None
         if self.dispatch == "tune":
             from . import autotune
             self.dispatch = autotune.tune_batch_dispatch(self, pulses)
         # everything a plan bakes in: the pulses, the tile, and per member its live wind, its star count and the
         # addresses of its star and jet tables (Sim.replace_stars may reallocate them)
-        key = (tuple(pulses), self.tile) + tuple(
+        key = (tuple(pulses), tuple(self.tile.items())) + tuple(
             (m.P["wind"], len(m.stars), m.star_arr.data_ptr(), m.jet_arr.data_ptr()) for m in self.members)
         if key == self._stage_key:
             return

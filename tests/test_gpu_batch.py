@@ -116,7 +116,7 @@ def test_autotuned_tiles_change_no_bit(torch_dev):
     assert len(keys) == 2 and all(set(rep[k]["ms"]) == {"64x8", "32x16", "64x16"} for k in keys)
     b = SimBatch(setups(), rays=RAYS)
     b.step(3)
-    assert b.tile in (1, 2, 3)
+    assert set(b.tile) == {1, 2} and all(t in (1, 2, 3) for t in b.tile.values())
     check_members(b, 3, "fp32")
 
 
@@ -294,4 +294,5 @@ def test_batch_dispatch_tuned_for_the_chosen_tile(torch_dev):
     keys = [k for k in autotune.report() if "batch_dispatch" in k]
     assert len(keys) == 2
     for k in keys:
-        assert f"'fp32', {b.tile}, " in k, (k, b.tile)
+        stage = int(k.rstrip(")").split(",")[-1])
+        assert f"'fp32', {b.tile[stage]}, {stage})" in k, (k, b.tile)
