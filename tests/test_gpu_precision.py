@@ -194,3 +194,38 @@ def test_half_velocity_wall_512_300_steps(env):
     e = rel_l1(half.rho.cpu().numpy(), r_ref)
     say(f"  rel-L1 rho fp16-vel vs fp32 after 300 steps = {e:.3e}")
     assert e <= 1e-3
+
+
+def test_forms_against_fp64_wall_512():
+    """Configuration 2 at its full 512^3: 10 steps (five ionizations) of each form against the FP64 oracle run of the
+    same steps, every field, on the host's cores. Bars: fp32 and fp32-fast within the drop-in tolerance (1e-5
+    rel-L1 on rho and momentum, 1e-4 of front cells); fp16-vel within 1e-3 on rho."""
+    import os
+    import torch
+    from nebulaviewer_amd import scenarios as S
+    from nebulaviewer_amd.sim import Sim
+    from oracle.oracle import OracleSim
+    steps, n = 10, 512
+    setup = S.make_setup("wall", n)
+    threads = min(16, os.cpu_count() or 1)
+    say(f"[wall 512^3 vs FP64] FP64 oracle, {steps} steps on {threads} host threads")
+    ref = OracleSim(setup, S.constants(setup), S.ray_grid(n), S.fuv_grid(n), np.float64, threads=threads)
+    for s in range(steps):
+        ref.step(1)
+        say(f"  FP64 step {s + 1}/{steps}")
+    ref_xi = np.asarray(ref.xi)
+    for prec in ("fp32", "fp32-fast", "fp16-vel"):
+        sim = Sim(fast=False, setup=S.make_setup("wall", n), verbose=False, precision=prec)
+        sim.step(steps)
+        errs = {name: rel_l1(getattr(sim, name).cpu().numpy(), getattr(ref, name)) for name in ("rho", "mx", "my", "mz")}
+        frac, nf = front_disagreement(sim.xi.cpu().numpy(), ref_xi)
+        say(f"  {prec:9s} rel-L1 " + " ".join(f"{k} {v:.2e}" for k, v in errs.items()) +
+            f"; xi front disagreement {frac:.2e} of {nf}")
+        if prec == "fp16-vel":
+            assert errs["rho"] <= 1e-3, errs
+        else:
+            for name, e in errs.items():
+                assert e <= 1e-5, (prec, name, e)
+            assert frac <= 1e-4, (prec, frac, nf)
+        del sim
+        torch.cuda.empty_cache()
