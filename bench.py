@@ -211,7 +211,8 @@ def make_measure_batch(a, S, KernelTimer):
                  kernels_ms={k: round(v["avg_ms"], 4) for k, v in ks.items()}, graph_replay=True,
                  tile={f"stage{k}": autotune.TILES.get(t) for k, t in b.tile.items()},
                  ray_bricks=getattr(b, "ray_bricks_used", None),
-                 dispatch={f"stage{k}": ("queue" if v else "hardware") for k, v in b.dispatch.items()})
+                 dispatch={f"stage{k}": (f"queue, {b.queue_wgs[k] or 'all'} workgroups/CU" if v else "hardware")
+                           for k, v in b.dispatch.items()})
         log(f"[bench] batch: {r['value'] / 1e9:.2f} G cell-updates/s ({cells / 1e6:.1f} M cells, {r['stars']} stars), "
             f"{r['ms_per_step']:.3f} ms/step, hydro {hyd_s * 1e3:.3f} ms/launch; kernels {r['kernels_ms']}")
         del b

@@ -159,6 +159,11 @@ int nv_hydro_stage_batch(const void* plan_host, const void* plan_dev, void* stre
  * when a run is exhausted; the last workgroup resets the counters). Bit-identical to nv_hydro_stage_batch; two
  * launches of one plan must not overlap (one stream). No reference counterpart (the reference has no batching). */
 int nv_hydro_stage_batch_queue(const void* plan_host, const void* plan_dev, void* stream);
+/* The persistent launch with its residency given: wgs_per_cu workgroups per CU (the grid's waves per CU are
+ * wgs_per_cu times the tile's wavefronts), capped at what the kernel's registers and LDS allow; 0 for that maximum,
+ * which is nv_hydro_stage_batch_queue. Bit-identical to nv_hydro_stage_batch at every residency. A launch choice for
+ * the batch tuner; no reference counterpart. */
+int nv_hydro_stage_batch_queue_ex(const void* plan_host, const void* plan_dev, int wgs_per_cu, void* stream);
 
 /* Mass budget in FP64 on the GPU (the north star's global conservation sums): out[0] += sum of rho over the grid,
  * out[1] += the net outward mass flux through the six domain faces, each face's float flux (the one the exact

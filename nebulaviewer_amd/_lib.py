@@ -43,6 +43,7 @@ SIGNATURES = {
     "nv_hydro_stage_batch_plan": [P, I, I, I, I, I, P, LL],
     "nv_hydro_stage_batch": [P, P, P],
     "nv_hydro_stage_batch_queue": [P, P, P],
+    "nv_hydro_stage_batch_queue_ex": [P, P, I, P],
     "nv_cfl_dt": [P, P, P, P, P, P, I, P],
     "nv_conservation_sums": [P] * 5 + [I, I, I, F, F, F, I, P, P],
     "nv_cast_rays_batch_plan_size": [I],

@@ -35,7 +35,7 @@ _cache = {}
 
 NAMES = {"sim": TILES.get, "batch": TILES.get, "rays": {False: "direct", True: "bricks"}.get,
          "batch_rays": {False: "direct", True: "bricks"}.get,
-         "batch_dispatch": {False: "hardware", True: "queue"}.get}
+         "batch_dispatch": lambda c: "hardware" if not c[0] else ("queue" if c[1] == 0 else f"queue-{c[1]}wg/CU")}
 
 
 def report():
@@ -159,13 +159,18 @@ def tune_batch_rays(batch, plans, reps=2):
     return _pick(key, measure, (False, True))
 
 
-def tune_batch_dispatch(batch, pulses, reps=3):
-    """{1: use_queue, 2: use_queue}: for each stage form, whether the persistent tile-queue launch
-    (nv_hydro_stage_batch_queue) beats one workgroup per tile. Timed on plans whose outputs go to a scratch buffer
-    (16 B per cell and member, freed afterwards), so the state is untouched; the queue form exists for HLL +
-    minmod only."""
+# how a stage's tiles reach the CUs: (persistent queue?, its resident workgroups per CU, 0 = as many as fit)
+DISPATCH_CANDIDATES = ((False, 0), (True, 0), (True, 1))
+
+
+def tune_batch_dispatch(batch, pulses, reps=3, candidates=DISPATCH_CANDIDATES):
+    """{1: (use_queue, wgs_per_cu), 2: ...}: for each stage form, whether the persistent tile-queue launch
+    (nv_hydro_stage_batch_queue_ex) beats one workgroup per tile, and at which residency: the waves per CU of the
+    persistent grid are its workgroups per CU times the tile's wavefronts (16 at full residency for every tile
+    shape, 8 with one 512-thread workgroup per CU). Timed on plans whose outputs go to a scratch buffer (16 B per
+    cell and member, freed afterwards), so the state is untouched; the queue form exists for HLL + minmod only."""
     if (batch.solver, batch.lim) != (1, 0):
-        return {1: False, 2: False}
+        return {1: (False, 0), 2: (False, 0)}
     shapes = tuple(tuple(m.shape) for m in batch.members)
     out = {}
     bufs, scratch = _scratch_outputs(batch)
@@ -174,10 +179,12 @@ def tune_batch_dispatch(batch, pulses, reps=3):
         key = ("batch_dispatch", shapes, batch.precision, batch.tile[stage], stage)
         plan = batch._stage_plan(stage, pulses, scratch=scratch, smax=smax.data_ptr())
 
-        def measure(queue, plan=plan):
-            entry = "nv_hydro_stage_batch_queue" if queue else "nv_hydro_stage_batch"
-            return _time(lambda: _lib.call(entry, *plan.args, batch._stream), batch.device, reps)
-        out[stage] = _pick(key, measure, (False, True))
+        def measure(c, plan=plan):
+            if c[0]:
+                return _time(lambda: _lib.call("nv_hydro_stage_batch_queue_ex", *plan.args, c[1], batch._stream),
+                             batch.device, reps)
+            return _time(lambda: _lib.call("nv_hydro_stage_batch", *plan.args, batch._stream), batch.device, reps)
+        out[stage] = _pick(key, measure, candidates)
     torch.cuda.synchronize(batch.device)
     del bufs
     return out

@@ -47,7 +47,7 @@ class _Plan:
 
 class SimBatch:
     def __init__(self, setups, precision="fp32", device="cuda:0", rays=None, ion_every=None, graph=False,
-                 verbose=False, tile="tune", dispatch="tune"):
+                 verbose=False, tile="tune", dispatch="tune", queue_wgs=0):
         if not setups:
             raise ValueError("a batch needs at least one member")
         self.device = torch.device(device)
@@ -58,10 +58,15 @@ class SimBatch:
         self.tile = {1: 0, 2: 0} if self.tune else {1: tile, 2: tile}
         self.ray_bricks = "auto"  # ray casts through density bricks: True, False or measured ("auto")
         # how the stage's tiles reach the CUs, per stage form: "hardware" (one workgroup per tile), "queue" (a
-        # persistent grid fed by a tile work queue, HLL + minmod only) or measured ("tune")
+        # persistent grid fed by a tile work queue, HLL + minmod only) or measured ("tune"); queue_wgs: resident
+        # workgroups per CU of the persistent grid (its waves per CU over the tile's wavefronts; 0: as many as the
+        # kernel's registers and LDS allow), measured with the dispatch form when that is tuned
         if dispatch not in ("tune", "hardware", "queue"):
             raise ValueError("dispatch must be 'tune', 'hardware' or 'queue'")
+        if int(queue_wgs) < 0:
+            raise ValueError("queue_wgs must be >= 0")
         self.dispatch = dispatch if dispatch == "tune" else {1: dispatch == "queue", 2: dispatch == "queue"}
+        self.queue_wgs = {1: int(queue_wgs), 2: int(queue_wgs)}
         self.members = [Sim(fast=False, setup=st, verbose=verbose, precision=precision, device=device, rays=rays,
                             ion_every=ion_every) for st in setups]
         m0 = self.members[0]
@@ -200,7 +205,9 @@ class SimBatch:
             self._stage_key = None
         if self.dispatch == "tune":
             from . import autotune
-            self.dispatch = autotune.tune_batch_dispatch(self, pulses)
+            choice = autotune.tune_batch_dispatch(self, pulses)     # {stage: (use the queue, workgroups per CU)}
+            self.dispatch = {st: c[0] for st, c in choice.items()}
+            self.queue_wgs = {st: c[1] for st, c in choice.items()}
         # everything a plan bakes in: the pulses, the tile, and per member its live wind, its star count and the
         # addresses of its star and jet tables (Sim.replace_stars may reallocate them)
         key = (tuple(pulses), tuple(self.tile.items())) + tuple(
@@ -307,9 +314,12 @@ class SimBatch:
     def _launch_hydro(self):
         strm = self._stream
         for stage in (1, 2):
-            entry = "nv_hydro_stage_batch_queue" if self.dispatch[stage] else "nv_hydro_stage_batch"
             tk = self._t0("hydro_stage")
-            _lib.call(entry, *self._plans_stage[stage - 1].args, strm)
+            if self.dispatch[stage]:
+                _lib.call("nv_hydro_stage_batch_queue_ex", *self._plans_stage[stage - 1].args,
+                          self.queue_wgs[stage], strm)
+            else:
+                _lib.call("nv_hydro_stage_batch", *self._plans_stage[stage - 1].args, strm)
             self._t1(tk)
 
     def _device_step(self, ion):

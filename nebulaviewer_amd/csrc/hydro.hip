@@ -1225,9 +1225,13 @@ extern "C" int nv_hydro_stage_batch_plan(const NvStageDesc* members, int n, int 
 
 namespace nv {
 // Hardware-dispatched (one workgroup per tile) or persistent (a queue-fed grid of resident workgroups) launch.
-static int batch_launch(const char* name, const void* plan_host, const void* plan_dev, bool persistent, void* stream) {
+// wgs_per_cu: resident workgroups per CU of the persistent grid (its waves per CU divided by the tile's wavefronts),
+// at most what the kernel's registers and LDS allow; 0 for that maximum.
+static int batch_launch(const char* name, const void* plan_host, const void* plan_dev, bool persistent,
+                        int wgs_per_cu, void* stream) {
   clear_error();
   NV_REQUIRE(plan_host && plan_dev, "%s: null plan", name);
+  NV_REQUIRE(wgs_per_cu >= 0, "%s: wgs_per_cu = %d must be >= 0", name, wgs_per_cu);
   BatchHeader H;
   memcpy(&H, plan_host, sizeof(H));
   NV_REQUIRE(H.magic == kBatchMagic && H.n > 0, "%s: not a stage batch plan", name);
@@ -1256,7 +1260,9 @@ static int batch_launch(const char* name, const void* plan_host, const void* pla
     set_error("%s: cannot query the device's occupancy", name);
     return NV_ERR_LAUNCH;
   }
-  const long long resident = std::max<long long>(8, ((long long)cus * std::max(per_cu, 1)) / 8 * 8);
+  const int fit = std::max(per_cu, 1);
+  const int use = wgs_per_cu > 0 ? std::min(wgs_per_cu, fit) : fit;
+  const long long resident = std::max<long long>(8, ((long long)cus * use) / 8 * 8);
   const long long grid = std::min<long long>(resident, ((long long)H.total + 7) / 8 * 8);
   hipLaunchKernelGGL(kern, dim3((unsigned)grid), block, 0, as_stream(stream), args, start, H.n, H.total, H.per_xcd,
                      queue);
@@ -1265,11 +1271,16 @@ static int batch_launch(const char* name, const void* plan_host, const void* pla
 }  // namespace nv
 
 extern "C" int nv_hydro_stage_batch(const void* plan_host, const void* plan_dev, void* stream) {
-  return nv::batch_launch("nv_hydro_stage_batch", plan_host, plan_dev, false, stream);
+  return nv::batch_launch("nv_hydro_stage_batch", plan_host, plan_dev, false, 0, stream);
 }
 
 extern "C" int nv_hydro_stage_batch_queue(const void* plan_host, const void* plan_dev, void* stream) {
-  return nv::batch_launch("nv_hydro_stage_batch_queue", plan_host, plan_dev, true, stream);
+  return nv::batch_launch("nv_hydro_stage_batch_queue", plan_host, plan_dev, true, 0, stream);
+}
+
+extern "C" int nv_hydro_stage_batch_queue_ex(const void* plan_host, const void* plan_dev, int wgs_per_cu,
+                                             void* stream) {
+  return nv::batch_launch("nv_hydro_stage_batch_queue_ex", plan_host, plan_dev, true, wgs_per_cu, stream);
 }
 
 extern "C" int nv_conservation_sums(const float* rho, const float* mx, const float* my, const float* mz,

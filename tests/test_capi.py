@@ -128,6 +128,8 @@ def test_batch_plans_validate(lib):
     ibuf = ctypes.create_string_buffer(int(lib.nv_ionize_apply_batch_plan_size(1)))
     assert lib.nv_ionize_apply_batch_plan(ions, 1, 0, ibuf, len(ibuf)) == 1001
     assert lib.nv_hydro_stage_batch(buf, buf, None) == 1001     # a refused plan never got its header
+    assert lib.nv_hydro_stage_batch_queue_ex(buf, buf, 0, None) == 1001
+    assert lib.nv_hydro_stage_batch_queue_ex(buf, buf, -1, None) == 1001 and b"wgs_per_cu" in lib.nv_last_error()
 
 
 def test_rejects_null_fields(lib):

@@ -145,15 +145,28 @@ def test_batch_queue_dispatch_matches_single(torch_dev, precision):
     check_members(b, 6, precision)
 
 
+def test_batch_queue_one_workgroup_per_cu_matches_single(torch_dev):
+    """The persistent launch at one resident workgroup per CU (nv_hydro_stage_batch_queue_ex: 8 waves per CU on
+    these 512-thread tiles, half the full residency) gives the members' single-grid steps bit for bit."""
+    from nebulaviewer_amd.batch import SimBatch
+    say("[batch] persistent tile queue at one workgroup per CU")
+    b = SimBatch(setups(), rays=RAYS, dispatch="queue", queue_wgs=1, graph=True)
+    assert b.dispatch == {1: True, 2: True} and b.queue_wgs == {1: 1, 2: 1}
+    b.step(4)
+    check_members(b, 4, "fp32")
+
+
 def test_batch_dispatch_is_tuned(torch_dev):
     from nebulaviewer_amd.batch import SimBatch
     from nebulaviewer_amd import autotune
     say("[batch] dispatch tuning")
     b = SimBatch(setups(), rays=RAYS)
     b.step(2)
-    assert set(b.dispatch) == {1, 2}
+    assert set(b.dispatch) == {1, 2} and set(b.queue_wgs) == {1, 2}
     keys = [k for k in autotune.report() if "batch_dispatch" in k]
     assert len(keys) >= 2
+    for k in keys:   # hardware dispatch, the queue at full residency and at one workgroup per CU, all measured
+        assert set(autotune.report()[k]["ms"]) == {"hardware", "queue", "queue-1wg/CU"}
     check_members(b, 2, "fp32")
 
 

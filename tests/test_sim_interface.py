@@ -68,3 +68,11 @@ def test_star_formation_is_refused():
     s = Sim("torus", res=16, rays=(16, 32), fast=False, verbose=False)
     with pytest.raises(NotImplementedError, match="trigger"):
         s.set_param("trigger", 4.0)
+
+
+def test_batch_residency_is_checked_before_any_device_work():
+    from nebulaviewer_amd.batch import SimBatch
+    with pytest.raises(ValueError, match="queue_wgs"):
+        SimBatch([object()], queue_wgs=-1)
+    with pytest.raises(ValueError, match="dispatch"):
+        SimBatch([object()], dispatch="other")
