@@ -725,10 +725,8 @@ __device__ __forceinline__ void stage_tile(const HydroArgs& A, int tile) {
   if constexpr (SMAX) block_max_atomic(tmax, red, A.smax);
 }
 
-// Workgroups are dealt to the 8 XCDs round robin (id % 8); give each XCD a contiguous run of tiles (z fastest,
-// then y, then the x chunk, then the next member of a batch) so the tiles whose halos overlap run on one XCD and
-// share its L2.
-__device__ __forceinline__ int xcd_tile(int per_xcd) { return (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3); }
+// Each XCD takes a contiguous run of tiles (xcd_tile: z fastest, then y, then the x chunk, then the next member of a
+// batch), so the tiles whose halos overlap run on one XCD and share its L2.
 
 template <class TL, class ST, class AR, int SOLVER, int LIM, bool SMAX, bool STAGE1>
 __global__ __launch_bounds__(TL::NT, 4) void hydro_stage_kernel(HydroArgs A) {

@@ -43,6 +43,11 @@ inline float pow2_reciprocal(float x) {
 // v clamped to [lo, hi] (lo <= hi at every call); one v_med3_i32
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
 
+// Workgroups are dealt to the 8 XCDs round robin (id % 8). A launch of 8 per_xcd workgroups that maps workgroup id to
+// (id % 8) per_xcd + id / 8 gives each XCD a contiguous run of per_xcd work items, so items that share data (the
+// halos of neighbouring tiles, the front-table rows of neighbouring directions) meet in one XCD's L2.
+__device__ __forceinline__ int xcd_tile(int per_xcd) { return (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3); }
+
 // C-order [nx][ny][nz] offset; 64-bit so 1024^3 grids (2^30 cells) index safely
 __device__ __forceinline__ long long cell(int i, int j, int k, int ny, int nz) {
   return ((long long)i * ny + j) * (long long)nz + k;

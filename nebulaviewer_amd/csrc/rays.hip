@@ -529,12 +529,19 @@ __device__ __forceinline__ void ionize_tile(const IonArgs& A, int bk, int bj, in
   if constexpr (SMAX) block_max_atomic(tmax, red, A.smax);
 }
 
+// One tile per workgroup, each XCD a contiguous run of tiles (z fastest, then y, then the x chunk): its cells then
+// see a narrower cone of directions from each star, so the front-table rows they gather stay in the XCD's L2 (the
+// 512^3 wall's pass: 1.07 against 1.15 ms with the tiles dealt round robin; the same bits).
 template <bool SMAX, bool VEL>
-__global__ __launch_bounds__(INT_) void ionize_apply_kernel(IonArgs A) {
-  ionize_tile<SMAX, VEL>(A, blockIdx.x, blockIdx.y, blockIdx.z);
+__global__ __launch_bounds__(INT_) void ionize_apply_kernel(IonArgs A, int per_xcd) {
+  const int t = xcd_tile(per_xcd);
+  if (t >= A.tiles) return;
+  ionize_tile<SMAX, VEL>(A, t % A.tiles_k, (t / A.tiles_k) % A.tiles_j, t / (A.tiles_k * A.tiles_j));
 }
 
-// Many grids in one launch: tile t of the flat list belongs to member s with start[s] <= t < start[s+1].
+// Many grids in one launch: tile t of the flat list belongs to member s with start[s] <= t < start[s+1]. The tiles
+// are dealt round robin to the XCDs: members differ in their star count, so contiguous runs of equal length would
+// give the XCDs unequal work (measured on the 64-member batch: 2.34 ms against 2.04).
 template <bool SMAX, bool VEL>
 __global__ __launch_bounds__(INT_) void ionize_apply_batch_kernel(const IonArgs* __restrict__ args,
                                                                   const int* __restrict__ start, int n, int total) {
@@ -905,10 +912,10 @@ static int ionize_apply_launch(const char* name, bool vel, const float* rfronts,
   const int rc = fill_ionize(name, rfronts, nu, nphi, star_xyz, nstars, dx, width, xi, dfront_h, dfront_f, nx, ny, nz,
                              f0, f1, f2, f3, cn2, ci2, smax_out, ion_jk(ny, nz), A);
   if (rc != NV_OK) return rc;
-  dim3 grid(A.tiles_k, A.tiles_j, (nx + A.ci - 1) / A.ci);
+  const int per_xcd = (A.tiles + 7) / 8;
   auto kern = !smax_out ? ionize_apply_kernel<false, false>
                         : (vel ? ionize_apply_kernel<true, true> : ionize_apply_kernel<true, false>);
-  hipLaunchKernelGGL(kern, grid, dim3(IK, IJ), 0, as_stream(stream), A);
+  hipLaunchKernelGGL(kern, dim3(8 * per_xcd), dim3(IK, IJ), 0, as_stream(stream), A, per_xcd);
   return check_launch(name);
 }
 
