@@ -231,7 +231,7 @@ def make_measure_batch(a, S, KernelTimer):
                  hydro_only_cell_updates_per_s=cells / (2 * hyd_s),
                  hydro_achieved_GBps=BYTES_PER_CELL_STEP[a.precision] / 2 * cells / hyd_s / 1e9,
                  kernels_ms={k: round(v["avg_ms"], 4) for k, v in ks.items()}, graph_replay=True,
-                 tile={f"stage{k}": autotune.TILES.get(t) for k, t in b.tile.items()},
+                 tile={f"stage{k}": autotune.tile_name(t) for k, t in b.tile.items()},
                  ray_bricks=getattr(b, "ray_bricks_used", None),
                  dispatch={f"stage{k}": (f"queue, {b.queue_wgs[k] or 'all'} workgroups/CU" if v else "hardware")
                            for k, v in b.dispatch.items()})
@@ -329,7 +329,7 @@ def main():
                  hydro_only_cell_updates_per_s=cells / (2 * hyd_s), hydro_achieved_GBps=achieved,
                  kernels_ms={k: round(v["avg_ms"], 4) for k, v in ks.items()}, ion_every=sim.ion_every,
                  rays=tuple(sim.rfronts.shape[1:]), nstars=len(sim.stars),
-                 tile={f"stage{k + 1}": autotune.TILES.get(t) for k, t in enumerate(sim.tile_shape)},
+                 tile={f"stage{k + 1}": autotune.tile_name(t) for k, t in enumerate(sim.tile_shape)},
                  ray_bricks=getattr(sim, "ray_bricks_used", None),
                  ionization_roofline=ionization_roofline(ks, cells, sum(1 for st in sim.stars if sim.t >= st[4]),
                                                          precision))

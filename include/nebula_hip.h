@@ -39,6 +39,11 @@ extern "C" {
 #define NV_TILE_64x8 1     /* 64 z x 8 y cells per workgroup (one wavefront per y row) */
 #define NV_TILE_32x16 2    /* 32 z x 16 y cells per workgroup (one wavefront per two rows) */
 #define NV_TILE_64x16 3    /* 64 z x 16 y cells per workgroup (1024 threads, one workgroup per CU) */
+/* A tile code may also carry the march depth: NV_TILE_DEPTH(n) | shape makes each workgroup advance its column
+ * over n x planes (at most the grid's nx; n >= nx marches the whole column in one workgroup); depth 0 is the
+ * library's choice (up to 64 planes, enough workgroups to fill the device). Like the shape it changes the work
+ * decomposition only, never a bit of the result. */
+#define NV_TILE_DEPTH(n) ((n) << 8)
 
 /* One grid of a batched RK stage: the arguments of nv_hydro_stage, with the four fields of U, U0 and the output
  * as arrays (float32, or float32 rho + binary16 velocity for NV_PREC_F16V). dt_dev: if non-null, the stage reads

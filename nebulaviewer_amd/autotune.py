@@ -2,7 +2,8 @@
 
 The stage kernel comes in three tile shapes (NV_TILE_* in include/nebula_hip.h): 64 z x 8 y and 32 z x 16 y
 (8 wavefronts per workgroup, two workgroups per CU) and 64 z x 16 y (16 wavefronts, one workgroup per CU, half
-the halo share). The shape changes the work decomposition only, never a bit of the result. Which one is fastest
+the halo share), each marching its column over the library's depth (up to 64 x planes) or over the whole column
+(NV_TILE_DEPTH). Shape and depth change the work decomposition only, never a bit of the result. Which one is fastest
 depends on the z extent (idle lanes of the last z tile), on how many tiles the launch has to fill 256 CUs with,
 and on how the halo share trades against occupancy, so it is measured rather than guessed: each candidate runs
 the stage on the caller's own fields, timed with HIP events on the launch stream, and the fastest is kept. The
@@ -29,11 +30,29 @@ from . import _lib
 TILES = {1: "64x8", 2: "32x16", 3: "64x16"}
 CANDIDATES = (1, 2, 3)
 PRECISION_CODES = {"fp32": 0, "fp32-fast": 1, "fp16-vel": 2}
+WHOLE_COLUMN = 1 << 15   # NV_TILE_DEPTH(n) with n >= every grid's nx: each workgroup marches its whole column
+
+
+def depth_code(shape, depth):
+    """the NV_TILE_* code of a tile shape (1-3) with a march depth (0: the library's choice)"""
+    return shape | (depth << 8)
+
+
+def tile_name(code):
+    """'64x8', or '64x8/whole' / '64x8/d<n>' when the code carries a march depth"""
+    shape, depth = code & 0xFF, code >> 8
+    name = TILES.get(shape, str(shape))
+    return name if depth == 0 else name + ("/whole" if depth >= WHOLE_COLUMN else f"/d{depth}")
+
+
+# each shape with the library's march depth and with whole-column marches (one workgroup per tile column: the
+# columns of one XCD's run advance together, so the halo lines they share are read while still in its L2)
+SIM_CANDIDATES = tuple(depth_code(t, d) for t in CANDIDATES for d in (0, WHOLE_COLUMN))
 
 _cache = {}
 
 
-NAMES = {"sim": TILES.get, "batch": TILES.get, "rays": {False: "direct", True: "bricks"}.get,
+NAMES = {"sim": tile_name, "batch": tile_name, "rays": {False: "direct", True: "bricks"}.get,
          "batch_rays": {False: "direct", True: "bricks"}.get,
          "batch_dispatch": lambda c: "hardware" if not c[0] else ("queue" if c[1] == 0 else f"queue-{c[1]}wg/CU")}
 
@@ -75,7 +94,7 @@ def _pick(key, measure, candidates, rounds=2):
     return best
 
 
-def tune_sim(sim, candidates=CANDIDATES, reps=5):
+def tune_sim(sim, candidates=SIM_CANDIDATES, reps=5):
     """The fastest tile of each stage form for this Sim's grid, precision, solver and limiter: (stage-1 tile,
     stage-2 tile). Stage 1 is timed as the step runs it (U -> the free scratch block sim.stage); stage 2 with the
     step's traffic (reads U as U1 and the scratch block as U0, writes the scratch block in place, fused CFL
@@ -103,7 +122,7 @@ def tune_sim(sim, candidates=CANDIDATES, reps=5):
     return tuple(out)
 
 
-def tune_batch(batch, pulses, candidates=CANDIDATES, reps=3):
+def tune_batch(batch, pulses, candidates=SIM_CANDIDATES, reps=3):
     """The fastest tile of each stage form for a SimBatch's member grids: {1: tile, 2: tile}. Stage 1 is timed on
     its own plan (it writes only the members' scratch blocks); stage 2 on a plan with the step's traffic (U as U1,
     the scratch blocks as U0) whose outputs go to a scratch buffer, so the state is untouched."""

@@ -105,42 +105,62 @@ struct Fast {
 // ------------------------------------------------------------------------------------------ storage
 __device__ __forceinline__ float h2f(uint16_t h) { return (float)__builtin_bit_cast(_Float16, h); }
 
+// Addressing. Field element (plane po, in-plane cell c) is at f + po + c. Each access goes through a buffer
+// resource whose base is the plane's first element: po is the same for the whole workgroup, so the base is scalar
+// arithmetic, and the cell's byte offset within the plane, cb = 4 c (a 2-byte field's is cb / 2), is the one
+// 32-bit vector operand (ny nz < 2^30). No 64-bit vector address arithmetic per access, and the per-thread offset
+// is one register for every field. The resource's range is the plane (kPlaneBytes4: 4 ny nz bytes of a 4-byte
+// field), which every in-plane offset lies inside.
+constexpr int kBufWord3 = 0x00020000;   // gfx9 buffer resource word 3: 32-bit data, no swizzle
+template <class T>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const void* f, long long po, unsigned bytes4) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(static_cast<const T*>(f) + po), 0,
+                                           (int)(sizeof(T) == 4 ? bytes4 : bytes4 >> 1), kBufWord3);
+}
+// aux: cache policy of the access, 0 (default) or 2 (non-temporal)
+template <class T, int AUX = 0>
+__device__ __forceinline__ T ldp(const void* f, long long po, unsigned cb, unsigned bytes4) {
+  const __amdgpu_buffer_rsrc_t r = plane_rsrc<T>(f, po, bytes4);
+  if constexpr (sizeof(T) == 4) return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, cb, 0, AUX));
+  else return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b16(r, cb >> 1, 0, AUX));
+}
+
 // The stage's outputs and its U0 are touched by one cell only, so they are stored and loaded non-temporally: they
 // do not take L2 lines from the state and xi planes, which neighbouring tiles read again as their halos.
-#ifndef NV_NT
-#define NV_NT 1
-#endif
+constexpr int kNT = 2;
 template <class T>
-__device__ __forceinline__ void st_once(T* p, long long o, T v) {
-  if constexpr (NV_NT) __builtin_nontemporal_store(v, p + o);
-  else p[o] = v;
+__device__ __forceinline__ void st_once(void* f, long long po, unsigned cb, unsigned bytes4, T v) {
+  const __amdgpu_buffer_rsrc_t r = plane_rsrc<T>(f, po, bytes4);
+  if constexpr (sizeof(T) == 4) __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, cb, 0, kNT);
+  else __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(uint16_t, v), r, cb >> 1, 0, kNT);
 }
 template <class T>
-__device__ __forceinline__ T ld_once(const T* p, long long o) {
-  if constexpr (NV_NT) return __builtin_nontemporal_load(p + o);
-  else return p[o];
+__device__ __forceinline__ T ld_once(const void* f, long long po, unsigned cb, unsigned bytes4) {
+  return ldp<T, kNT>(f, po, cb, bytes4);
 }
 
 struct F32 {
   static constexpr bool VEL = false;   // stores momentum
   static __device__ __forceinline__ float4 load(const void* f0, const void* f1, const void* f2, const void* f3,
-                                                long long o) {
-    return make_float4(((const float*)f0)[o], ((const float*)f1)[o], ((const float*)f2)[o], ((const float*)f3)[o]);
+                                                long long po, unsigned cb, unsigned b4) {
+    return make_float4(ldp<float>(f0, po, cb, b4), ldp<float>(f1, po, cb, b4), ldp<float>(f2, po, cb, b4),
+                       ldp<float>(f3, po, cb, b4));
   }
   static __device__ __forceinline__ float4 load_once(const void* f0, const void* f1, const void* f2, const void* f3,
-                                                     long long o) {
-    return make_float4(ld_once((const float*)f0, o), ld_once((const float*)f1, o), ld_once((const float*)f2, o),
-                       ld_once((const float*)f3, o));
+                                                     long long po, unsigned cb, unsigned b4) {
+    return make_float4(ld_once<float>(f0, po, cb, b4), ld_once<float>(f1, po, cb, b4), ld_once<float>(f2, po, cb, b4),
+                       ld_once<float>(f3, po, cb, b4));
   }
   // stores r and r*v; returns the velocity the CFL reduction sees (stored m / stored rho)
   template <class AR>
-  static __device__ __forceinline__ float3 store(void* o0, void* o1, void* o2, void* o3, long long o, float r,
-                                                 typename AR::Rcp rr, float v0, float v1, float v2, bool want_v) {
+  static __device__ __forceinline__ float3 store(void* o0, void* o1, void* o2, void* o3, long long po, unsigned cb,
+                                                 unsigned b4, float r, typename AR::Rcp rr, float v0, float v1,
+                                                 float v2, bool want_v) {
     const float m0 = r * v0, m1 = r * v1, m2 = r * v2;
-    st_once((float*)o0, o, r);
-    st_once((float*)o1, o, m0);
-    st_once((float*)o2, o, m1);
-    st_once((float*)o3, o, m2);
+    st_once<float>(o0, po, cb, b4, r);
+    st_once<float>(o1, po, cb, b4, m0);
+    st_once<float>(o2, po, cb, b4, m1);
+    st_once<float>(o3, po, cb, b4, m2);
     if (!want_v) return make_float3(0.0f, 0.0f, 0.0f);
     const float m[3] = {m0, m1, m2};
     float v[3];
@@ -152,23 +172,24 @@ struct F32 {
 struct F16V {
   static constexpr bool VEL = true;    // stores velocity
   static __device__ __forceinline__ float4 load(const void* f0, const void* f1, const void* f2, const void* f3,
-                                                long long o) {
-    return make_float4(((const float*)f0)[o], h2f(((const uint16_t*)f1)[o]), h2f(((const uint16_t*)f2)[o]),
-                       h2f(((const uint16_t*)f3)[o]));
+                                                long long po, unsigned cb, unsigned b4) {
+    return make_float4(ldp<float>(f0, po, cb, b4), h2f(ldp<uint16_t>(f1, po, cb, b4)),
+                       h2f(ldp<uint16_t>(f2, po, cb, b4)), h2f(ldp<uint16_t>(f3, po, cb, b4)));
   }
   static __device__ __forceinline__ float4 load_once(const void* f0, const void* f1, const void* f2, const void* f3,
-                                                     long long o) {
-    return make_float4(ld_once((const float*)f0, o), h2f(ld_once((const uint16_t*)f1, o)),
-                       h2f(ld_once((const uint16_t*)f2, o)), h2f(ld_once((const uint16_t*)f3, o)));
+                                                     long long po, unsigned cb, unsigned b4) {
+    return make_float4(ld_once<float>(f0, po, cb, b4), h2f(ld_once<uint16_t>(f1, po, cb, b4)),
+                       h2f(ld_once<uint16_t>(f2, po, cb, b4)), h2f(ld_once<uint16_t>(f3, po, cb, b4)));
   }
   template <class AR>
-  static __device__ __forceinline__ float3 store(void* o0, void* o1, void* o2, void* o3, long long o, float r,
-                                                 typename AR::Rcp, float v0, float v1, float v2, bool) {
+  static __device__ __forceinline__ float3 store(void* o0, void* o1, void* o2, void* o3, long long po, unsigned cb,
+                                                 unsigned b4, float r, typename AR::Rcp, float v0, float v1, float v2,
+                                                 bool) {
     const uint16_t h0 = f32_to_f16_bits(v0), h1 = f32_to_f16_bits(v1), h2 = f32_to_f16_bits(v2);
-    st_once((float*)o0, o, r);
-    st_once((uint16_t*)o1, o, h0);
-    st_once((uint16_t*)o2, o, h1);
-    st_once((uint16_t*)o3, o, h2);
+    st_once<float>(o0, po, cb, b4, r);
+    st_once<uint16_t>(o1, po, cb, b4, h0);
+    st_once<uint16_t>(o2, po, cb, b4, h1);
+    st_once<uint16_t>(o3, po, cb, b4, h2);
     return make_float3(h2f(h0), h2f(h1), h2f(h2));   // the velocity the next step will read
   }
 };
@@ -311,6 +332,12 @@ __device__ __forceinline__ float4 face_flux(const float4& Lq, const float4& Rq, 
   return make_float4(F[0], F[1], F[2], F[3]);
 }
 
+// One plane of a cell column's register window: primitives, conserved state, xi, and c^2 and c.
+struct Win {
+  float4 q, U;
+  float x, c2, c;
+};
+
 // Which stage forms load the halo of the next plane one plane ahead (four more registers per thread): stage 1;
 // stage 2 already holds U0 and the CFL max and would spill at its 128-register limit.
 template <bool STAGE1>
@@ -347,7 +374,8 @@ __device__ __forceinline__ void stage_tile(const HydroArgs& A, int tile) {
   const bool active = (k < nz) && (j < ny);
   const long long plane = (long long)ny * nz;
   // own column, clamped (threads past the domain edge still supply the clamped ghost values of ldc)
-  const long long col = (long long)min(j, ny - 1) * nz + min(k, nz - 1);
+  const unsigned cb = 4u * (unsigned)(min(j, ny - 1) * nz + min(k, nz - 1));   // its byte offset in a plane
+  const unsigned pb4 = 4u * (unsigned)plane;                                     // bytes of a float plane
   const float floor_ = A.floor_, cn2 = A.cn2, dci = A.ci2 - A.cn2;
   const float dt = A.dt_dev ? *A.dt_dev : A.dt;
   // The switched-on stars (weight > 0) and, among them, those with an active jet (strength > 0), one bit per star:
@@ -373,9 +401,9 @@ __device__ __forceinline__ void stage_tile(const HydroArgs& A, int tile) {
   };
 
   auto load_col = [&](int p, float4& raw, float& x) {
-    const long long o = (long long)clampi(p, 0, nx - 1) * plane + col;
-    raw = ST::load(A.f0, A.f1, A.f2, A.f3, o);
-    x = A.xi[o];
+    const long long po = (long long)clampi(p, 0, nx - 1) * plane;
+    raw = ST::load(A.f0, A.f1, A.f2, A.f3, po, cb, pb4);
+    x = ldp<float>(A.xi, po, cb, pb4);
   };
 
   // the halo cells' LDS slots and in-plane offsets depend on the tile only: tabulated once for the whole march
@@ -410,33 +438,32 @@ __device__ __forceinline__ void stage_tile(const HydroArgs& A, int tile) {
   }
   __syncthreads();
 
-  // register window of the own column: primitives of planes i, i+1, i+2; c^2, c of planes i, i+1; the
-  // conserved state and xi of planes i, i+1, i+2; the high x edge of plane i; the flux of face i-1/2
-  float4 qw[3], Uraw[3], hix, fLx;
-  float xraw[3], c2w[2], cw[2];
+  // register window of the own column: three slots holding planes i, i+1, i+2 (primitives, conserved state, xi;
+  // c^2 and c of the first two), plus the high x edge of plane i and the flux of face i-1/2.
+  Win w0, w1, w2;
+  float4 hix, fLx;
   {
-    float4 raw, q[4], Uc, lo, hm1;
+    float4 raw, q0, q1, Uc, lo, hm1;
     float x, c2m1, cm1;
-    for (int s = 0; s < 4; ++s) {      // planes i0-2 .. i0+1
-      load_col(i0 - 2 + s, raw, x);
-      decode<ST, AR>(raw, floor_, q[s], Uc);
-      if (s == 1) {
-        c2m1 = AR::mad(x, dci, cn2);
-        cm1 = AR::sqrt(c2m1);
-      } else if (s >= 2) {
-        Uraw[s - 2] = Uc;
-        xraw[s - 2] = x;
-        c2w[s - 2] = AR::mad(x, dci, cn2);
-        cw[s - 2] = AR::sqrt(c2w[s - 2]);
-      }
-    }
-    load_col(i0 + 2, raw, xraw[2]);
-    qw[0] = q[2];
-    qw[1] = q[3];
-    decode<ST, AR>(raw, floor_, qw[2], Uraw[2]);
-    edges<AR, LM>(q[0], q[1], q[2], lo, hm1);   // plane i0-1
-    edges<AR, LM>(q[1], q[2], q[3], lo, hix);   // plane i0
-    fLx = face_flux<AR, 0, SOLVER>(hm1, lo, c2m1, c2w[0], cm1, cw[0], floor_);
+    load_col(i0 - 2, raw, x);
+    decode<ST, AR>(raw, floor_, q0, Uc);
+    load_col(i0 - 1, raw, x);
+    decode<ST, AR>(raw, floor_, q1, Uc);
+    c2m1 = AR::mad(x, dci, cn2);
+    cm1 = AR::sqrt(c2m1);
+    load_col(i0, raw, w0.x);
+    decode<ST, AR>(raw, floor_, w0.q, w0.U);
+    w0.c2 = AR::mad(w0.x, dci, cn2);
+    w0.c = AR::sqrt(w0.c2);
+    load_col(i0 + 1, raw, w1.x);
+    decode<ST, AR>(raw, floor_, w1.q, w1.U);
+    w1.c2 = AR::mad(w1.x, dci, cn2);
+    w1.c = AR::sqrt(w1.c2);
+    load_col(i0 + 2, raw, w2.x);
+    decode<ST, AR>(raw, floor_, w2.q, w2.U);
+    edges<AR, LM>(q0, q1, w0.q, lo, hm1);     // plane i0-1
+    edges<AR, LM>(q1, w0.q, w1.q, lo, hix);   // plane i0
+    fLx = face_flux<AR, 0, SOLVER>(hm1, lo, c2m1, w0.c2, cm1, w0.c, floor_);
   }
 
   const Rcp rdx = AR::rcp(A.dx);
@@ -451,26 +478,27 @@ __device__ __forceinline__ void stage_tile(const HydroArgs& A, int tile) {
   const bool hq = tid >= NT - NHALO, hc = tid < NCHALO;
   auto halo_load = [&](int p, float4& raw) {
     const long long pb = (long long)p * plane;
-    if (hq) raw = ST::load(A.f0, A.f1, A.f2, A.f3, pb + htab[NT - 1 - tid].y);
-    else if (hc) raw.x = A.xi[pb + ctab[tid].y];
+    if (hq) raw = ST::load(A.f0, A.f1, A.f2, A.f3, pb, 4u * (unsigned)htab[NT - 1 - tid].y, pb4);
+    else if (hc) raw.x = ldp<float>(A.xi, pb, 4u * (unsigned)ctab[tid].y, pb4);
   };
   float4 hraw = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   if constexpr (PF) halo_load(i0, hraw);
 
-  for (int i = i0; i < i1; ++i) {
+  // plane i with the window slots A, B, C holding planes i, i+1, i+2; plane i+3 is loaded into slot A
+  auto plane_step = [&](Win& A_, Win& B_, Win& C_, int i) __attribute__((always_inline)) {
     // next window plane (i+3): issued now, consumed at the end of this iteration
     float4 rnext;
     float xnext;
     load_col(i + 3, rnext, xnext);
     float4 U0raw = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    const long long o = (long long)i * plane + col;
-    if (!STAGE1 && !A.u0_is_u && active) U0raw = ST::load_once(A.g0, A.g1, A.g2, A.g3, o);
+    const long long po = (long long)i * plane;
+    if (!STAGE1 && !A.u0_is_u && active) U0raw = ST::load_once(A.g0, A.g1, A.g2, A.g3, po, cb, pb4);
 
     // ---- A: plane i into LDS (own cell from the window, halo from global)
     const int cq = (ty + 2) * PK + tx + 2;
     const int cc = (ty + 1) * CK + tx + 1;
-    sq[cq] = qw[0];
-    sc[cc] = make_float2(c2w[0], cw[0]);
+    sq[cq] = A_.q;
+    sc[cc] = make_float2(A_.c2, A_.c);
     // halo cells from the last wavefronts up (the first ones take the c^2 halo here and the tile-boundary edges
     // and faces later), so that no wavefront carries two of these jobs between two barriers
     if constexpr (PF) {
@@ -487,12 +515,12 @@ __device__ __forceinline__ void stage_tile(const HydroArgs& A, int tile) {
       for (int h = NT - 1 - tid; h < NHALO; h += NT) {
         const int2 e = htab[h];
         float4 qh, Uh;
-        decode<ST, AR>(ST::load(A.f0, A.f1, A.f2, A.f3, pbase + e.y), floor_, qh, Uh);
+        decode<ST, AR>(ST::load(A.f0, A.f1, A.f2, A.f3, pbase, 4u * (unsigned)e.y, pb4), floor_, qh, Uh);
         sq[e.x] = qh;
       }
       for (int h = tid; h < NCHALO; h += NT) {
         const int2 e = ctab[h];
-        const float c2 = AR::mad(A.xi[pbase + e.y], dci, cn2);
+        const float c2 = AR::mad(ldp<float>(A.xi, pbase, 4u * (unsigned)e.y, pb4), dci, cn2);
         sc[e.x] = make_float2(c2, AR::sqrt(c2));
       }
     }
@@ -501,9 +529,9 @@ __device__ __forceinline__ void stage_tile(const HydroArgs& A, int tile) {
 
     // ---- B: edge states. Own cell in y and z; the tile's outer neighbours for the boundary faces.
     float4 loy, loz, hi;
-    edges<AR, LM>(sq[cq - PK], qw[0], sq[cq + PK], loy, hi);   // the own cell is qw[0], which sq[cq] holds
+    edges<AR, LM>(sq[cq - PK], A_.q, sq[cq + PK], loy, hi);   // the own cell is A_.q, which sq[cq] holds
     ehy[(ty + 1) * TK + tx] = hi;
-    edges<AR, LM>(sq[cq - 1], qw[0], sq[cq + 1], loz, hi);
+    edges<AR, LM>(sq[cq - 1], A_.q, sq[cq + 1], loz, hi);
     ehz[ty * (TK + 1) + tx + 1] = hi;
     {
       float4 lo_, hi_;
@@ -530,8 +558,8 @@ __device__ __forceinline__ void stage_tile(const HydroArgs& A, int tile) {
     }
     // x: the window gives plane i+1's edges; its low edge closes face i+1/2, its high edge is carried
     float4 lox1, hix1;
-    edges<AR, LM>(qw[0], qw[1], qw[2], lox1, hix1);
-    const float4 fRx = face_flux<AR, 0, SOLVER>(hix, lox1, c2w[0], c2w[1], cw[0], cw[1], floor_);
+    edges<AR, LM>(A_.q, B_.q, C_.q, lox1, hix1);
+    const float4 fRx = face_flux<AR, 0, SOLVER>(hix, lox1, A_.c2, B_.c2, A_.c, B_.c, floor_);
     __syncthreads();
 
     // ---- C: the low y and z faces of every cell, and the tile's high y and z boundary faces
@@ -562,7 +590,7 @@ __device__ __forceinline__ void stage_tile(const HydroArgs& A, int tile) {
 #pragma unroll
       for (int c = 0; c < 4; ++c)   // the reference's 0 + ... only turns a -0 sum into +0: dropped
         div[c] = ((comp(fRx, c) - comp(fLx, c)) + (comp(fRy, c) - comp(fLy, c))) + (comp(fRz, c) - comp(fLz, c));
-      const float4 Uc = Uraw[0];
+      const float4 Uc = A_.U;
       const float U[4] = {Uc.x, Uc.y, Uc.z, Uc.w};
       // T = Un (sim.py:486) being formed: the fast policy starts from U - (dt / dx) div with one scale and adds
       // dt rho a for each source; the exact one forms dU = -div / dx plus rho a per source, then U + dt dU
@@ -592,7 +620,7 @@ __device__ __forceinline__ void stage_tile(const HydroArgs& A, int tile) {
         }
       };
       const float p0 = ((float)i + 0.5f) * dx, p1 = ((float)j + 0.5f) * dx, p2 = ((float)k + 0.5f) * dx;
-      const float x_ion = xraw[0];
+      const float x_ion = A_.x;
       if (A.wind > 0.0f && x_ion > 0.01f) {
         // radiation pressure / wind on ionized gas (sim.py:463-473)
         float acc0 = 0.0f, acc1 = 0.0f, acc2 = 0.0f;
@@ -672,11 +700,11 @@ __device__ __forceinline__ void stage_tile(const HydroArgs& A, int tile) {
           m1 = r * (v1 * f);
           m2 = r * (v2 * f);
         }
-        st_once((float*)A.o0, o, r);
-        st_once((float*)A.o1, o, m0);
-        st_once((float*)A.o2, o, m1);
-        st_once((float*)A.o3, o, m2);
-        if constexpr (SMAX) tmax = fmaxf(tmax, ((fabsf(m0) + fabsf(m1)) + fabsf(m2)) * AR::rcp(r) + cw[0]);
+        st_once<float>(A.o0, po, cb, pb4, r);
+        st_once<float>(A.o1, po, cb, pb4, m0);
+        st_once<float>(A.o2, po, cb, pb4, m1);
+        st_once<float>(A.o3, po, cb, pb4, m2);
+        if constexpr (SMAX) tmax = fmaxf(tmax, ((fabsf(m0) + fabsf(m1)) + fabsf(m2)) * AR::rcp(r) + A_.c);
       } else {
       const Rcp rr = AR::rcp(r);
       const float mo[3] = {out[1], out[2], out[3]};
@@ -694,11 +722,11 @@ __device__ __forceinline__ void stage_tile(const HydroArgs& A, int tile) {
           v2 = v2 * f;
         }
       }
-      const float3 vs = ST::template store<AR>(A.o0, A.o1, A.o2, A.o3, o, r, rr, v0, v1, v2, SMAX);
+      const float3 vs = ST::template store<AR>(A.o0, A.o1, A.o2, A.o3, po, cb, pb4, r, rr, v0, v1, v2, SMAX);
       if constexpr (SMAX) {
         // max_speed of the written state (nebula3d.py:154-162): L1 norm of its velocity plus this cell's
-        // sound speed sqrt(cn2 + xi (ci2 - cn2)), which is cw[0]
-        const float s = ((fabsf(vs.x) + fabsf(vs.y)) + fabsf(vs.z)) + cw[0];
+        // sound speed sqrt(cn2 + xi (ci2 - cn2)), which is A_.c
+        const float s = ((fabsf(vs.x) + fabsf(vs.y)) + fabsf(vs.z)) + A_.c;
         tmax = fmaxf(tmax, s);
       }
       }
@@ -706,21 +734,20 @@ __device__ __forceinline__ void stage_tile(const HydroArgs& A, int tile) {
     fLx = fRx;
     hix = hix1;
 
-    // ---- shift the window by one plane
-    float4 Unext;
-    qw[0] = qw[1];
-    qw[1] = qw[2];
-    decode<ST, AR>(rnext, floor_, qw[2], Unext);
-    Uraw[0] = Uraw[1];
-    Uraw[1] = Uraw[2];
-    Uraw[2] = Unext;
-    c2w[0] = c2w[1];
-    cw[0] = cw[1];
-    c2w[1] = AR::mad(xraw[2], dci, cn2);   // plane i+2 becomes plane (i+1)+1
-    cw[1] = AR::sqrt(c2w[1]);
-    xraw[0] = xraw[1];
-    xraw[1] = xraw[2];
-    xraw[2] = xnext;
+    // ---- advance the window by one plane: plane i+2 (slot C) gets its c^2 and c, plane i+3 takes slot A
+    C_.c2 = AR::mad(C_.x, dci, cn2);
+    C_.c = AR::sqrt(C_.c2);
+    decode<ST, AR>(rnext, floor_, A_.q, A_.U);
+    A_.x = xnext;
+  };
+  // The compiler turns the rotation into register moves. (Unrolling the march three ways, one rotation per copy,
+  // removes the moves but measured 4-7% slower per launch at 512^3.)
+  for (int i = i0; i < i1; ++i) {
+    plane_step(w0, w1, w2, i);
+    const Win t = w0;
+    w0 = w1;
+    w1 = w2;
+    w2 = t;
   }
   if constexpr (SMAX) block_max_atomic(tmax, red, A.smax);
 }
@@ -962,7 +989,7 @@ static int auto_tile(int nz) {
 // columns the launch covers (this grid's own, or a whole batch's), which sets the march depth ci so that the
 // launch has about 2048 x-chunks of tiles in all.
 static int fill_stage(const char* name, const NvStageDesc& d, int solver, int lim, int tile, long long jk_total,
-                      HydroArgs& A) {
+                      HydroArgs& A, int depth = 0) {
   NV_REQUIRE(d.u[0] && d.u[1] && d.u[2] && d.u[3] && d.xi && d.u0[0] && d.u0[1] && d.u0[2] && d.u0[3] && d.out[0] &&
                  d.out[1] && d.out[2] && d.out[3],
              "%s: null field pointer", name);
@@ -974,7 +1001,7 @@ static int fill_stage(const char* name, const NvStageDesc& d, int solver, int li
   NV_REQUIRE(d.out[0] != d.u[0] && d.out[1] != d.u[1] && d.out[2] != d.u[2] && d.out[3] != d.u[3],
              "%s: output aliases the input U", name);
   NV_REQUIRE(d.floor_ > 0.0f && d.dx > 0.0f, "%s: floor and dx must be > 0", name);
-  NV_REQUIRE((long long)d.ny * d.nz < (1LL << 31), "%s: ny nz = %lld, at or above 2^31 cells per x plane", name,
+  NV_REQUIRE((long long)d.ny * d.nz < (1LL << 30), "%s: ny nz = %lld, at or above 2^30 cells per x plane", name,
              (long long)d.ny * d.nz);
   A.f0 = d.u[0]; A.f1 = d.u[1]; A.f2 = d.u[2]; A.f3 = d.u[3]; A.xi = d.xi;
   A.g0 = d.u0[0]; A.g1 = d.u0[1]; A.g2 = d.u0[2]; A.g3 = d.u0[3];
@@ -1001,8 +1028,9 @@ static int fill_stage(const char* name, const NvStageDesc& d, int solver, int li
     if ((double)h < j2) h = nextafterf(h, INFINITY);
     A.jet_r2_hi = d.jet_r > 0.0f ? h : -1.0f;
   }
-  // march depth: long enough to amortise the 4-plane prologue, short enough to fill the chip
-  A.ci = (int)std::min<long long>(64, std::max<long long>(4, (long long)d.nx * jk_total / 2048));
+  // march depth: the caller's, or long enough to amortise the 4-plane prologue and short enough to fill the chip
+  A.ci = depth > 0 ? std::min(depth, d.nx)
+                   : (int)std::min<long long>(64, std::max<long long>(4, (long long)d.nx * jk_total / 2048));
   A.tiles_k = (d.nz + tile_tk(tile) - 1) / tile_tk(tile);
   A.tiles_j = (d.ny + tile_tj(tile) - 1) / tile_tj(tile);
   A.tiles = A.tiles_k * A.tiles_j * ((d.nx + A.ci - 1) / A.ci);
@@ -1018,10 +1046,13 @@ static bool is_stage1(const HydroArgs& A) { return A.u0_is_u && A.a == 0.0f && A
 
 static int launch_desc(Mode mode, const char* name, const NvStageDesc& d, int solver, int lim, int tile,
                        void* stream) {
-  NV_REQUIRE(tile >= TILE_AUTO && tile <= TILE_64x16, "%s: bad tile shape %d", name, tile);
+  const int depth = tile >> 8;   // x planes per workgroup (NV_TILE_DEPTH), 0 for the library's choice
+  tile &= 0xff;
+  NV_REQUIRE(tile >= TILE_AUTO && tile <= TILE_64x16 && depth >= 0, "%s: bad tile shape %d (depth %d)", name, tile,
+             depth);
   if (tile == TILE_AUTO) tile = auto_tile(d.nz);
   HydroArgs A;
-  const int rc = fill_stage(name, d, solver, lim, tile, tiles_jk(tile, d.ny, d.nz), A);
+  const int rc = fill_stage(name, d, solver, lim, tile, tiles_jk(tile, d.ny, d.nz), A, depth);
   if (rc != NV_OK) return rc;
   HydroKernel kern = pick_mode(tile, mode, solver, lim, d.smax_out != nullptr, is_stage1(A));
   hipLaunchKernelGGL(kern, dim3(8 * A.per_xcd), dim3(tile_tk(tile), tile_tj(tile), 1), 0, as_stream(stream), A);
@@ -1178,7 +1209,10 @@ extern "C" int nv_hydro_stage_batch_plan(const NvStageDesc* members, int n, int 
              nv::plan_bytes(n), plan_bytes);
   NV_REQUIRE(precision >= NV_PREC_F32 && precision <= NV_PREC_F16V, "nv_hydro_stage_batch_plan: bad precision %d",
              precision);
-  NV_REQUIRE(tile >= TILE_AUTO && tile <= TILE_64x16, "nv_hydro_stage_batch_plan: bad tile shape %d", tile);
+  const int depth = tile >> 8;   // x planes per workgroup (NV_TILE_DEPTH; each member's nx at most)
+  tile &= 0xff;
+  NV_REQUIRE(tile >= TILE_AUTO && tile <= TILE_64x16 && depth >= 0,
+             "nv_hydro_stage_batch_plan: bad tile shape %d (depth %d)", tile, depth);
   long long jk_total = 0;
   for (int s = 0; s < n; ++s) {
     NV_REQUIRE(members[s].nx > 0 && members[s].ny > 0 && members[s].nz > 0,
@@ -1196,7 +1230,7 @@ extern "C" int nv_hydro_stage_batch_plan(const NvStageDesc* members, int n, int 
   for (int s = 0; s < n; ++s) {
     snprintf(name, sizeof(name), "nv_hydro_stage_batch_plan: member %d", s);
     HydroArgs A;
-    const int rc = fill_stage(name, members[s], solver, lim, tile, jk_total, A);
+    const int rc = fill_stage(name, members[s], solver, lim, tile, jk_total, A, depth);
     if (rc != NV_OK) return rc;
     NV_REQUIRE((members[s].smax_out != nullptr) == sm, "%s: smax_out must be set for all members or none", name);
     stage1 = stage1 && is_stage1(A);

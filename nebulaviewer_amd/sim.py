@@ -68,8 +68,9 @@ class Sim:
             raise ValueError(f"precision must be one of {self.PRECISIONS}")
         if long_rays not in ("fp64", "fp32"):
             raise ValueError("long_rays must be 'fp64' or 'fp32'")
-        if tile != "tune" and tile not in (0, 1, 2, 3):
-            raise ValueError("tile must be 'tune' or an NV_TILE_* code (0 = the library's rule)")
+        if tile != "tune" and not (isinstance(tile, int) and tile >= 0 and (tile & 0xFF) in (0, 1, 2, 3)):
+            raise ValueError("tile must be 'tune' or an NV_TILE_* code (0 = the library's rule), optionally with "
+                             "NV_TILE_DEPTH(n) (n << 8)")
         self.precision = precision
         self.long_rays = long_rays
         self.tile = tile

@@ -55,6 +55,18 @@ def test_batch_device_clock_matches_single(torch_dev, precision):
     check_members(b, 5, precision)
 
 
+@pytest.mark.parametrize("tile", [2 | (1 << 15 << 8), 1 | (5 << 8)])
+def test_batch_march_depth_matches_single(torch_dev, tile):
+    """batch stage plans whose workgroups march whole member columns, or 5-plane x chunks (NV_TILE_DEPTH), with
+    both dispatch forms: every member equals its single Sim"""
+    from nebulaviewer_amd.batch import SimBatch
+    for dispatch in ("hardware", "queue"):
+        say(f"[batch] tile code {tile:#x}, {dispatch}")
+        b = SimBatch(setups(), precision="fp32", rays=RAYS, tile=tile, dispatch=dispatch)
+        b.step(4)
+        check_members(b, 4, "fp32")
+
+
 def test_batch_graph_replay_matches_single(torch_dev):
     """every replay of the two captured steps (with and without ionization) against an eager batch stepped
     beside it, with eager work and a synchronization between replays; then against the single Sims"""
@@ -102,21 +114,24 @@ def test_autotuned_tiles_change_no_bit(torch_dev):
     from nebulaviewer_amd import scenarios as S
     say("[autotune] Sim 40^3 torus and a batch, tuned vs fixed tiles")
     runs = {}
-    for tile in ("tune", 0, 1, 2, 3):
+    fixed = (0, 1, 2, 3, 3 | (autotune.WHOLE_COLUMN << 8), 1 | (7 << 8))   # shapes, and shapes with march depths
+    for tile in ("tune",) + fixed:
         sim = Sim(fast=False, setup=S.make_setup("torus", 40, 0), verbose=False, rays=RAYS, tile=tile)
         sim.step(4)
         runs[tile] = sim
-    for tile in (0, 1, 2, 3):
+    for tile in fixed:
         for name in ("rho", "mx", "my", "mz", "xi"):
             same(getattr(runs["tune"], name), getattr(runs[tile], name), f"tile {tile} {name}")
         assert runs["tune"].t == runs[tile].t
-    assert len(runs["tune"].tile_shape) == 2 and all(t in (1, 2, 3) for t in runs["tune"].tile_shape)
+    assert len(runs["tune"].tile_shape) == 2 and all(t & 0xFF in (1, 2, 3) for t in runs["tune"].tile_shape)
     rep = autotune.report()
     keys = [k for k in rep if "'sim', (40, 40, 40), 'fp32', " in k]
-    assert len(keys) == 2 and all(set(rep[k]["ms"]) == {"64x8", "32x16", "64x16"} for k in keys)
+    names = {autotune.tile_name(c) for c in autotune.SIM_CANDIDATES}
+    assert names == {"64x8", "32x16", "64x16", "64x8/whole", "32x16/whole", "64x16/whole"}
+    assert len(keys) == 2 and all(set(rep[k]["ms"]) == names for k in keys)
     b = SimBatch(setups(), rays=RAYS)
     b.step(3)
-    assert set(b.tile) == {1, 2} and all(t in (1, 2, 3) for t in b.tile.values())
+    assert set(b.tile) == {1, 2} and all(t & 0xFF in (1, 2, 3) for t in b.tile.values())
     check_members(b, 3, "fp32")
 
 

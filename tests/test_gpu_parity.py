@@ -110,11 +110,12 @@ def test_hydro_stage_bitwise(env, okern32, shape, solver, lim, mode):
 
 
 @pytest.mark.parametrize("shape", [(12, 9, 14), (33, 40, 96), (9, 35, 160), (17, 24, 64)])
-@pytest.mark.parametrize("tile", [1, 2, 3])
+# shapes 1-3, and shapes with a march depth (NV_TILE_DEPTH: one-plane and three-plane x chunks, whole columns)
+@pytest.mark.parametrize("tile", [1, 2, 3, 1 | (1 << 8), 2 | (3 << 8), 3 | (1 << 23), 1 | (1 << 15)])
 @pytest.mark.parametrize("precision", [0, 2])
 def test_hydro_stage_tile_shapes(env, okern32, shape, tile, precision):
-    """nv_hydro_stage_ex with each workgroup tile shape: bitwise the oracle (FP32 exact), bitwise the automatic
-    shape's result (binary16-velocity form) - the shape changes only the work decomposition"""
+    """nv_hydro_stage_ex with each workgroup tile shape and march depth: bitwise the oracle (FP32 exact), bitwise
+    the automatic shape's result (binary16-velocity form) - shape and depth change only the work decomposition"""
     say(f"[hydro tile] shape={shape} tile={tile} precision={precision}")
     torch, dev, L = env
     dx = F32(1.0 / max(shape))
