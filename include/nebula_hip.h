@@ -64,7 +64,8 @@ typedef struct NvStageDesc {
 /* One ray-cast job of a batch: the arguments of nv_cast_rays (wide != 0: those of nv_cast_rays_wide). bricks:
  * null, or a 16-byte aligned scratch of nv_ray_bricks_size(nx, ny, nz) bytes that nv_cast_rays_batch fills with
  * rho's brick array (as nv_ray_bricks) before the casts sample it; jobs on one grid pass the same rho and bricks,
- * built once. */
+ * built once. dirs: null, or the direction table nv_ray_dirs wrote for this job's (nu, nphi), which jobs with the
+ * same ray grid share (the same bits as the directions the cast computes itself). */
 typedef struct NvRayDesc {
   const float* rho;
   int nx, ny, nz;
@@ -74,10 +75,13 @@ typedef struct NvRayDesc {
   float* rfront;
   int nu, nphi, wide;
   float* bricks;
+  const float* dirs; /* null, or nv_ray_dirs(nu, nphi): the rays' directions, read instead of computed per ray */
+  float* rf_range;   /* null, or nv_ray_range_floats(nu, nphi) floats whose first two nv_cast_rays_batch sets to the
+                      * least and greatest entry of rfront (the rest is its scratch) */
 } NvRayDesc;
 
 /* One grid of a batched ionize pass: the arguments of nv_ionize_apply (state = rho, mx, my, mz, or rho, vx, vy, vz
- * binary16 for nv_ionize_apply_h). */
+ * binary16 for nv_ionize_apply_h), and optionally the stars' front ranges. */
 typedef struct NvIonDesc {
   const float* rfronts;
   int nu, nphi;
@@ -91,6 +95,10 @@ typedef struct NvIonDesc {
   const void* state[4];
   float cn2, ci2;
   float* smax_out;
+  /* null, or per star the nv_ray_range_floats(nu, nphi) floats of its job's NvRayDesc.rf_range, one star after the
+   * other: the pass then skips a star where a proven bound shows it changes neither xi nor dfront (the same bits;
+   * no reference counterpart) */
+  const float* rf_ranges;
 } NvIonDesc;
 
 /* Render packing parameters: the reference's PackParams struct (render.py:77-88), same member order. */
@@ -220,6 +228,16 @@ int nv_cast_rays_wide(const float* rho, int nx, int ny, int nz, float sx, float 
  * stage block, which is free while ionize runs, is allocated that large). Built once per ionization. No reference
  * counterpart: it is the sampler's data layout, not a reference step. */
 long long nv_ray_bricks_size(int nx, int ny, int nz);
+
+/* The direction of every ray of an (nu, nphi) grid (nebula3d.py:252-257): dirs[4 (iu nphi + ip) + c] = d_c for
+ * c < 3 (the fourth float 0), a 16-byte aligned buffer of 16 nu nphi bytes. No reference counterpart (the reference
+ * evaluates the direction in every ray's thread, nebula3d.py:247-257); a batch's casts read it (NvRayDesc.dirs). */
+int nv_ray_dirs(int nu, int nphi, float* dirs, void* stream);
+
+/* Floats of a ray job's front range (NvRayDesc.rf_range) for an (nu, nphi) ray grid: the (least, greatest) pair and
+ * the casts' partial ranges; -1 for a bad grid. No reference counterpart (the ranges only let the batched ionize
+ * pass skip work, NvIonDesc.rf_ranges). */
+long long nv_ray_range_floats(int nu, int nphi);
 int nv_ray_bricks(const float* rho, int nx, int ny, int nz, float* bricks, void* stream);
 
 /* nv_cast_rays (wide == 0) or nv_cast_rays_wide (wide != 0), sampling the density from its brick array when

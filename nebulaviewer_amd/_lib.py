@@ -28,6 +28,8 @@ SIGNATURES = {
     "nv_cast_rays_wide": [P, I, I, I, F, F, F, F, F, I, F, F, F, F, F, P, I, I, P],
     "nv_ray_bricks_size": [I, I, I],
     "nv_ray_bricks": [P, I, I, I, P, P],
+    "nv_ray_dirs": [I, I, P, P],
+    "nv_ray_range_floats": [I, I],
     "nv_cast_rays_ex": [P, P, I, I, I, F, F, F, F, F, I, F, F, F, F, F, P, I, I, I, P],
     "nv_cast_rays_noalloc": [P, I, I, I, F, F, F, F, F, I, F, F, F, F, F, P, I, I, I, P],
     "nv_apply_front": [P, I, I, F, F, F, F, F, P, P, I, I, I, P],
@@ -65,7 +67,8 @@ SIGNATURES = {
 }
 
 RESTYPES = {"nv_last_error": ctypes.c_char_p, "nv_hydro_stage_batch_plan_size": LL,
-            "nv_cast_rays_batch_plan_size": LL, "nv_ionize_apply_batch_plan_size": LL, "nv_ray_bricks_size": LL}
+            "nv_cast_rays_batch_plan_size": LL, "nv_ionize_apply_batch_plan_size": LL, "nv_ray_bricks_size": LL,
+            "nv_ray_range_floats": LL}
 
 
 class NvStageDesc(ctypes.Structure):
@@ -80,14 +83,14 @@ class NvRayDesc(ctypes.Structure):
     """one job of nv_cast_rays_batch_plan"""
     _fields_ = [("rho", P), ("nx", I), ("ny", I), ("nz", I), ("sx", F), ("sy", F), ("sz", F), ("Q", F), ("dr", F),
                 ("nsteps", I), ("dx", F), ("rcap", F), ("bx", F), ("by", F), ("bz", F), ("rfront", P), ("nu", I),
-                ("nphi", I), ("wide", I), ("bricks", P)]
+                ("nphi", I), ("wide", I), ("bricks", P), ("dirs", P), ("rf_range", P)]
 
 
 class NvIonDesc(ctypes.Structure):
     """one grid of nv_ionize_apply_batch_plan"""
     _fields_ = [("rfronts", P), ("nu", I), ("nphi", I), ("star_xyz", P), ("nstars", I), ("dx", F), ("width", F),
                 ("xi", P), ("dfront_h", P), ("dfront_f", P), ("nx", I), ("ny", I), ("nz", I), ("state", P * 4),
-                ("cn2", F), ("ci2", F), ("smax_out", P)]
+                ("cn2", F), ("ci2", F), ("smax_out", P), ("rf_ranges", P)]
 
 
 class NvPackParams(ctypes.Structure):
@@ -115,7 +118,7 @@ def load(path=LIB_PATH):
         fn = getattr(lib, name)
         fn.argtypes = args
         fn.restype = RESTYPES.get(name, ctypes.c_int)
-    if lib.nv_abi_version() != 3:
+    if lib.nv_abi_version() != 4:
         raise NebulaHipError("libnebula_hip.so ABI version mismatch")
     _lib = lib
     return lib

@@ -47,9 +47,12 @@ class _Plan:
 
 class SimBatch:
     def __init__(self, setups, precision="fp32", device="cuda:0", rays=None, ion_every=None, graph=False,
-                 verbose=False, tile="tune", dispatch="tune", queue_wgs=0):
+                 verbose=False, tile="tune", dispatch="tune", queue_wgs=0, cull_stars=True):
         if not setups:
             raise ValueError("a batch needs at least one member")
+        # members with two or more stars: the casts record each star's front range and the ionize pass skips, per
+        # cell, the stars that provably change neither xi nor dfront (the same bits either way)
+        self.cull_stars = bool(cull_stars)
         self.device = torch.device(device)
         self.precision = precision
         # workgroup tile shape of each stage form: "tune" (measured per stage, nebulaviewer_amd.autotune), 0 (the
@@ -91,6 +94,8 @@ class SimBatch:
         self.graph = graph
         self._graphs = {}
         self._ion_key = None
+        self._dirs = {}            # (nu, nphi) -> direction table of that ray grid (nv_ray_dirs)
+        self._ranges = {}          # member -> its stars' front ranges (2 floats per star slot)
         self._stage_key = None
         self.timer = None
         # the CFL maximum of the current state, as Sim._compute_dt takes it when no fused value is cached
@@ -218,6 +223,16 @@ class SimBatch:
         self._stage_key = key
         self._graphs.clear()
 
+    def _ray_dirs(self, nu, nphi):
+        """device address of the direction table of an (nu, nphi) ray grid (nv_ray_dirs), built once per batch and
+        shared by every job on that grid"""
+        tab = self._dirs.get((nu, nphi))
+        if tab is None:
+            tab = torch.empty(4 * nu * nphi, dtype=torch.float32, device=self.device)
+            _lib.call("nv_ray_dirs", nu, nphi, tab.data_ptr(), self._stream)
+            self._dirs[(nu, nphi)] = tab
+        return tab.data_ptr()
+
     def _ion_plans(self):
         """ray-cast jobs of every switched-on star of every member, and the fused ionize pass per member"""
         key = tuple((tuple(self._on(m)), tuple(round(m.q_eff(i), 12) for i in self._on(m))) for m in self.members)
@@ -231,6 +246,12 @@ class SimBatch:
             on = self._on(m)
             nu, nphi = m.rfronts.shape[1:]
             Lx, Ly, Lz = m.bmax
+            # per active star its front range (nv_ray_range_floats), which the casts write and the ionize pass
+            # reads to skip stars that change no bit of a cell
+            per = int(_lib.load().nv_ray_range_floats(nu, nphi))
+            rng = self._ranges.get(s)
+            if rng is None or rng.numel() < per * len(m.rfronts):
+                rng = self._ranges[s] = torch.zeros(per * len(m.rfronts), dtype=torch.float32, device=self.device)
             for slot, i in enumerate(on):
                 j = _lib.NvRayDesc()
                 j.rho = m.U[0].data_ptr()
@@ -245,6 +266,8 @@ class SimBatch:
                 j.rfront = m.rfronts[slot].data_ptr()
                 j.nu, j.nphi = nu, nphi
                 j.wide = 1 if m.long_rays == "fp64" else 0
+                j.dirs = self._ray_dirs(nu, nphi)
+                j.rf_range = rng.data_ptr() + 4 * per * slot if self.cull_stars and len(on) >= 2 else None
                 # the member's 16 B/cell stage buffer is free while ionize runs: a brick plan builds the density
                 # bricks there and the casts sample them
                 j.bricks = m._brick_buf.data_ptr() if m.ray_bricks is not False else None
@@ -268,6 +291,7 @@ class SimBatch:
                 d.state[c] = m.U[c].data_ptr()
             d.cn2, d.ci2 = m.cn2, m.ci2
             d.smax_out = self.smax.data_ptr() + 4 * s
+            d.rf_ranges = rng.data_ptr() if self.cull_stars and len(on) >= 2 else None
         self._plan_rays = None
         if jobs:
             def ray_plan(use_bricks):

@@ -116,9 +116,16 @@ __device__ __forceinline__ float4 ld4(const float4* __restrict__ B, unsigned i) 
   return *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(B) + (OFF)i * 16u);
 }
 
+// A brick sample in two parts: brick_pos finds the sample's bricks and weights, brick_interp interpolates the eight
+// corners.
+struct BrickPos {
+  unsigned ia;          // brick (i0+1, j0+1, k0+1); the second brick is ia + (ny+1)(nz+1)
+  float fx, fy, fz;
+  unsigned keep;        // ~0 inside the box, 0 outside (the sample is +0)
+};
+
 template <class OFF, class SC>
-__device__ __forceinline__ float sample_brick(const float4* __restrict__ B, int nx, int ny, int nz, float px, float py,
-                                              float pz, const SC& cs) {
+__device__ __forceinline__ BrickPos brick_pos(int nx, int ny, int nz, float px, float py, float pz, const SC& cs) {
   const float ux = cs(px) - 0.5f, uy = cs(py) - 0.5f, uz = cs(pz) - 0.5f;
   const float hx = (float)nx - 0.5f, hy = (float)ny - 0.5f, hz = (float)nz - 0.5f;
   const bool outside = ux < -0.5f || uy < -0.5f || uz < -0.5f || ux > hx || uy > hy || uz > hz;
@@ -126,31 +133,44 @@ __device__ __forceinline__ float sample_brick(const float4* __restrict__ B, int 
   // consecutive samples can be in flight together. Inside, the clamp leaves the coordinates unchanged.
   const float gx = fminf(fmaxf(ux, -0.5f), hx), gy = fminf(fmaxf(uy, -0.5f), hy), gz = fminf(fmaxf(uz, -0.5f), hz);
   const float flx = floorf(gx), fly = floorf(gy), flz = floorf(gz);
-  const int i0 = (int)flx, j0 = (int)fly, k0 = (int)flz;
-  const float fx = gx - flx, fy = gy - fly, fz = gz - flz;
+  const int k0 = (int)flz;
+  BrickPos s;
+  s.fx = gx - flx;
+  s.fy = gy - fly;
+  s.fz = gz - flz;
   const unsigned qy = ny + 1, qz = nz + 1;
-  unsigned ia;
   if constexpr (sizeof(OFF) == 4) {
     // the padded row (i0 + 1) qy + j0 + 1 is below 2^24 on this path: one exact fma of integer-valued floats,
     // then a 24-bit multiply-add (an integer (i0 + 1) qy, rewritten by the compiler as i0 qy + qy, becomes a
     // quarter-rate 32-bit multiply)
     const unsigned row = (unsigned)fmaf(flx, (float)qy, fly + (float)(qy + 1));
-    ia = __umul24(row, qz) + (unsigned)(k0 + 1);
+    s.ia = __umul24(row, qz) + (unsigned)(k0 + 1);
   } else {
-    ia = cell_index<OFF>(i0 + 1, j0 + 1, k0 + 1, qy, qz);
+    s.ia = cell_index<OFF>((int)flx + 1, (int)fly + 1, k0 + 1, qy, qz);
   }
-  const float4 a = ld4<OFF>(B, ia), b = ld4<OFF>(B, ia + qy * qz);
+  // +0 outside by a bit mask, not a select: the compiler turns a select over loaded values back into a branch
+  s.keep = outside ? 0u : ~0u;
+  return s;
+}
+
+// the trilinear interpolation of sample_n (nebula3d.py:210-226) from the bricks a = (i0+1, ...), b = (i0+2, ...)
+__device__ __forceinline__ float brick_interp(const float4& a, const float4& b, const BrickPos& s) {
   const float c000 = a.x, c100 = b.x, c001 = a.y, c101 = b.y;
   const float c010 = a.z, c110 = b.z, c011 = a.w, c111 = b.w;
-  const float c00 = c000 + (c100 - c000) * fx;
-  const float c10 = c010 + (c110 - c010) * fx;
-  const float c01 = c001 + (c101 - c001) * fx;
-  const float c11 = c011 + (c111 - c011) * fx;
-  const float c0 = c00 + (c10 - c00) * fy;
-  const float c1 = c01 + (c11 - c01) * fy;
-  // +0 outside by a bit mask, not a select: the compiler turns a select over loaded values back into a branch
-  const unsigned keep = outside ? 0u : ~0u;
-  return __uint_as_float(__float_as_uint(c0 + (c1 - c0) * fz) & keep);
+  const float c00 = c000 + (c100 - c000) * s.fx;
+  const float c10 = c010 + (c110 - c010) * s.fx;
+  const float c01 = c001 + (c101 - c001) * s.fx;
+  const float c11 = c011 + (c111 - c011) * s.fx;
+  const float c0 = c00 + (c10 - c00) * s.fy;
+  const float c1 = c01 + (c11 - c01) * s.fy;
+  return __uint_as_float(__float_as_uint(c0 + (c1 - c0) * s.fz) & s.keep);
+}
+
+template <class OFF, class SC>
+__device__ __forceinline__ float sample_brick(const float4* __restrict__ B, int nx, int ny, int nz, float px, float py,
+                                              float pz, const SC& cs) {
+  const BrickPos s = brick_pos<OFF>(nx, ny, nz, px, py, pz, cs);
+  return brick_interp(ld4<OFF>(B, s.ia), ld4<OFF>(B, s.ia + (unsigned)(ny + 1) * (unsigned)(nz + 1)), s);
 }
 
 }  // namespace nv

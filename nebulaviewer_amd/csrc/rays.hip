@@ -41,6 +41,18 @@ __device__ __forceinline__ void box_range(const float o[3], const float d[3], co
   *t1o = t1;
 }
 
+// box_range for a star strictly inside the box: each axis's entry quotient is negative, so t0 = 0, and t1 is the
+// least of the exit quotients, the same divisions box_range takes its maximum of (three divisions instead of six)
+__device__ __forceinline__ float box_exit_inside(const float o[3], const float d[3], const float bm[3]) {
+  float t1 = 1e9f;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float da = d[a];
+    if (fabsf(da) >= 1e-9f) t1 = fminf(t1, da > 0.0f ? (bm[a] - o[a]) / da : (0.0f - o[a]) / da);
+  }
+  return t1;
+}
+
 // direction of angular cell (iu, ip) (nebula3d.py:252-257)
 __device__ __forceinline__ void ray_dir(int iu, int ip, int nu, int nphi, float d[3]) {
   const float u = -1.0f + ((float)iu + 0.5f) * 2.0f / (float)nu;
@@ -58,24 +70,33 @@ __device__ __forceinline__ float ldf(const uint16_t* __restrict__ f, unsigned i)
   return (float)__builtin_bit_cast(_Float16, f[i]);
 }
 
-// Direction cell of thread t: each wavefront takes an 8 x 8 patch of (u, phi) cells, so its 64 rays stay
-// within a few cells of each other and their corner gathers share cache lines.
-__device__ __forceinline__ bool ray_of_thread(long long t, int nu, int nphi, int* iu, int* ip) {
+// Direction cell of lane `lane` of direction patch `patch`: each wavefront takes an 8 x 8 patch of (u, phi) cells,
+// so its 64 rays stay within a few cells of each other and their corner gathers share cache lines. Each quad of
+// lanes (4q .. 4q+3), which the L1 serves together, takes a 2 x 2 block of (u, phi) cells rather than a row of
+// four: its rays lie closer together and touch fewer cache lines (the casts at 512^3: 1.35 -> 1.22 ms). The
+// patch index is the same for the whole wavefront (wave_patch), so its division is scalar arithmetic.
+__device__ __forceinline__ bool ray_of_patch(int patch, int lane, int nu, int nphi, int* iu, int* ip) {
   const int tp = (nphi + 7) >> 3;
-  const long long patch = t >> 6;
-  const int lane = (int)(t & 63);
-  *iu = (int)(patch / tp) * 8 + (lane >> 3);
-  *ip = (int)(patch % tp) * 8 + (lane & 7);
+  const int pu = patch / tp;
+  *iu = pu * 8 + ((lane >> 4) << 1) + ((lane >> 1) & 1);
+  *ip = (patch - pu * tp) * 8 + (((lane >> 2) & 3) << 1) + (lane & 1);
   return *iu < nu && *ip < nphi;
+}
+// patch of this wavefront when block b of 256 threads holds patches 4b .. 4b+3
+__device__ __forceinline__ int wave_patch(int b) {
+  return b * 4 + (int)__builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 }
 
 // One ray-cast job: a star, its direction grid and the density grid it marches through.
 struct RayJob {
   const float* rho;
   const float4* bricks;   // nv_ray_bricks of rho (the two-load sampler), or null (the paired sampler)
+  const float4* dirs;     // nv_ray_dirs of (nu, nphi): each ray's direction, or null (computed per ray)
+  float* rf_range;        // (least, greatest) entry of rfront, written after the casts, or null
   float* rfront;
   Recip rdx;         // dx and 1/dx correctly rounded (the shared reciprocal of the general cell scaling)
   int nx, ny, nz, nsteps, nu, nphi, wide;
+  int inside;        // the star lies strictly inside the box (0 < s < b on every axis)
   float sx, sy, sz, Q, dr, dx, rcap, bx, by, bz;
   float inv;         // 1/dx when dx is a power of two, else 0
 };
@@ -84,18 +105,31 @@ struct RayJob {
 // of sample n+1 are in flight while sample n is tested, and the column is still accumulated sample by sample in
 // the reference order. WIDE: a ray whose marched length (n1 - n0) dr exceeds 512 cells keeps its column in FP64
 // (the test col + inc > Q and the interpolation of rf in double); shorter rays are unchanged.
+// Returns the ray's front radius (>= +0), or NaN for a lane without a ray.
 template <bool PAIR, bool WIDE, class OFF, class SC, bool BRICK>
-__device__ __forceinline__ void cast_one(const RayJob& J, const SC& cs, long long t) {
+__device__ __forceinline__ float cast_one(const RayJob& J, const SC& cs, int patch, int lane) {
   int iu, ip;
-  if (!ray_of_thread(t, J.nu, J.nphi, &iu, &ip)) return;
+  if (!ray_of_patch(patch, lane, J.nu, J.nphi, &iu, &ip)) return __builtin_nanf("");
   const int nx = J.nx, ny = J.ny, nz = J.nz;
   const float sx = J.sx, sy = J.sy, sz = J.sz, Q = J.Q, dr = J.dr;
   float d[3];
-  ray_dir(iu, ip, J.nu, J.nphi, d);
+  if (J.dirs) {
+    const float4 dv = J.dirs[iu * J.nphi + ip];
+    d[0] = dv.x;
+    d[1] = dv.y;
+    d[2] = dv.z;
+  } else {
+    ray_dir(iu, ip, J.nu, J.nphi, d);
+  }
   const float st[3] = {sx, sy, sz}, bm[3] = {J.bx, J.by, J.bz};
   float t0, t1;
-  box_range(st, d, bm, &t0, &t1);
-  const int n0 = (int)(t0 / dr);
+  int n0 = 0;
+  if (J.inside) {
+    t1 = box_exit_inside(st, d, bm);   // t0 = 0 and n0 = (int)(0 / dr) = 0
+  } else {
+    box_range(st, d, bm, &t0, &t1);
+    n0 = (int)(t0 / dr);
+  }
   const int n1 = min(J.nsteps, (int)(t1 / dr) + 2);
   const bool wide = WIDE && (float)(n1 - n0) * dr > 512.0f * J.dx;
   float col = 0.0f, rf = J.rcap;
@@ -122,6 +156,8 @@ __device__ __forceinline__ void cast_one(const RayJob& J, const SC& cs, long lon
     const float rb = ((float)(n + 1) + 0.5f) * dr;
     float na, nb;
     if constexpr (BRICK) {
+      // (Skipping the loads of a sample whose bricks repeat the previous sample's, about 40% of consecutive
+      // samples, with per-lane predicated loads, measured 8% slower at 512^3.)
       na = sample_brick<OFF>(J.bricks, nx, ny, nz, sx + d[0] * ra, sy + d[1] * ra, sz + d[2] * ra, cs);
       nb = sample_brick<OFF>(J.bricks, nx, ny, nz, sx + d[0] * rb, sy + d[1] * rb, sz + d[2] * rb, cs);
     } else {
@@ -133,11 +169,34 @@ __device__ __forceinline__ void cast_one(const RayJob& J, const SC& cs, long lon
     if (step(rb, nb * nb * rb * rb * dr)) break;
   }
   J.rfront[(long long)iu * J.nphi + ip] = rf;
+  return rf;
+}
+
+// A job's front range: floats [0, 2) hold the least and greatest entry of its front table, floats [2 + 2b, 4 + 2b)
+// the least and greatest front radius of the job's workgroup b (256 rays), which the casts write with plain stores
+// and rf_range_kernel then reduces into [0, 2). NaN marks a workgroup (or lane) without a ray.
+__device__ __forceinline__ void block_rf_range(float rf, float* range, int b) {
+  __shared__ float lo_[4], hi_[4];
+  float lo = rf, hi = rf;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {   // fminf / fmaxf ignore NaN operands
+    lo = fminf(lo, __shfl_xor(lo, off, 64));
+    hi = fmaxf(hi, __shfl_xor(hi, off, 64));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    lo_[threadIdx.x >> 6] = lo;
+    hi_[threadIdx.x >> 6] = hi;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    range[2 + 2 * b] = fminf(fminf(lo_[0], lo_[1]), fminf(lo_[2], lo_[3]));
+    range[3 + 2 * b] = fmaxf(fmaxf(hi_[0], hi_[1]), fmaxf(hi_[2], hi_[3]));
+  }
 }
 
 template <bool PAIR, bool WIDE, class OFF, class SC, bool BRICK>
 __global__ __launch_bounds__(256) void cast_rays_kernel(RayJob J, SC cs) {
-  cast_one<PAIR, WIDE, OFF, SC, BRICK>(J, cs, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+  cast_one<PAIR, WIDE, OFF, SC, BRICK>(J, cs, wave_patch(blockIdx.x), threadIdx.x & 63);
 }
 
 // The ghost-layered brick array of a density field for sample_brick: brick (p, q, r), p in [0, nx+2), q in
@@ -158,10 +217,25 @@ __global__ __launch_bounds__(256) void ray_bricks_kernel(const float* __restrict
   for (int r = threadIdx.x; r <= nz; r += 256) brick_cell(f, B, nx, ny, nz, blockIdx.y, blockIdx.x, r);
 }
 
-// blocks of 256 rays each job needs (64-ray direction patches, nebula ray_of_thread)
+// The direction of every (u, phi) cell of a ray grid, (d0, d1, d2, 0) at iu nphi + ip, as ray_dir computes it: the
+// casts of a batch read it instead of evaluating the direction (a double-precision sine and cosine, a division
+// and a square root) once per ray and star.
+__global__ __launch_bounds__(256) void ray_dirs_kernel(int nu, int nphi, float4* __restrict__ dirs) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= nu * nphi) return;
+  const int iu = t / nphi;
+  float d[3];
+  ray_dir(iu, t - iu * nphi, nu, nphi, d);
+  dirs[t] = make_float4(d[0], d[1], d[2], 0.0f);
+}
+
+// blocks of 256 rays each job needs (64-ray direction patches, ray_of_patch)
 __host__ __device__ inline long long ray_blocks(int nu, int nphi) {
   return ((long long)((nu + 7) / 8) * ((nphi + 7) / 8) * 64 + 255) / 256;
 }
+
+// floats of a job's front range (NvRayDesc.rf_range): the (least, greatest) pair and one pair per workgroup
+__host__ __device__ inline long long range_floats(int nu, int nphi) { return 2 + 2 * ray_blocks(nu, nphi); }
 
 // Many ray-cast jobs in one launch: block b belongs to the job s with start[s] <= b < start[s+1]. The cell
 // scaling and the FP64 column are per job (block-uniform branches); paired z loads and 32-bit offsets are shared.
@@ -177,25 +251,24 @@ __global__ __launch_bounds__(256) void cast_rays_batch_kernel(const RayJob* __re
     else hi = mid;
   }
   const RayJob& J = jobs[lo];
-  const long long t = (long long)(b - start[lo]) * 256 + threadIdx.x;
+  const int patch = wave_patch(b - start[lo]), lane = threadIdx.x & 63;
   auto by_scale = [&](auto brick) {
     constexpr bool BR = decltype(brick)::value;
     if (J.inv != 0.0f) {
       ScalePow2 sc;
       sc.inv = J.inv;
-      if (J.wide) cast_one<PAIR, true, OFF, ScalePow2, BR>(J, sc, t);
-      else cast_one<PAIR, false, OFF, ScalePow2, BR>(J, sc, t);
-    } else {
-      ScaleDiv sc;
-      sc.rdx = J.rdx;
-      if (J.wide) cast_one<PAIR, true, OFF, ScaleDiv, BR>(J, sc, t);
-      else cast_one<PAIR, false, OFF, ScaleDiv, BR>(J, sc, t);
+      if (J.wide) return cast_one<PAIR, true, OFF, ScalePow2, BR>(J, sc, patch, lane);
+      return cast_one<PAIR, false, OFF, ScalePow2, BR>(J, sc, patch, lane);
     }
+    ScaleDiv sc;
+    sc.rdx = J.rdx;
+    if (J.wide) return cast_one<PAIR, true, OFF, ScaleDiv, BR>(J, sc, patch, lane);
+    return cast_one<PAIR, false, OFF, ScaleDiv, BR>(J, sc, patch, lane);
   };
   // without brick jobs the kernel holds the direct sampler alone (the brick sampler's registers would lower the
   // occupancy of every job)
-  if (ANY_BRICK && J.bricks) by_scale(std::true_type{});
-  else by_scale(std::false_type{});
+  const float rf = ANY_BRICK && J.bricks ? by_scale(std::true_type{}) : by_scale(std::false_type{});
+  if (J.rf_range) block_rf_range(rf, J.rf_range, b - start[lo]);   // every workgroup holds rays of one job only
 }
 
 // The brick arrays of several grids in one launch: block b belongs to brick job s with bstart[s] <= b <
@@ -401,6 +474,7 @@ struct IonArgs {
   int tiles_k, tiles_j, tiles;     // tile grid: z tiles, y tiles, all tiles (x chunks of ci planes slowest)
   float dx, cn2, ci2;
   Recip rwidth;                    // the ramp width and 1 / width, correctly rounded (its shared reciprocal)
+  const float* rf_ranges;          // per star (least, greatest) front radius of its table, or null (no culling)
 };
 
 // One 32 z x 16 y tile of the fused ionize pass, marching over x chunk bi.
@@ -409,7 +483,15 @@ struct IonArgs {
 // and cell t + 512 where that exists) go to an LDS ring; plane i is smoothed from planes i-1, i, i+1. The first
 // star's rfront gathers for a thread's cells of plane i+2 are issued before plane i is smoothed and stored, and
 // used after: the smoothing, the stores and the CFL max run while the gathers are in flight.
-template <bool SMAX, bool VEL>
+//
+// Star culling (with A.rf_ranges and two or more stars). xi is the maximum over stars of the front ramp and dfront
+// the minimum of r - rf, so the stars may be visited in any order. A cell visits first the star whose front may
+// lie nearest (least r - rf_min), and then skips a star when r - rf >= max(df, w) is certain for it (w the ramp
+// width): then its ramp is 0 and min(df, r - rf) is df, so skipping it changes no bit. The test uses a lower
+// bound on r (the hardware square root of the same sum cell_sph takes the exact root of, within an ulp) and the
+// star's greatest front radius, each widened by 2^-18 relative, far beyond the rounding of r and of the
+// interpolated rf. A cell away from every front then evaluates one star's direction and front instead of all.
+template <bool SMAX, bool VEL, bool CULL>
 __device__ __forceinline__ void ionize_tile(const IonArgs& A, int bk, int bj, int bi) {
   __shared__ float sx_[XR][XN];   // unsmoothed xi (max over stars of the front ramp)
   __shared__ float sd_[XR][XN];   // fp32 dfront (min over stars of r - rf)
@@ -425,14 +507,54 @@ __device__ __forceinline__ void ionize_tile(const IonArgs& A, int bk, int bj, in
   // wave-uniform index) are scalar, instead of a vector load and wait per star and cell
   const CONST_AS float* stars = (const CONST_AS float*)A.star_xyz;
 
+  // CULL: some grid of the launch carries front ranges; this tile culls with two to 64 stars
+  const bool cull = CULL && A.rf_ranges != nullptr && A.nstars >= 2 && A.nstars <= 64;
+  __shared__ float ranges[CULL ? 128 : 1];   // per star its least and greatest front radius
+  if (cull && tid < 2 * A.nstars) ranges[tid] = A.rf_ranges[(tid >> 1) * range_floats(A.nu, A.nphi) + (tid & 1)];
+  if (cull) __syncthreads();
   // halo-plane cell t of plane p (clamped cell positions, as ldc in smooth_xi): its distance, direction and the
   // front taps of star st
-  auto star_taps = [&](int p, int t, int st, float& r, FrontTaps& T) {
+  auto cell_of = [&](int p, int t, int& ic, int& jc, int& kc) {
     const int jj = t / XK, kk = t - jj * XK;
-    const int ic = clampi(p, 0, nx - 1), jc = clampi(j0 + jj - 1, 0, ny - 1), kc = clampi(k0 + kk - 1, 0, nz - 1);
+    ic = clampi(p, 0, nx - 1);
+    jc = clampi(j0 + jj - 1, 0, ny - 1);
+    kc = clampi(k0 + kk - 1, 0, nz - 1);
+  };
+  auto star_taps = [&](int p, int t, int st, float& r, FrontTaps& T) {
+    int ic, jc, kc;
+    cell_of(p, t, ic, jc, kc);
     float u, phi;
     cell_sph(ic, jc, kc, stars[3 * st], stars[3 * st + 1], stars[3 * st + 2], A.dx, &r, &u, &phi);
     front_taps(A.rfronts + st * tab, A.nu, A.nphi, u, phi, T);
+  };
+  // the distance from star st to the cell centre by the hardware square root of the sum cell_sph forms (within an
+  // ulp of its correctly rounded root)
+  auto r_fast = [&](int ic, int jc, int kc, int st) {
+    const float r0 = ((float)ic + 0.5f) * A.dx - stars[3 * st];
+    const float r1 = ((float)jc + 0.5f) * A.dx - stars[3 * st + 1];
+    const float r2 = ((float)kc + 0.5f) * A.dx - stars[3 * st + 2];
+    return __builtin_amdgcn_sqrtf((r0 * r0 + r1 * r1) + r2 * r2);
+  };
+  // the star this cell visits first: 0, or with culling the one of least r - rf_min
+  auto first_star = [&](int p, int t) {
+    if (!cull) return 0;
+    int ic, jc, kc;
+    cell_of(p, t, ic, jc, kc);
+    int best = 0;
+    float bv = r_fast(ic, jc, kc, 0) - ranges[0];
+    for (int st = 1; st < A.nstars; ++st) {
+      const float v = r_fast(ic, jc, kc, st) - ranges[2 * st];
+      if (v < bv) {
+        bv = v;
+        best = st;
+      }
+    }
+    return best;
+  };
+  // true when star st certainly leaves (x, df) of the cell unchanged: r - rf >= max(df, w)
+  auto skip_star = [&](int ic, int jc, int kc, int st, float df) {
+    const float m = fmaxf(df, A.rwidth.b) + ranges[2 * st + 1];
+    return r_fast(ic, jc, kc, st) * (1.0f - 0x1p-18f) >= m * (1.0f + 0x1p-18f);
   };
   // one star's ramp into the running max (xi) and min (dfront)
   auto accumulate = [&](float r, float rf, float& x, float& df) {
@@ -440,9 +562,12 @@ __device__ __forceinline__ void ionize_tile(const IonArgs& A, int bk, int bj, in
     if (val > x) x = val;
     df = fminf(df, r - rf);
   };
-  // stars s0.. of halo cell t of plane p, written to the ring; x, df carry the stars before s0
+  // every star but s0 of halo cell t of plane p, written to the ring; x, df hold star s0's contribution
   auto finish_cell = [&](int p, int t, int s0, float x, float df) {
-    for (int st = s0; st < A.nstars; ++st) {
+    int ic, jc, kc;
+    cell_of(p, t, ic, jc, kc);
+    for (int st = 0; st < A.nstars; ++st) {
+      if (st == s0 || (cull && skip_star(ic, jc, kc, st, df))) continue;
       float r;
       FrontTaps T;
       star_taps(p, t, st, r, T);
@@ -454,7 +579,18 @@ __device__ __forceinline__ void ionize_tile(const IonArgs& A, int bk, int bj, in
   };
   // whole plane p: every halo cell, every star (xi.zero_(), df.fill_(1e3): sim.py:930-931)
   auto make_plane = [&](int p) {
-    for (int t = tid; t < XN; t += INT_) finish_cell(p, t, 0, 0.0f, 1e3f);
+    for (int t = tid; t < XN; t += INT_) {
+      float x = 0.0f, df = 1e3f;
+      int s0 = -1;
+      if (A.nstars > 0) {
+        s0 = first_star(p, t);
+        float r;
+        FrontTaps T;
+        star_taps(p, t, s0, r, T);
+        accumulate(r, front_lerp(A.rfronts + s0 * tab, T), x, df);
+      }
+      finish_cell(p, t, s0, x, df);
+    }
   };
 
   make_plane(i0 - 1);
@@ -482,9 +618,14 @@ __device__ __forceinline__ void ionize_tile(const IonArgs& A, int bk, int bj, in
     const bool second = tid + INT_ < XN;   // this thread also owns a cell of the second pass
     float r0 = 0.0f, r1 = 0.0f;
     FrontTaps T0, T1;
+    int s0 = -1, s1 = -1;
     if (next && A.nstars > 0) {
-      star_taps(i + 2, tid, 0, r0, T0);
-      if (second) star_taps(i + 2, tid + INT_, 0, r1, T1);
+      s0 = first_star(i + 2, tid);
+      star_taps(i + 2, tid, s0, r0, T0);
+      if (second) {
+        s1 = first_star(i + 2, tid + INT_);
+        star_taps(i + 2, tid + INT_, s1, r1, T1);
+      }
     }
 
     if (active) {
@@ -515,13 +656,13 @@ __device__ __forceinline__ void ionize_tile(const IonArgs& A, int bk, int bj, in
     if (next) {
       // plane i+2 goes to the ring slot of plane i-2, which no thread reads after the last barrier
       float x = 0.0f, df = 1e3f;
-      if (A.nstars > 0) accumulate(r0, front_lerp(A.rfronts, T0), x, df);
-      finish_cell(i + 2, tid, 1, x, df);
+      if (A.nstars > 0) accumulate(r0, front_lerp(A.rfronts + s0 * tab, T0), x, df);
+      finish_cell(i + 2, tid, s0, x, df);
       if (second) {
         x = 0.0f;
         df = 1e3f;
-        if (A.nstars > 0) accumulate(r1, front_lerp(A.rfronts, T1), x, df);
-        finish_cell(i + 2, tid + INT_, 1, x, df);
+        if (A.nstars > 0) accumulate(r1, front_lerp(A.rfronts + s1 * tab, T1), x, df);
+        finish_cell(i + 2, tid + INT_, s1, x, df);
       }
     }
     __syncthreads();
@@ -536,13 +677,13 @@ template <bool SMAX, bool VEL>
 __global__ __launch_bounds__(INT_) void ionize_apply_kernel(IonArgs A, int per_xcd) {
   const int t = xcd_tile(per_xcd);
   if (t >= A.tiles) return;
-  ionize_tile<SMAX, VEL>(A, t % A.tiles_k, (t / A.tiles_k) % A.tiles_j, t / (A.tiles_k * A.tiles_j));
+  ionize_tile<SMAX, VEL, false>(A, t % A.tiles_k, (t / A.tiles_k) % A.tiles_j, t / (A.tiles_k * A.tiles_j));
 }
 
 // Many grids in one launch: tile t of the flat list belongs to member s with start[s] <= t < start[s+1]. The tiles
 // are dealt round robin to the XCDs: members differ in their star count, so contiguous runs of equal length would
 // give the XCDs unequal work (measured on the 64-member batch: 2.34 ms against 2.04).
-template <bool SMAX, bool VEL>
+template <bool SMAX, bool VEL, bool CULL>
 __global__ __launch_bounds__(INT_) void ionize_apply_batch_kernel(const IonArgs* __restrict__ args,
                                                                   const int* __restrict__ start, int n, int total) {
   const int t = blockIdx.x;
@@ -556,7 +697,7 @@ __global__ __launch_bounds__(INT_) void ionize_apply_batch_kernel(const IonArgs*
   const IonArgs& A = args[lo];
   const int l = t - start[lo];
   const int bk = l % A.tiles_k, bj = (l / A.tiles_k) % A.tiles_j, bi = l / (A.tiles_k * A.tiles_j);
-  ionize_tile<SMAX, VEL>(A, bk, bj, bi);
+  ionize_tile<SMAX, VEL, CULL>(A, bk, bj, bi);
 }
 
 // ---------------------------------------------------------------------------------------- FUV
@@ -695,11 +836,12 @@ static void dispatch_sampler(int nz, long long cells, long long rows, float dx, 
 static void fill_ray_job(const float* rho, int nx, int ny, int nz, float sx, float sy, float sz, float Q, float dr,
                          int nsteps, float dx, float rcap, float bx, float by, float bz, float* rfront, int nu,
                          int nphi, bool wide, RayJob& J) {
-  J.rho = rho; J.bricks = nullptr; J.rfront = rfront;
+  J.rho = rho; J.bricks = nullptr; J.dirs = nullptr; J.rf_range = nullptr; J.rfront = rfront;
   J.rdx = Recip{1.0 / (double)dx, dx};
   J.nx = nx; J.ny = ny; J.nz = nz; J.nsteps = nsteps; J.nu = nu; J.nphi = nphi; J.wide = wide ? 1 : 0;
   J.sx = sx; J.sy = sy; J.sz = sz; J.Q = Q; J.dr = dr; J.dx = dx; J.rcap = rcap; J.bx = bx; J.by = by; J.bz = bz;
   J.inv = pow2_reciprocal(dx);
+  J.inside = (sx > 0.0f && sx < bx && sy > 0.0f && sy < by && sz > 0.0f && sz < bz) ? 1 : 0;
 }
 
 extern "C" long long nv_ray_bricks_size(int nx, int ny, int nz);
@@ -772,6 +914,19 @@ extern "C" int nv_ray_bricks(const float* rho, int nx, int ny, int nz, float* br
   hipLaunchKernelGGL(ray_bricks_kernel, dim3((unsigned)(ny + 1), (unsigned)(nx + 2)), dim3(256), 0, as_stream(stream),
                      rho, reinterpret_cast<float4*>(bricks), nx, ny, nz);
   return check_launch("nv_ray_bricks");
+}
+
+extern "C" long long nv_ray_range_floats(int nu, int nphi) {
+  return nu > 0 && nphi > 0 ? range_floats(nu, nphi) : -1;
+}
+
+extern "C" int nv_ray_dirs(int nu, int nphi, float* dirs, void* stream) {
+  clear_error();
+  NV_REQUIRE(dirs && ((uintptr_t)dirs & 15) == 0, "nv_ray_dirs: dirs must be a 16-byte aligned buffer");
+  NV_REQUIRE(nu > 0 && nphi > 0 && (long long)nu * nphi < (1LL << 31), "nv_ray_dirs: bad sizes");
+  hipLaunchKernelGGL(ray_dirs_kernel, dim3((unsigned)(((long long)nu * nphi + 255) / 256)), dim3(256), 0,
+                     as_stream(stream), nu, nphi, reinterpret_cast<float4*>(dirs));
+  return check_launch("nv_ray_dirs");
 }
 
 extern "C" int nv_cast_rays_ex(const float* rho, const float* bricks, int nx, int ny, int nz, float sx, float sy,
@@ -894,6 +1049,7 @@ static int fill_ionize(const char* name, const float* rfronts, int nu, int nphi,
   A.nu = nu; A.nphi = nphi; A.nstars = nstars; A.nx = nx; A.ny = ny; A.nz = nz;
   A.dx = dx; A.cn2 = cn2; A.ci2 = ci2;
   A.rwidth = Recip{1.0 / (double)width, width};
+  A.rf_ranges = nullptr;
   A.ci = (int)std::min<long long>(64, std::max<long long>(4, (long long)nx * jk_total / 2048));
   A.tiles_k = (nz + IK - 1) / IK;
   A.tiles_j = (ny + IJ - 1) / IJ;
@@ -983,11 +1139,39 @@ extern "C" int nv_fuv_apply(const float* atts, int nr, int nu, int nphi, const f
 // Plan layout (host blob, uploaded once): a 64-byte header, the jobs' kernel arguments, then start[0..n]; a ray
 // plan continues with its brick jobs (at most one per job) and their block prefix bstart[0..nb].
 namespace nv {
+// Each job's front range from its workgroups' partial ranges: one workgroup per job.
+__global__ __launch_bounds__(256) void rf_range_kernel(const RayJob* __restrict__ jobs) {
+  __shared__ float lo_[4], hi_[4];
+  const RayJob& J = jobs[blockIdx.x];
+  if (!J.rf_range) return;
+  const int nb = (int)ray_blocks(J.nu, J.nphi);
+  float lo = __builtin_nanf(""), hi = lo;
+  for (int b = threadIdx.x; b < nb; b += 256) {
+    lo = fminf(lo, J.rf_range[2 + 2 * b]);
+    hi = fmaxf(hi, J.rf_range[3 + 2 * b]);
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    lo = fminf(lo, __shfl_xor(lo, off, 64));
+    hi = fmaxf(hi, __shfl_xor(hi, off, 64));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    lo_[threadIdx.x >> 6] = lo;
+    hi_[threadIdx.x >> 6] = hi;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    J.rf_range[0] = fminf(fminf(lo_[0], lo_[1]), fminf(lo_[2], lo_[3]));
+    J.rf_range[1] = fmaxf(fmaxf(hi_[0], hi_[1]), fmaxf(hi_[2], hi_[3]));
+  }
+}
+
 struct RayBatchHeader {
   uint32_t magic;
   int n, total, pair, off32;
   int nb, btotal;          // ray plans: brick jobs and their blocks
-  int pad[9];
+  int ranges;              // some job asks for its front range (ray plans) or member carries them (ionize plans)
+  int pad[8];
 };
 static_assert(sizeof(RayBatchHeader) == 64, "plan header layout");
 constexpr uint32_t kRayMagic = 0x4e565231u;   // "NVR1"
@@ -1016,7 +1200,7 @@ extern "C" int nv_cast_rays_batch_plan(const NvRayDesc* jobs, int n, void* plan,
   int* bstart = reinterpret_cast<int*>(bj + n);
   long long total = 0, btotal = 0;
   int nb = 0;
-  bool pair = true, off32 = true;
+  bool pair = true, off32 = true, ranges = false;
   for (int s = 0; s < n; ++s) {
     const NvRayDesc& d = jobs[s];
     NV_REQUIRE(d.rho && d.rfront, "nv_cast_rays_batch_plan: job %d: null pointer", s);
@@ -1036,6 +1220,11 @@ extern "C" int nv_cast_rays_batch_plan(const NvRayDesc* jobs, int n, void* plan,
     RayJob J;
     fill_ray_job(d.rho, d.nx, d.ny, d.nz, d.sx, d.sy, d.sz, d.Q, d.dr, d.nsteps, d.dx, d.rcap, d.bx, d.by, d.bz,
                  d.rfront, d.nu, d.nphi, d.wide != 0, J);
+    NV_REQUIRE(!d.dirs || ((uintptr_t)d.dirs & 15) == 0, "nv_cast_rays_batch_plan: job %d: dirs must be 16-byte aligned",
+               s);
+    J.dirs = reinterpret_cast<const float4*>(d.dirs);
+    J.rf_range = d.rf_range;
+    ranges = ranges || d.rf_range != nullptr;
     if (d.bricks) {
       const char* r = reinterpret_cast<const char*>(d.rho);
       const char* b = reinterpret_cast<const char*>(d.bricks);
@@ -1080,6 +1269,7 @@ extern "C" int nv_cast_rays_batch_plan(const NvRayDesc* jobs, int n, void* plan,
   H.off32 = off32 ? 1 : 0;
   H.nb = nb;
   H.btotal = (int)btotal;
+  H.ranges = ranges ? 1 : 0;
   memcpy(plan, &H, sizeof(H));
   return NV_OK;
 }
@@ -1105,6 +1295,7 @@ extern "C" int nv_cast_rays_batch(const void* plan_host, const void* plan_dev, v
   };
   auto kern = H.nb > 0 ? pick(std::true_type{}) : pick(std::false_type{});
   hipLaunchKernelGGL(kern, dim3(H.total), dim3(256), 0, as_stream(stream), jobs, start, H.n, H.total);
+  if (H.ranges) hipLaunchKernelGGL(rf_range_kernel, dim3(H.n), dim3(256), 0, as_stream(stream), jobs);
   return check_launch("nv_cast_rays_batch");
 }
 
@@ -1124,6 +1315,7 @@ extern "C" int nv_ionize_apply_batch_plan(const NvIonDesc* members, int n, int v
   IonArgs* out = reinterpret_cast<IonArgs*>(static_cast<char*>(plan) + sizeof(H));
   int* start = reinterpret_cast<int*>(out + n);
   const bool sm = members[0].smax_out != nullptr;
+  bool ranges = false;
   long long total = 0;
   char name[64];
   for (int s = 0; s < n; ++s) {
@@ -1135,6 +1327,8 @@ extern "C" int nv_ionize_apply_batch_plan(const NvIonDesc* members, int n, int v
                                d.dfront_f, d.nx, d.ny, d.nz, d.state[0], d.state[1], d.state[2], d.state[3], d.cn2,
                                d.ci2, d.smax_out, jk_total, A);
     if (rc != NV_OK) return rc;
+    A.rf_ranges = d.rf_ranges;
+    ranges = ranges || d.rf_ranges != nullptr;
     memcpy(&out[s], &A, sizeof(A));
     start[s] = (int)total;
     total += A.tiles;
@@ -1146,6 +1340,7 @@ extern "C" int nv_ionize_apply_batch_plan(const NvIonDesc* members, int n, int v
   H.total = (int)total;
   H.pair = sm ? 1 : 0;     // reused: the CFL reduction is on
   H.off32 = vel ? 1 : 0;   // reused: binary16-velocity state
+  H.ranges = ranges ? 1 : 0;
   memcpy(plan, &H, sizeof(H));
   return NV_OK;
 }
@@ -1158,8 +1353,12 @@ extern "C" int nv_ionize_apply_batch(const void* plan_host, const void* plan_dev
   NV_REQUIRE(H.magic == kIonMagic && H.n > 0, "nv_ionize_apply_batch: not an ionize batch plan");
   const IonArgs* args = reinterpret_cast<const IonArgs*>(static_cast<const char*>(plan_dev) + sizeof(H));
   const int* start = reinterpret_cast<const int*>(args + H.n);
-  auto kern = !H.pair ? ionize_apply_batch_kernel<false, false>
-                      : (H.off32 ? ionize_apply_batch_kernel<true, true> : ionize_apply_batch_kernel<true, false>);
+  auto pick = [&](auto cull) {
+    constexpr bool C = decltype(cull)::value;
+    return !H.pair ? ionize_apply_batch_kernel<false, false, C>
+                   : (H.off32 ? ionize_apply_batch_kernel<true, true, C> : ionize_apply_batch_kernel<true, false, C>);
+  };
+  auto kern = H.ranges ? pick(std::true_type{}) : pick(std::false_type{});
   hipLaunchKernelGGL(kern, dim3(H.total), dim3(IK, IJ), 0, as_stream(stream), args, start, H.n, H.total);
   return check_launch("nv_ionize_apply_batch");
 }

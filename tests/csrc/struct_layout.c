@@ -19,12 +19,12 @@ int main(void) {
   OFF(NvRayDesc, sy); OFF(NvRayDesc, sz); OFF(NvRayDesc, Q); OFF(NvRayDesc, dr); OFF(NvRayDesc, nsteps);
   OFF(NvRayDesc, dx); OFF(NvRayDesc, rcap); OFF(NvRayDesc, bx); OFF(NvRayDesc, by); OFF(NvRayDesc, bz);
   OFF(NvRayDesc, rfront); OFF(NvRayDesc, nu); OFF(NvRayDesc, nphi); OFF(NvRayDesc, wide);
-  OFF(NvRayDesc, bricks);
+  OFF(NvRayDesc, bricks); OFF(NvRayDesc, dirs); OFF(NvRayDesc, rf_range);
   printf("NvIonDesc %zu\n", sizeof(NvIonDesc));
   OFF(NvIonDesc, rfronts); OFF(NvIonDesc, nu); OFF(NvIonDesc, nphi); OFF(NvIonDesc, star_xyz);
   OFF(NvIonDesc, nstars); OFF(NvIonDesc, dx); OFF(NvIonDesc, width); OFF(NvIonDesc, xi); OFF(NvIonDesc, dfront_h);
   OFF(NvIonDesc, dfront_f); OFF(NvIonDesc, nx); OFF(NvIonDesc, ny); OFF(NvIonDesc, nz); OFF(NvIonDesc, state);
-  OFF(NvIonDesc, cn2); OFF(NvIonDesc, ci2); OFF(NvIonDesc, smax_out);
+  OFF(NvIonDesc, cn2); OFF(NvIonDesc, ci2); OFF(NvIonDesc, smax_out); OFF(NvIonDesc, rf_ranges);
   printf("NvPackParams %zu\n", sizeof(NvPackParams));
   OFF(NvPackParams, c_ha); OFF(NvPackParams, c_o3); OFF(NvPackParams, c_s2); OFF(NvPackParams, c_dg);
   OFF(NvPackParams, c_sh); OFF(NvPackParams, gs); OFF(NvPackParams, ambient); OFF(NvPackParams, edge);

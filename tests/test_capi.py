@@ -40,7 +40,7 @@ def test_every_declared_symbol_is_exported_and_bound(lib):
 
 
 def test_abi_version(lib):
-    assert lib.nv_abi_version() == 3
+    assert lib.nv_abi_version() == 4
 
 
 def struct_fields(name):
@@ -174,3 +174,19 @@ def test_rejects_bad_enums_and_sizes(lib):
 def test_error_is_raised_in_python(lib):
     with pytest.raises(_lib.NebulaHipError, match="nv_smooth_xi"):
         _lib.call("nv_smooth_xi", None, None, 4, 4, 4, None)
+
+
+def test_ray_dirs_validates(lib):
+    """nv_ray_dirs refuses a missing or misaligned table and empty grids before any launch"""
+    assert lib.nv_ray_dirs(8, 8, None, None) == 1001
+    assert lib.nv_ray_dirs(8, 8, ctypes.c_void_p(0x1004), None) == 1001
+    assert b"16-byte aligned" in lib.nv_last_error()
+    assert lib.nv_ray_dirs(0, 8, ctypes.c_void_p(0x1000), None) == 1001
+    assert b"bad sizes" in lib.nv_last_error()
+
+
+def test_ray_range_floats(lib):
+    """the front-range buffer of a ray job: the (least, greatest) pair and one pair per 256-ray workgroup"""
+    assert lib.nv_ray_range_floats(512, 1024) == 2 + 2 * (512 * 1024 // 256)
+    assert lib.nv_ray_range_floats(7, 9) == 2 + 2 * 1   # one 8 x 8 patch, one workgroup
+    assert lib.nv_ray_range_floats(0, 9) == -1

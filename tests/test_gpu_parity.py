@@ -671,3 +671,27 @@ def test_rk2_step_mass_budget(env):
     tol = 12 * np.sqrt(np.sum(np.spacing(rho0).astype(np.float64) ** 2))
     assert abs((m2 - m0) - expect) <= tol, (m2 - m0, expect, tol)
     assert abs(expect) > 10 * tol
+
+
+@pytest.mark.parametrize("nu,nphi", [(512, 1024), (24, 40), (7, 9)])
+def test_ray_dirs_table(env, nu, nphi):
+    """nv_ray_dirs against the ray direction of nebula3d.py:252-257 restated in float32 numpy with the oracle's
+    shared sine and cosine, bit for bit (the batched casts read this table instead of computing it)"""
+    from oracle.oracle import load
+    say(f"[ray dirs] {nu}x{nphi}")
+    torch, dev, L = env
+    olib = load()
+    tab = torch.empty(4 * nu * nphi, dtype=torch.float32, device=dev)
+    L.call("nv_ray_dirs", nu, nphi, tab.data_ptr(), stream(env))
+    torch.cuda.synchronize()
+    got = N(tab).reshape(nu, nphi, 4)
+    iu = np.arange(nu, dtype=F32)
+    u = F32(-1) + (iu + F32(0.5)) * F32(2) / F32(nu)
+    s = np.sqrt(np.maximum(F32(1) - u * u, F32(0)))
+    phi = (np.arange(nphi, dtype=F32) + F32(0.5)) * F32(2) * F32(math.pi) / F32(nphi)
+    sn = np.array([olib.or_sinf_det(float(v)) for v in phi], F32)
+    cs = np.array([olib.or_cosf_det(float(v)) for v in phi], F32)
+    np.testing.assert_array_equal(got[..., 0], s[:, None] * cs[None, :])
+    np.testing.assert_array_equal(got[..., 1], s[:, None] * sn[None, :])
+    np.testing.assert_array_equal(got[..., 2], np.broadcast_to(u[:, None], (nu, nphi)))
+    assert not got[..., 3].any()
