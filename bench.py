@@ -218,7 +218,7 @@ def make_measure_batch(a, S, KernelTimer):
         b.step(a.steps)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
-        timer = KernelTimer()
+        timer = KernelTimer(reserve=32 * b.ion_every)
         b.timer = timer                        # per-kernel events (eager launches, no graph)
         b.step(2 * b.ion_every)
         torch.cuda.synchronize()
@@ -313,7 +313,7 @@ def main():
         log(f"[bench] {precision}: warmup {a.warmup} steps, timing {a.steps} steps")
         sim.step(a.warmup)
         torch.cuda.synchronize()
-        timer = KernelTimer()
+        timer = KernelTimer(reserve=16 * a.steps)   # every event of the timed steps created before them
         sim.timer = timer
         torch.cuda.synchronize()
         t0 = time.perf_counter()
