@@ -292,17 +292,30 @@ int nv_ionize_apply(const float* rfronts, int nu, int nphi, const float* star_xy
                     const float* rho, const float* mx, const float* my, const float* mz, float cn2, float ci2,
                     float* smax_out, void* stream);
 
+/* Forms of the fused ionize pass (bits): NV_ION_VEL takes the CFL maximum over the binary16-velocity state (as
+ * nv_ionize_apply_h); NV_ION_FAST evaluates each cell in fast FP32 arithmetic (the hardware square root and
+ * reciprocal for the cell's distance, direction, atan2 quotient and velocities; float products for the divisions
+ * by 2 pi, 6 and the ramp width): xi, dfront and the CFL maximum within a few ulp of the exact pass, for the
+ * fast precision forms (NV_PREC_F32_FAST, NV_PREC_F16V), not bit-identical to the reference. */
+#define NV_ION_VEL 1
+#define NV_ION_FAST 2
+
 /* nv_ionize_apply with the CFL maximum taken over the binary16-velocity state (rho is not read). */
 int nv_ionize_apply_h(const float* rfronts, int nu, int nphi, const float* star_xyz, int nstars, float dx,
                       float width, float* xi, uint16_t* dfront_h, float* dfront_f, int nx, int ny, int nz,
                       const float* rho, const uint16_t* vx, const uint16_t* vy, const uint16_t* vz, float cn2,
                       float ci2, float* smax_out, void* stream);
 
-/* Batched fused ionize pass over many grids in one launch, each member bit-identical to nv_ionize_apply
- * (vel = 0) or nv_ionize_apply_h (vel = 1) on that member alone; smax_out set for all members or none. No
- * reference counterpart (sim.py:927-947 is per Sim). Plan and launch as for nv_hydro_stage_batch. */
+/* nv_ionize_apply / nv_ionize_apply_h of one grid given as a descriptor, in a form (NV_ION_* bits; 0 is
+ * nv_ionize_apply exactly, NV_ION_VEL nv_ionize_apply_h). rf_ranges is ignored (single grids do not cull). */
+int nv_ionize_apply_ex(const NvIonDesc* d, int form, void* stream);
+
+/* Batched fused ionize pass over many grids in one launch, each member bit-identical to nv_ionize_apply_ex of
+ * the same form (NV_ION_* bits: 0 is nv_ionize_apply, NV_ION_VEL nv_ionize_apply_h) on that member alone;
+ * smax_out set for all members or none. No reference counterpart (sim.py:927-947 is per Sim). Plan and launch as
+ * for nv_hydro_stage_batch. */
 long long nv_ionize_apply_batch_plan_size(int n);
-int nv_ionize_apply_batch_plan(const NvIonDesc* members, int n, int vel, void* plan, long long plan_bytes);
+int nv_ionize_apply_batch_plan(const NvIonDesc* members, int n, int form, void* plan, long long plan_bytes);
 int nv_ionize_apply_batch(const void* plan_host, const void* plan_dev, void* stream);
 
 /* FUV dust attenuation att[nr][nu][nphi] along rays from one star. Replaces nebula3d.cast_fuv

@@ -14,6 +14,10 @@ I = ctypes.c_int
 F = ctypes.c_float
 LL = ctypes.c_longlong
 
+# forms of the fused ionize pass (NV_ION_* in include/nebula_hip.h)
+NV_ION_VEL = 1
+NV_ION_FAST = 2
+
 # entry point -> argument types (order and meaning as in include/nebula_hip.h)
 SIGNATURES = {
     "nv_last_error": [],
@@ -51,6 +55,7 @@ SIGNATURES = {
     "nv_cast_rays_batch_plan_size": [I],
     "nv_cast_rays_batch_plan": [P, I, P, LL],
     "nv_cast_rays_batch": [P, P, P],
+    "nv_ionize_apply_ex": [P, I, P],
     "nv_ionize_apply_batch_plan_size": [I],
     "nv_ionize_apply_batch_plan": [P, I, I, P, LL],
     "nv_ionize_apply_batch": [P, P, P],
@@ -118,7 +123,7 @@ def load(path=LIB_PATH):
         fn = getattr(lib, name)
         fn.argtypes = args
         fn.restype = RESTYPES.get(name, ctypes.c_int)
-    if lib.nv_abi_version() != 4:
+    if lib.nv_abi_version() != 5:
         raise NebulaHipError("libnebula_hip.so ABI version mismatch")
     _lib = lib
     return lib

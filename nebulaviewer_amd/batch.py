@@ -241,7 +241,7 @@ class SimBatch:
         P = _lib.P
         jobs = []
         ions = (_lib.NvIonDesc * self.n)()
-        vel = 1 if self.precision == "fp16-vel" else 0
+        form = self.members[0].ion_form   # every member has the batch's precision
         for s, m in enumerate(self.members):
             on = self._on(m)
             nu, nphi = m.rfronts.shape[1:]
@@ -318,7 +318,7 @@ class SimBatch:
             self.ray_bricks_used = bool(use)
 
         def build_ion(buf, nbytes):
-            _lib.call("nv_ionize_apply_batch_plan", ctypes.cast(ions, P), self.n, vel, buf, nbytes)
+            _lib.call("nv_ionize_apply_batch_plan", ctypes.cast(ions, P), self.n, form, buf, nbytes)
         self._plan_ion = _Plan("nv_ionize_apply_batch_plan_size", build_ion, self.n, self.device)
         self._ion_key = key
         self._graphs.clear()

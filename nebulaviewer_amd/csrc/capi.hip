@@ -27,4 +27,4 @@ int check_launch(const char* what) {
 }  // namespace nv
 
 extern "C" const char* nv_last_error(void) { return nv::g_err; }
-extern "C" int nv_abi_version(void) { return 4; }
+extern "C" int nv_abi_version(void) { return 5; }

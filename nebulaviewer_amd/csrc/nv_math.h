@@ -96,7 +96,10 @@ NV_HD inline float cosf_det(float x) { float s, c; sincosf_det(x, &s, &c); retur
 // degree 11: truncation below s^12/13 < 1.1e-8 relative); the correction s q is under 2.4% of s, so its own
 // rounding is damped. Then pi/2 - a above the diagonal, pi - a for x < 0, and the sign of y. At most 2 ulp from
 // the correctly rounded atan2 (tests/test_oracle_math.py sweeps it against libm); C99 signed-zero conventions.
-NV_HD inline float atan2f_det(float yf, float xf) {
+// atan2f_core takes the one division as div(num, den) (den > 0): IEEE division in atan2f_det, a hardware
+// reciprocal in the GPU's fast ionize pass.
+template <class Div>
+NV_HD inline float atan2f_core(float yf, float xf, Div div) {
   const float ay = fabsf(yf), ax = fabsf(xf);
   const bool steep = ay > ax;
   const float lo = steep ? ax : ay, hi = steep ? ay : ax;
@@ -108,7 +111,7 @@ NV_HD inline float atan2f_det(float yf, float xf) {
     const bool big = lo > TAN_PI_12_F32 * hi;
     const float num = big ? SQRT3_F32 * lo - hi : lo;
     const float den = big ? lo + SQRT3_F32 * hi : hi;
-    const float s = num / den;
+    const float s = div(num, den);
     const float s2 = s * s;
     float q = -1.0f / 11.0f;
     q = q * s2 + 1.0f / 9.0f;
@@ -123,6 +126,11 @@ NV_HD inline float atan2f_det(float yf, float xf) {
   }
   return signbit(yf) ? -a : a;
 }
+
+struct IeeeDiv {
+  NV_HD float operator()(float a, float b) const { return a / b; }
+};
+NV_HD inline float atan2f_det(float yf, float xf) { return atan2f_core(yf, xf, IeeeDiv{}); }
 
 // exp(x) for finite x; results below the float range flush to +0 (x < -104), above it to +inf (x > 89).
 NV_HD inline float expf_det(float xf) {

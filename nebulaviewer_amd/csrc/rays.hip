@@ -295,15 +295,23 @@ __global__ __launch_bounds__(256) void ray_bricks_batch_kernel(const BrickJob* _
   for (int r = threadIdx.x; r <= J.nz; r += 256) brick_cell(J.f, J.B, J.nx, J.ny, J.nz, p, q, r);
 }
 
-// cell centre seen from a star: r, u = cos(theta), phi in [0, 2pi) (nebula3d.py:284-290)
+// a / b through the hardware reciprocal (b > 0; within 2 ulp of the quotient)
+struct RcpDiv {
+  __device__ float operator()(float a, float b) const { return a * __builtin_amdgcn_rcpf(b); }
+};
+
+// cell centre seen from a star: r, u = cos(theta), phi in [0, 2pi) (nebula3d.py:284-290). FAST: the hardware
+// square root (within 1 ulp of the exact one) and the hardware reciprocal for r2 / r and atan2's quotient.
+template <bool FAST = false>
 __device__ __forceinline__ void cell_sph(int i, int j, int k, float sx, float sy, float sz, float dx, float* r_o,
                                          float* u_o, float* phi_o) {
   const float r0 = ((float)i + 0.5f) * dx - sx, r1 = ((float)j + 0.5f) * dx - sy, r2 = ((float)k + 0.5f) * dx - sz;
-  const float r = fmaxf(sqrt_rn((r0 * r0 + r1 * r1) + r2 * r2), 1e-6f);
-  float phi = nvm::atan2f_det(r1, r0);
+  const float s = (r0 * r0 + r1 * r1) + r2 * r2;
+  const float r = fmaxf(FAST ? __builtin_amdgcn_sqrtf(s) : sqrt_rn(s), 1e-6f);
+  float phi = FAST ? nvm::atan2f_core(r1, r0, RcpDiv{}) : nvm::atan2f_det(r1, r0);
   if (phi < 0.0f) phi = phi + 2.0f * PI_F;
   *r_o = r;
-  *u_o = r2 / r;
+  *u_o = FAST ? r2 * __builtin_amdgcn_rcpf(r) : r2 / r;
   *phi_o = phi;
 }
 
@@ -320,6 +328,9 @@ __device__ __forceinline__ int wrap_phi(int ip, int nphi) {
 // div_r needs): div_r(phi, R2PI) is phi / (2.0f * PI_F) bit for bit
 constexpr Recip R2PI = {1.0 / (double)(2.0f * PI_F), 2.0f * PI_F};
 constexpr Recip R6 = {1.0 / 6.0, 6.0f};
+// the same divisions as float products (the fast ionize pass): 1 / (2 pi) and 1 / 6 rounded to float
+constexpr float INV_2PI_F = (float)(1.0 / (double)(2.0f * PI_F));
+constexpr float INV_6_F = (float)(1.0 / 6.0);
 
 // row[pa] and row[pb] for pb = pa + 1 or the wrap pb = 0: one 8-byte load of (q, q+1), q = min(pa, nphi-2), when
 // nphi >= 2 (row[0] loaded on its own only at the wrap)
@@ -349,10 +360,11 @@ struct FrontTaps {
   bool wrap;
 };
 
+template <bool FAST = false>
 __device__ __forceinline__ void front_taps(const float* __restrict__ rfront, int nu, int nphi, float u, float phi,
                                            FrontTaps& T) {
   const float fu = (u + 1.0f) * 0.5f * (float)nu - 0.5f;
-  const float fp = div_r(phi, R2PI) * (float)nphi - 0.5f;
+  const float fp = (FAST ? phi * INV_2PI_F : div_r(phi, R2PI)) * (float)nphi - 0.5f;
   const int iu = (int)floorf(fu), ip = (int)floorf(fp);
   T.tu = fu - (float)iu;
   T.tp = fp - (float)ip;
@@ -475,6 +487,7 @@ struct IonArgs {
   float dx, cn2, ci2;
   Recip rwidth;                    // the ramp width and 1 / width, correctly rounded (its shared reciprocal)
   const float* rf_ranges;          // per star (least, greatest) front radius of its table, or null (no culling)
+  float rw;                        // 1 / width rounded to float (the fast pass's ramp division)
 };
 
 // One 32 z x 16 y tile of the fused ionize pass, marching over x chunk bi.
@@ -491,7 +504,7 @@ struct IonArgs {
 // bound on r (the hardware square root of the same sum cell_sph takes the exact root of, within an ulp) and the
 // star's greatest front radius, each widened by 2^-18 relative, far beyond the rounding of r and of the
 // interpolated rf. A cell away from every front then evaluates one star's direction and front instead of all.
-template <bool SMAX, bool VEL, bool CULL>
+template <bool SMAX, bool VEL, bool CULL, bool FAST>
 __device__ __forceinline__ void ionize_tile(const IonArgs& A, int bk, int bj, int bi) {
   __shared__ float sx_[XR][XN];   // unsmoothed xi (max over stars of the front ramp)
   __shared__ float sd_[XR][XN];   // fp32 dfront (min over stars of r - rf)
@@ -524,11 +537,11 @@ __device__ __forceinline__ void ionize_tile(const IonArgs& A, int bk, int bj, in
     int ic, jc, kc;
     cell_of(p, t, ic, jc, kc);
     float u, phi;
-    cell_sph(ic, jc, kc, stars[3 * st], stars[3 * st + 1], stars[3 * st + 2], A.dx, &r, &u, &phi);
-    front_taps(A.rfronts + st * tab, A.nu, A.nphi, u, phi, T);
+    cell_sph<FAST>(ic, jc, kc, stars[3 * st], stars[3 * st + 1], stars[3 * st + 2], A.dx, &r, &u, &phi);
+    front_taps<FAST>(A.rfronts + st * tab, A.nu, A.nphi, u, phi, T);
   };
   // the distance from star st to the cell centre by the hardware square root of the sum cell_sph forms (within an
-  // ulp of its correctly rounded root)
+  // ulp of its correctly rounded root; with FAST, the bits cell_sph takes before its floor of 1e-6)
   auto r_fast = [&](int ic, int jc, int kc, int st) {
     const float r0 = ((float)ic + 0.5f) * A.dx - stars[3 * st];
     const float r1 = ((float)jc + 0.5f) * A.dx - stars[3 * st + 1];
@@ -558,7 +571,7 @@ __device__ __forceinline__ void ionize_tile(const IonArgs& A, int bk, int bj, in
   };
   // one star's ramp into the running max (xi) and min (dfront)
   auto accumulate = [&](float r, float rf, float& x, float& df) {
-    const float val = fminf(fmaxf(div_r(rf - r, A.rwidth) + 0.5f, 0.0f), 1.0f);
+    const float val = fminf(fmaxf((FAST ? (rf - r) * A.rw : div_r(rf - r, A.rwidth)) + 0.5f, 0.0f), 1.0f);
     if (val > x) x = val;
     df = fminf(df, r - rf);
   };
@@ -634,7 +647,8 @@ __device__ __forceinline__ void ionize_tile(const IonArgs& A, int bk, int bj, in
       const float* pm = sx_[(((i - 1) % XR) + XR) % XR];
       const float* pp = sx_[(((i + 1) % XR) + XR) % XR];
       // neighbours in the order of nebula3d.py:313-314
-      const float m = div_r(pm[c] + pp[c] + pc[c - XK] + pc[c + XK] + pc[c - 1] + pc[c + 1], R6);
+      const float sum = pm[c] + pp[c] + pc[c - XK] + pc[c + XK] + pc[c - 1] + pc[c + 1];
+      const float m = FAST ? sum * INV_6_F : div_r(sum, R6);
       const float xs = 0.5f * pc[c] + 0.5f * m;
       A.xi[o] = xs;
       const float df = sd_[((i % XR) + XR) % XR][c];
@@ -642,12 +656,21 @@ __device__ __forceinline__ void ionize_tile(const IonArgs& A, int bk, int bj, in
       else A.df_f[o] = df;
       if constexpr (SMAX) {
         float s;
+        const float c2 = A.cn2 + xs * (A.ci2 - A.cn2);
+        const float c = FAST ? __builtin_amdgcn_sqrtf(c2) : sqrt_rn(c2);
         if constexpr (VEL) {
-          s = ((fabsf(f1) + fabsf(f2)) + fabsf(f3)) + sqrt_rn(A.cn2 + xs * (A.ci2 - A.cn2));
+          s = ((fabsf(f1) + fabsf(f2)) + fabsf(f3)) + c;
         } else {
           float v0, v1, v2;
-          div3(f1, f2, f3, f0, &v0, &v1, &v2);
-          s = ((fabsf(v0) + fabsf(v1)) + fabsf(v2)) + sqrt_rn(A.cn2 + xs * (A.ci2 - A.cn2));
+          if constexpr (FAST) {
+            const float ir = __builtin_amdgcn_rcpf(f0);
+            v0 = f1 * ir;
+            v1 = f2 * ir;
+            v2 = f3 * ir;
+          } else {
+            div3(f1, f2, f3, f0, &v0, &v1, &v2);
+          }
+          s = ((fabsf(v0) + fabsf(v1)) + fabsf(v2)) + c;
         }
         tmax = fmaxf(tmax, s);
       }
@@ -673,17 +696,17 @@ __device__ __forceinline__ void ionize_tile(const IonArgs& A, int bk, int bj, in
 // One tile per workgroup, each XCD a contiguous run of tiles (z fastest, then y, then the x chunk): its cells then
 // see a narrower cone of directions from each star, so the front-table rows they gather stay in the XCD's L2 (the
 // 512^3 wall's pass: 1.07 against 1.15 ms with the tiles dealt round robin; the same bits).
-template <bool SMAX, bool VEL>
+template <bool SMAX, bool VEL, bool FAST>
 __global__ __launch_bounds__(INT_) void ionize_apply_kernel(IonArgs A, int per_xcd) {
   const int t = xcd_tile(per_xcd);
   if (t >= A.tiles) return;
-  ionize_tile<SMAX, VEL, false>(A, t % A.tiles_k, (t / A.tiles_k) % A.tiles_j, t / (A.tiles_k * A.tiles_j));
+  ionize_tile<SMAX, VEL, false, FAST>(A, t % A.tiles_k, (t / A.tiles_k) % A.tiles_j, t / (A.tiles_k * A.tiles_j));
 }
 
 // Many grids in one launch: tile t of the flat list belongs to member s with start[s] <= t < start[s+1]. The tiles
 // are dealt round robin to the XCDs: members differ in their star count, so contiguous runs of equal length would
 // give the XCDs unequal work (measured on the 64-member batch: 2.34 ms against 2.04).
-template <bool SMAX, bool VEL, bool CULL>
+template <bool SMAX, bool VEL, bool CULL, bool FAST>
 __global__ __launch_bounds__(INT_) void ionize_apply_batch_kernel(const IonArgs* __restrict__ args,
                                                                   const int* __restrict__ start, int n, int total) {
   const int t = blockIdx.x;
@@ -697,7 +720,7 @@ __global__ __launch_bounds__(INT_) void ionize_apply_batch_kernel(const IonArgs*
   const IonArgs& A = args[lo];
   const int l = t - start[lo];
   const int bk = l % A.tiles_k, bj = (l / A.tiles_k) % A.tiles_j, bi = l / (A.tiles_k * A.tiles_j);
-  ionize_tile<SMAX, VEL, CULL>(A, bk, bj, bi);
+  ionize_tile<SMAX, VEL, CULL, FAST>(A, bk, bj, bi);
 }
 
 // ---------------------------------------------------------------------------------------- FUV
@@ -1049,6 +1072,7 @@ static int fill_ionize(const char* name, const float* rfronts, int nu, int nphi,
   A.nu = nu; A.nphi = nphi; A.nstars = nstars; A.nx = nx; A.ny = ny; A.nz = nz;
   A.dx = dx; A.cn2 = cn2; A.ci2 = ci2;
   A.rwidth = Recip{1.0 / (double)width, width};
+  A.rw = 1.0f / width;
   A.rf_ranges = nullptr;
   A.ci = (int)std::min<long long>(64, std::max<long long>(4, (long long)nx * jk_total / 2048));
   A.tiles_k = (nz + IK - 1) / IK;
@@ -1059,18 +1083,27 @@ static int fill_ionize(const char* name, const float* rfronts, int nu, int nphi,
 
 static long long ion_jk(int ny, int nz) { return (long long)((nz + IK - 1) / IK) * ((ny + IJ - 1) / IJ); }
 
-static int ionize_apply_launch(const char* name, bool vel, const float* rfronts, int nu, int nphi,
+// the kernel instance of a form (NV_ION_VEL, NV_ION_FAST bits) with or without the CFL maximum
+template <bool FAST>
+static auto ionize_kernel_of(bool smax, bool vel) {
+  return !smax ? ionize_apply_kernel<false, false, FAST>
+               : (vel ? ionize_apply_kernel<true, true, FAST> : ionize_apply_kernel<true, false, FAST>);
+}
+
+static int ionize_apply_launch(const char* name, int form, const float* rfronts, int nu, int nphi,
                                const float* star_xyz, int nstars, float dx, float width, float* xi, uint16_t* dfront_h,
                                float* dfront_f, int nx, int ny, int nz, const void* f0, const void* f1,
                                const void* f2, const void* f3, float cn2, float ci2, float* smax_out, void* stream) {
   clear_error();
+  NV_REQUIRE(form >= 0 && form <= (NV_ION_VEL | NV_ION_FAST), "%s: bad form %d", name, form);
   IonArgs A;
   const int rc = fill_ionize(name, rfronts, nu, nphi, star_xyz, nstars, dx, width, xi, dfront_h, dfront_f, nx, ny, nz,
                              f0, f1, f2, f3, cn2, ci2, smax_out, ion_jk(ny, nz), A);
   if (rc != NV_OK) return rc;
   const int per_xcd = (A.tiles + 7) / 8;
-  auto kern = !smax_out ? ionize_apply_kernel<false, false>
-                        : (vel ? ionize_apply_kernel<true, true> : ionize_apply_kernel<true, false>);
+  const bool vel = (form & NV_ION_VEL) != 0;
+  auto kern = (form & NV_ION_FAST) ? ionize_kernel_of<true>(smax_out != nullptr, vel)
+                                   : ionize_kernel_of<false>(smax_out != nullptr, vel);
   hipLaunchKernelGGL(kern, dim3(8 * per_xcd), dim3(IK, IJ), 0, as_stream(stream), A, per_xcd);
   return check_launch(name);
 }
@@ -1079,7 +1112,7 @@ extern "C" int nv_ionize_apply(const float* rfronts, int nu, int nphi, const flo
                                float width, float* xi, uint16_t* dfront_h, float* dfront_f, int nx, int ny, int nz,
                                const float* rho, const float* mx, const float* my, const float* mz, float cn2,
                                float ci2, float* smax_out, void* stream) {
-  return ionize_apply_launch("nv_ionize_apply", false, rfronts, nu, nphi, star_xyz, nstars, dx, width, xi, dfront_h,
+  return ionize_apply_launch("nv_ionize_apply", 0, rfronts, nu, nphi, star_xyz, nstars, dx, width, xi, dfront_h,
                              dfront_f, nx, ny, nz, rho, mx, my, mz, cn2, ci2, smax_out, stream);
 }
 
@@ -1087,8 +1120,16 @@ extern "C" int nv_ionize_apply_h(const float* rfronts, int nu, int nphi, const f
                                  float width, float* xi, uint16_t* dfront_h, float* dfront_f, int nx, int ny, int nz,
                                  const float* rho, const uint16_t* vx, const uint16_t* vy, const uint16_t* vz,
                                  float cn2, float ci2, float* smax_out, void* stream) {
-  return ionize_apply_launch("nv_ionize_apply_h", true, rfronts, nu, nphi, star_xyz, nstars, dx, width, xi, dfront_h,
+  return ionize_apply_launch("nv_ionize_apply_h", NV_ION_VEL, rfronts, nu, nphi, star_xyz, nstars, dx, width, xi, dfront_h,
                              dfront_f, nx, ny, nz, rho, vx, vy, vz, cn2, ci2, smax_out, stream);
+}
+
+extern "C" int nv_ionize_apply_ex(const NvIonDesc* d, int form, void* stream) {
+  clear_error();
+  NV_REQUIRE(d, "nv_ionize_apply_ex: null descriptor");
+  return ionize_apply_launch("nv_ionize_apply_ex", form, d->rfronts, d->nu, d->nphi, d->star_xyz, d->nstars, d->dx,
+                             d->width, d->xi, d->dfront_h, d->dfront_f, d->nx, d->ny, d->nz, d->state[0], d->state[1],
+                             d->state[2], d->state[3], d->cn2, d->ci2, d->smax_out, stream);
 }
 
 extern "C" int nv_cast_fuv(const float* rho, const float* xi, int nx, int ny, int nz, float sx, float sy, float sz,
@@ -1171,7 +1212,8 @@ struct RayBatchHeader {
   int n, total, pair, off32;
   int nb, btotal;          // ray plans: brick jobs and their blocks
   int ranges;              // some job asks for its front range (ray plans) or member carries them (ionize plans)
-  int pad[8];
+  int fast;                // ionize plans: the fast arithmetic (NV_ION_FAST)
+  int pad[7];
 };
 static_assert(sizeof(RayBatchHeader) == 64, "plan header layout");
 constexpr uint32_t kRayMagic = 0x4e565231u;   // "NVR1"
@@ -1301,8 +1343,10 @@ extern "C" int nv_cast_rays_batch(const void* plan_host, const void* plan_dev, v
 
 extern "C" long long nv_ionize_apply_batch_plan_size(int n) { return n > 0 ? batch_bytes<IonArgs>(n) : -1; }
 
-extern "C" int nv_ionize_apply_batch_plan(const NvIonDesc* members, int n, int vel, void* plan, long long plan_bytes) {
+extern "C" int nv_ionize_apply_batch_plan(const NvIonDesc* members, int n, int form, void* plan,
+                                          long long plan_bytes) {
   clear_error();
+  NV_REQUIRE(form >= 0 && form <= (NV_ION_VEL | NV_ION_FAST), "nv_ionize_apply_batch_plan: bad form %d", form);
   NV_REQUIRE(members && plan && n > 0, "nv_ionize_apply_batch_plan: need n > 0 members and a plan buffer");
   NV_REQUIRE(plan_bytes >= batch_bytes<IonArgs>(n), "nv_ionize_apply_batch_plan: plan needs %lld bytes",
              batch_bytes<IonArgs>(n));
@@ -1339,7 +1383,8 @@ extern "C" int nv_ionize_apply_batch_plan(const NvIonDesc* members, int n, int v
   H.n = n;
   H.total = (int)total;
   H.pair = sm ? 1 : 0;     // reused: the CFL reduction is on
-  H.off32 = vel ? 1 : 0;   // reused: binary16-velocity state
+  H.off32 = (form & NV_ION_VEL) ? 1 : 0;   // reused: binary16-velocity state
+  H.fast = (form & NV_ION_FAST) ? 1 : 0;
   H.ranges = ranges ? 1 : 0;
   memcpy(plan, &H, sizeof(H));
   return NV_OK;
@@ -1353,12 +1398,14 @@ extern "C" int nv_ionize_apply_batch(const void* plan_host, const void* plan_dev
   NV_REQUIRE(H.magic == kIonMagic && H.n > 0, "nv_ionize_apply_batch: not an ionize batch plan");
   const IonArgs* args = reinterpret_cast<const IonArgs*>(static_cast<const char*>(plan_dev) + sizeof(H));
   const int* start = reinterpret_cast<const int*>(args + H.n);
-  auto pick = [&](auto cull) {
-    constexpr bool C = decltype(cull)::value;
-    return !H.pair ? ionize_apply_batch_kernel<false, false, C>
-                   : (H.off32 ? ionize_apply_batch_kernel<true, true, C> : ionize_apply_batch_kernel<true, false, C>);
+  auto pick = [&](auto cull, auto fast) {
+    constexpr bool C = decltype(cull)::value, F = decltype(fast)::value;
+    return !H.pair ? ionize_apply_batch_kernel<false, false, C, F>
+                   : (H.off32 ? ionize_apply_batch_kernel<true, true, C, F>
+                              : ionize_apply_batch_kernel<true, false, C, F>);
   };
-  auto kern = H.ranges ? pick(std::true_type{}) : pick(std::false_type{});
+  auto by_cull = [&](auto fast) { return H.ranges ? pick(std::true_type{}, fast) : pick(std::false_type{}, fast); };
+  auto kern = H.fast ? by_cull(std::true_type{}) : by_cull(std::false_type{});
   hipLaunchKernelGGL(kern, dim3(H.total), dim3(IK, IJ), 0, as_stream(stream), args, start, H.n, H.total);
   return check_launch("nv_ionize_apply_batch");
 }

@@ -29,6 +29,7 @@ rays, and every ray of grids up to 294 cells on a side, are bit-identical to the
 reference's float column on every ray (nebula3d.py:262-268).
 """
 import copy
+import ctypes
 import math
 import time
 
@@ -121,6 +122,32 @@ class Sim:
     @property
     def vel_state(self):
         return self.precision == "fp16-vel"
+
+    @property
+    def ion_form(self):
+        """the fused ionize pass's form (NV_ION_* bits): the fast precision forms evaluate it in fast arithmetic"""
+        return (_lib.NV_ION_VEL if self.vel_state else 0) | (_lib.NV_ION_FAST if self.precision != "fp32" else 0)
+
+    def ion_desc(self, nstars, width):
+        """NvIonDesc of this grid's fused ionize pass over its first nstars rfront tables"""
+        nx, ny, nz = self.shape
+        half = self.dfront.dtype == torch.float16
+        d = _lib.NvIonDesc()
+        d.rfronts = self.rfronts.data_ptr()
+        d.nu, d.nphi = self.rfronts.shape[1:]
+        d.star_xyz = self.star_xyz.data_ptr()
+        d.nstars = nstars
+        d.dx = self.dx
+        d.width = width
+        d.xi = self.xi.data_ptr()
+        d.dfront_h = self.dfront.data_ptr() if half else None
+        d.dfront_f = None if half else self.dfront.data_ptr()
+        d.nx, d.ny, d.nz = nx, ny, nz
+        for c in range(4):
+            d.state[c] = self.U[c].data_ptr()
+        d.cn2, d.ci2 = self.cn2, self.ci2
+        d.smax_out = self.smax.data_ptr()
+        return d
 
     @property
     def rho(self):
@@ -417,11 +444,9 @@ class Sim:
             if len(on):
                 self.star_xyz[:len(on)].copy_(torch.from_numpy(xyz))
             self.smax.zero_()
+            d = self.ion_desc(len(on), width)
             tk = self._t0("ionize_apply")
-            _lib.call("nv_ionize_apply_h" if self.vel_state else "nv_ionize_apply", self._p(self.rfronts), nu,
-                      nphi, self._p(self.star_xyz), len(on), _f(self.dx), _f(width), self._p(self.xi),
-                      self._p(self.dfront) if half else None, None if half else self._p(self.dfront), nx, ny, nz,
-                      *[self._p(t) for t in self.U], _f(self.cn2), _f(self.ci2), self._p(self.smax), strm)
+            _lib.call("nv_ionize_apply_ex", ctypes.byref(d), self.ion_form, strm)
             self._t1(tk)
             self._smax_ok = True
         else:

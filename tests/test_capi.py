@@ -40,7 +40,7 @@ def test_every_declared_symbol_is_exported_and_bound(lib):
 
 
 def test_abi_version(lib):
-    assert lib.nv_abi_version() == 4
+    assert lib.nv_abi_version() == 5
 
 
 def struct_fields(name):
@@ -127,6 +127,7 @@ def test_batch_plans_validate(lib):
     ions = (_lib.NvIonDesc * 1)()
     ibuf = ctypes.create_string_buffer(int(lib.nv_ionize_apply_batch_plan_size(1)))
     assert lib.nv_ionize_apply_batch_plan(ions, 1, 0, ibuf, len(ibuf)) == 1001
+    assert lib.nv_ionize_apply_batch_plan(ions, 1, 4, ibuf, len(ibuf)) == 1001 and b"form" in lib.nv_last_error()
     assert lib.nv_hydro_stage_batch(buf, buf, None) == 1001     # a refused plan never got its header
     assert lib.nv_hydro_stage_batch_queue_ex(buf, buf, 0, None) == 1001
     assert lib.nv_hydro_stage_batch_queue_ex(buf, buf, -1, None) == 1001 and b"wgs_per_cu" in lib.nv_last_error()
@@ -157,6 +158,17 @@ def test_rejects_bad_enums_and_sizes(lib):
     rc = lib.nv_ionize_apply(p, 4, 4, p, 1, f(0.1), f(0.1), p, p, p, 8, 8, 8, None, None, None, None, f(0), f(1),
                              None, None)
     assert rc == 1001 and b"exactly one" in lib.nv_last_error()
+    # the descriptor form: a null descriptor, a form outside the NV_ION_* bits, the same argument checks
+    assert lib.nv_ionize_apply_ex(None, 0, None) == 1001 and b"null" in lib.nv_last_error()
+    d = _lib.NvIonDesc()
+    d.rfronts, d.star_xyz, d.xi, d.dfront_h = 16, 16, 16, 16
+    d.nu = d.nphi = 4
+    d.nstars, d.dx, d.width = 1, 0.1, 0.1
+    d.nx = d.ny = d.nz = 8
+    assert lib.nv_ionize_apply_ex(ctypes.byref(d), 4, None) == 1001 and b"form" in lib.nv_last_error()
+    d.dfront_f = 16
+    assert lib.nv_ionize_apply_ex(ctypes.byref(d), _lib.NV_ION_FAST, None) == 1001
+    assert b"exactly one" in lib.nv_last_error()
     rc = lib.nv_cast_rays(p, 8, 8, 8, f(0), f(0), f(0), f(1), f(0), 10, f(0.1), f(1), f(1), f(1), f(1), p, 4, 4,
                           None)
     assert rc == 1001 and b"dr" in lib.nv_last_error()

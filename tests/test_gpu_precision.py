@@ -115,6 +115,82 @@ def run_torus(env, precision, steps, n=64):
     return sim
 
 
+@pytest.mark.parametrize("nst", [1, 3])
+def test_fast_ionize_pass_against_fp64(env, okern32, okern64, nst):
+    """The fused ionize pass in fast arithmetic (NV_ION_FAST) against the FP64 oracle's apply_front + smooth_xi on
+    the same fronts (the exact pass's casts), beside the exact pass: its xi, float dfront and CFL maximum are as
+    close to FP64 as the exact pass's (its largest xi and dfront errors within 4x the exact pass's plus 1e-6, no
+    more cells off by > 1e-3, the CFL maximum within 1e-6 relative)."""
+    say(f"[fast ionize vs FP64] nstars={nst}")
+    torch, dev, L = env
+    shape = (48, 40, 56)
+    rng = np.random.default_rng(31)
+    rho = rng.lognormal(0, 1.0, shape).astype(F32)
+    U = [rho] + [(rho * rng.normal(0, 0.5, shape)).astype(F32) for _ in range(3)]
+    dx = 1.0 / shape[2]
+    Lb = tuple(n * dx for n in shape)
+    diag = math.sqrt(sum(v * v for v in Lb)) + 0.6
+    nu, nphi = 64, 128
+    dr = 0.5 * dx
+    nsteps = int(diag / dr) + 1
+    stars = [(0.4 * Lb[0], 0.35 * Lb[1], 0.5 * Lb[2]), (0.9 * Lb[0], 0.1 * Lb[1], 0.2 * Lb[2]),
+             (0.2 * Lb[0], 0.8 * Lb[1], 0.6 * Lb[2])][:nst]
+    Qs = [0.01, 0.05, 0.002]
+    width, cn2, ci2 = 1.2 * dx, 1 / 144, 1.0
+    gU = [torch.from_numpy(u).to(dev) for u in U]
+    grf = torch.zeros((nst, nu, nphi), dtype=torch.float32, device=dev)
+    strm = torch.cuda.current_stream().cuda_stream
+    for k, st in enumerate(stars):
+        L.call("nv_cast_rays", gU[0].data_ptr(), *shape, *(f(v) for v in st), f(Qs[k]), f(dr), nsteps, f(dx),
+               f(diag), *(f(v) for v in Lb), grf[k].data_ptr(), nu, nphi, strm)
+    xyz = torch.from_numpy(np.array(stars, F32).reshape(-1, 3)).to(dev)
+    out = {}
+    for form in (0, L.NV_ION_FAST):
+        gxi = torch.full(shape, 7.0, dtype=torch.float32, device=dev)
+        gdf = torch.zeros(shape, dtype=torch.float32, device=dev)
+        smax = torch.zeros(1, dtype=torch.float32, device=dev)
+        d = L.NvIonDesc()
+        d.rfronts, d.nu, d.nphi = grf.data_ptr(), nu, nphi
+        d.star_xyz, d.nstars, d.dx, d.width = xyz.data_ptr(), nst, dx, width
+        d.xi, d.dfront_h, d.dfront_f = gxi.data_ptr(), None, gdf.data_ptr()
+        d.nx, d.ny, d.nz = shape
+        for c in range(4):
+            d.state[c] = gU[c].data_ptr()
+        d.cn2, d.ci2, d.smax_out = cn2, ci2, smax.data_ptr()
+        L.call("nv_ionize_apply_ex", ctypes.byref(d), form, strm)
+        torch.cuda.synchronize()
+        out[form] = (gxi.cpu().numpy().astype(np.float64), gdf.cpu().numpy().astype(np.float64), float(smax.item()))
+    rfs = grf.cpu().numpy()
+    # the exact pass is the oracle's float32 sequence bit for bit
+    xi32 = np.zeros(shape, F32)
+    df32 = np.full(shape, 1e3, F32)
+    for k, st in enumerate(stars):
+        okern32.apply_front(rfs[k], st, dx, width, xi32, df32)
+    sm32 = np.zeros(shape, F32)
+    okern32.smooth_xi(xi32, sm32)
+    assert np.array_equal(out[0][0], sm32) and np.array_equal(out[0][1], df32)
+    # the FP64 oracle on the same fronts
+    xi64 = np.zeros(shape, np.float64)
+    df64 = np.full(shape, 1e3, np.float64)
+    for k, st in enumerate(stars):
+        okern64.apply_front(rfs[k].astype(np.float64), st, dx, width, xi64, df64)
+    sm64 = np.zeros(shape, np.float64)
+    okern64.smooth_xi(xi64, sm64)
+    smax64 = okern64.max_speed(*[u.astype(np.float64) for u in U], sm64, cn2, ci2)
+    stats = {}
+    for form, (xi, df, sm) in out.items():
+        e = np.abs(xi - sm64)
+        stats[form] = (float(e.max()), int((e > 1e-3).sum()), float(np.abs(df - df64).max()), abs(sm / smax64 - 1))
+        say(f"  form {form}: xi max err {stats[form][0]:.2e}, cells > 1e-3: {stats[form][1]}, "
+            f"dfront max err {stats[form][2]:.2e}, smax rel err {stats[form][3]:.2e}")
+    ex, fa = stats[0], stats[L.NV_ION_FAST]
+    assert not np.array_equal(out[0][0], out[L.NV_ION_FAST][0]), "the fast form ran the exact arithmetic"
+    assert fa[0] <= 4 * ex[0] + 1e-6, (fa, ex)
+    assert fa[1] <= ex[1], (fa, ex)
+    assert fa[2] <= 4 * ex[2] + 1e-6, (fa, ex)
+    assert fa[3] <= 1e-6, (fa, ex)
+
+
 def test_fast_sim_within_dropin_tolerance(env):
     exact = run_torus(env, "fp32", 100)
     fast = run_torus(env, "fp32-fast", 100)
