@@ -272,7 +272,8 @@ __global__ __launch_bounds__(256) void cast_rays_batch_kernel(const RayJob* __re
 }
 
 // The brick arrays of several grids in one launch: block b belongs to brick job s with bstart[s] <= b <
-// bstart[s+1]; each job's blocks are its padded rows, (nx + 2)(ny + 1) of them (ray_bricks_kernel's work).
+// bstart[s+1]; each job's blocks take 256 consecutive bricks of its array in storage order (ray_bricks_kernel's
+// work, without its one-row-per-block idle threads: the batch's rows are 49-161 bricks long).
 struct BrickJob {
   const float* f;
   float4* B;
@@ -290,9 +291,11 @@ __global__ __launch_bounds__(256) void ray_bricks_batch_kernel(const BrickJob* _
     else hi = mid;
   }
   const BrickJob& J = jobs[lo];
-  const int row = b - bstart[lo];   // row = p (ny + 1) + q, one padded row per block
-  const int p = row / (J.ny + 1), q = row - p * (J.ny + 1);
-  for (int r = threadIdx.x; r <= J.nz; r += 256) brick_cell(J.f, J.B, J.nx, J.ny, J.nz, p, q, r);
+  const unsigned w = (unsigned)J.nz + 1u, h = (unsigned)J.ny + 1u;   // bricks per padded row, rows per plane
+  const unsigned idx = (unsigned)(b - bstart[lo]) * 256u + threadIdx.x;   // < (nx + 2)(ny + 1)(nz + 1) < 2^31
+  const unsigned row = idx / w, r = idx - row * w;                         // row = p (ny + 1) + q
+  const unsigned p = row / h, q = row - p * h;
+  if (p < (unsigned)J.nx + 2u) brick_cell(J.f, J.B, J.nx, J.ny, J.nz, (int)p, (int)q, (int)r);
 }
 
 // a / b through the hardware reciprocal (b > 0; within 2 ulp of the quotient)
@@ -1292,9 +1295,11 @@ extern "C" int nv_cast_rays_batch_plan(const NvRayDesc* jobs, int n, void* plan,
         B.nx = d.nx; B.ny = d.ny; B.nz = d.nz;
         B.pad = 0;
         memcpy(&bj[nb], &B, sizeof(B));
+        const long long nbricks = (long long)(d.nx + 2) * (d.ny + 1) * (d.nz + 1);
+        NV_REQUIRE(nbricks < (1LL << 31), "nv_cast_rays_batch_plan: job %d: above 2^31 bricks", s);
         bstart[nb++] = (int)btotal;
-        btotal += (long long)(d.nx + 2) * (d.ny + 1);
-        NV_REQUIRE(btotal < (1LL << 31) - 1, "nv_cast_rays_batch_plan: too many brick rows");
+        btotal += (nbricks + 255) / 256;
+        NV_REQUIRE(btotal < (1LL << 31) - 1, "nv_cast_rays_batch_plan: too many brick blocks");
       }
     }
     memcpy(&out[s], &J, sizeof(J));
