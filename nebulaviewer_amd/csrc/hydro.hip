@@ -34,6 +34,9 @@
 //               to binary16 every stage stalls updates below half an fp16 ulp (measured: rel-L1 density error
 //               4e-3 after 100 steps of the 64^3 torus against 1.4e-4 with float32 density).
 #include <string.h>
+#include <map>
+#include <mutex>
+#include <utility>
 #include "nv_common.h"
 #include "nv_divide.h"
 
@@ -64,6 +67,7 @@ struct HydroArgs {
   float* smax;
   int nx, ny, nz, nstars, ci, u0_is_u;
   int tiles_k, tiles_j, tiles, per_xcd;     // tile grid and the number of tiles each XCD takes
+  int run;                                  // tiles an XCD takes in a row before the next XCD's (band_tile)
   float wind, jet_cos, jet_r, pulse, dt, dx, a, b, floor_, cn2, ci2, vmax;
   float vmax2_lo;                           // largest float <= vmax^2: |v|^2 <= vmax2_lo means no cap
   float inv_dx;                             // 1/dx when dx is a power of two (exact), else 0
@@ -754,12 +758,21 @@ __device__ __forceinline__ void stage_tile(const HydroArgs& A, int tile) {
   if constexpr (SMAX) block_max_atomic(tmax, red, A.smax);
 }
 
-// Each XCD takes a contiguous run of tiles (xcd_tile: z fastest, then y, then the x chunk, then the next member of a
-// batch), so the tiles whose halos overlap run on one XCD and share its L2.
+// Each XCD takes runs of consecutive tiles (z fastest, then y, then the x chunk, then the next member of a batch),
+// so the tiles whose halos overlap run on one XCD and share its L2. With run = per_xcd an XCD takes one contiguous
+// run (xcd_tile). When the tiles outnumber the workgroups the device holds at once, run is the number an XCD holds:
+// the XCDs then take bands of run tiles side by side, band after band, so the workgroups resident at one time
+// cover 8 run consecutive tiles, and a band's neighbour rows are being read at the same time by the next XCD
+// (through the shared memory-side cache) rather than a whole round of workgroups earlier or later. The dispatcher
+// starts an XCD's workgroups in launch order, so its s-th workgroup takes tile (s / run) 8 run + x run + s % run.
+__device__ __forceinline__ int band_tile(int per_xcd, int run) {
+  const int x = blockIdx.x & 7, s = blockIdx.x >> 3;
+  return run >= per_xcd ? x * per_xcd + s : (s / run) * (8 * run) + x * run + (s - (s / run) * run);
+}
 
 template <class TL, class ST, class AR, int SOLVER, int LIM, bool SMAX, bool STAGE1>
 __global__ __launch_bounds__(TL::NT, 4) void hydro_stage_kernel(HydroArgs A) {
-  const int tile = xcd_tile(A.per_xcd);
+  const int tile = band_tile(A.per_xcd, A.run);
   if (tile >= A.tiles) return;
   stage_tile<TL, ST, AR, SOLVER, LIM, SMAX, STAGE1>(A, tile);
 }
@@ -1037,6 +1050,7 @@ static int fill_stage(const char* name, const NvStageDesc& d, int solver, int li
   A.tiles_j = (d.ny + tile_tj(tile) - 1) / tile_tj(tile);
   A.tiles = A.tiles_k * A.tiles_j * ((d.nx + A.ci - 1) / A.ci);
   A.per_xcd = (A.tiles + 7) / 8;
+  A.run = A.per_xcd;
   return NV_OK;
 }
 
@@ -1045,6 +1059,26 @@ static long long tiles_jk(int tile, int ny, int nz) {
 }
 
 static bool is_stage1(const HydroArgs& A) { return A.u0_is_u && A.a == 0.0f && A.b == 1.0f; }
+
+// workgroups of kern (nt threads) one XCD holds at once: its CUs times the kernel's occupancy per CU; 0 when the
+// device cannot be queried (the launch then keeps one contiguous run per XCD). Cached per kernel and device.
+static int resident_per_xcd(HydroKernel kern, int nt) {
+  static std::mutex mu;
+  static std::map<std::pair<const void*, int>, int> cache;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  std::lock_guard<std::mutex> lock(mu);
+  const auto key = std::make_pair(reinterpret_cast<const void*>(kern), dev);
+  const auto it = cache.find(key);
+  if (it != cache.end()) return it->second;
+  int cus = 0, per_cu = 0;
+  int r = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, nt, 0) == hipSuccess && per_cu > 0)
+    r = cus / 8 * per_cu;
+  cache[key] = r;
+  return r;
+}
 
 static int launch_desc(Mode mode, const char* name, const NvStageDesc& d, int solver, int lim, int tile,
                        void* stream) {
@@ -1057,6 +1091,13 @@ static int launch_desc(Mode mode, const char* name, const NvStageDesc& d, int so
   const int rc = fill_stage(name, d, solver, lim, tile, tiles_jk(tile, d.ny, d.nz), A, depth);
   if (rc != NV_OK) return rc;
   HydroKernel kern = pick_mode(tile, mode, solver, lim, d.smax_out != nullptr, is_stage1(A));
+  // more tiles than one round of resident workgroups: bands of as many tiles as an XCD holds (band_tile); the
+  // launch then covers whole bands (the tiles past A.tiles return at once)
+  const int per_xcd_res = resident_per_xcd(kern, tile_tk(tile) * tile_tj(tile));
+  if (per_xcd_res > 0 && A.per_xcd > per_xcd_res) {
+    A.run = per_xcd_res;
+    A.per_xcd = (A.per_xcd + per_xcd_res - 1) / per_xcd_res * per_xcd_res;
+  }
   hipLaunchKernelGGL(kern, dim3(8 * A.per_xcd), dim3(tile_tk(tile), tile_tj(tile), 1), 0, as_stream(stream), A);
   return check_launch(name);
 }

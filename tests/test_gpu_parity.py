@@ -109,9 +109,11 @@ def test_hydro_stage_bitwise(env, okern32, shape, solver, lim, mode):
     assert float(smax.item()) == ref_smax
 
 
-@pytest.mark.parametrize("shape", [(12, 9, 14), (33, 40, 96), (9, 35, 160), (17, 24, 64)])
+# (40, 48, 192) with one-plane x chunks has more tiles than the device holds workgroups at once (720 of 64 x 8, 360
+# of 64 x 16): those launches deal the tiles to the XCDs in bands
+@pytest.mark.parametrize("shape", [(12, 9, 14), (33, 40, 96), (9, 35, 160), (17, 24, 64), (40, 48, 192)])
 # shapes 1-3, and shapes with a march depth (NV_TILE_DEPTH: one-plane and three-plane x chunks, whole columns)
-@pytest.mark.parametrize("tile", [1, 2, 3, 1 | (1 << 8), 2 | (3 << 8), 3 | (1 << 23), 1 | (1 << 15)])
+@pytest.mark.parametrize("tile", [1, 2, 3, 1 | (1 << 8), 2 | (3 << 8), 3 | (1 << 8), 3 | (1 << 23), 1 | (1 << 15)])
 @pytest.mark.parametrize("precision", [0, 2])
 def test_hydro_stage_tile_shapes(env, okern32, shape, tile, precision):
     """nv_hydro_stage_ex with each workgroup tile shape and march depth: bitwise the oracle (FP32 exact), bitwise
