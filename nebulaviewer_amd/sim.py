@@ -55,7 +55,7 @@ class Sim:
 
     def __init__(self, scenario="torus", res=None, params=None, ic_mode="nebula3d", rays=None, fuv_grid=None,
                  ion_every=None, device="cuda:0", verbose=True, fast=True, setup=None, fused=True, seed=None,
-                 precision="fp32", long_rays="fp64", tile="tune", ray_bricks="auto"):
+                 precision="fp32", long_rays="fp64", tile="tune", ray_bricks="auto", device_clock=True):
         # the reference's constructor (sim.py:553-565): positional order up to fast, fast=True as the viewer runs it
         # (1-voxel rays, ionization every >= 3 steps, sim.py:743, 758); bake.py and the benchmark pass fast=False
         if ic_mode not in self.IC_MODES:
@@ -80,6 +80,10 @@ class Sim:
         if ray_bricks not in ("auto", True, False):
             raise ValueError("ray_bricks must be 'auto', True or False")
         self.ray_bricks = ray_bricks
+        # main-phase steps whose host-side state does not follow the clock take their dt on the device (nv_cfl_dt):
+        # no readback per step, the same bits (False: the host computes every dt, sim.py:1009-1012)
+        self.device_clock = bool(device_clock)
+        self._clk = None
         self.device = torch.device(device)
         self.fast = fast
         self.verbose = verbose
@@ -246,6 +250,7 @@ class Sim:
         self.star_xyz = torch.zeros((self.cap, 3), **f32)
         self.lums = torch.zeros(self.cap, **f32)
         self._wkey = None
+        self._xyz_key = None
         self._smax_ok = False
         self.generation = getattr(self, "generation", 0) + 1
         nu, nphi = self.rays_override or S.ray_grid(nz, self.fast)
@@ -378,6 +383,15 @@ class Sim:
         jw += [(0.0, 0.0, 1.0, 0.0)] * (self.cap - len(jw))
         self.jet_arr.copy_(torch.tensor(np.array(jw, dtype=np.float32)))
 
+    def _upload_xyz(self, on):
+        """star_xyz[:len(on)] = the positions of the switched-on stars; the copy (from pageable host memory, which
+        waits for the stream) only when they changed"""
+        xyz = np.array([self.stars[i][:3] for i in on], dtype=np.float32).reshape(-1, 3)
+        key = xyz.tobytes()
+        if len(on) and key != self._xyz_key:
+            self.star_xyz[:len(on)].copy_(torch.from_numpy(xyz))
+        self._xyz_key = key
+
     def replace_stars(self, stars, kinds, jets, jkinds):
         self.stars = [tuple(float(v) for v in s) for s in stars]
         self.kinds = list(kinds)
@@ -393,6 +407,7 @@ class Sim:
             self.rfronts = torch.zeros((self.cap,) + tuple(self.rfronts.shape[1:]), **f32)
             self.atts = None
         self._wkey = None
+        self._xyz_key = None
         self._set_star_weights()
         self.ionize()
 
@@ -440,9 +455,7 @@ class Sim:
         width = 1.2 * self.dx
         half = self.dfront.dtype == torch.float16
         if self.fused:
-            xyz = np.array([self.stars[i][:3] for i in on], dtype=np.float32).reshape(-1, 3)
-            if len(on):
-                self.star_xyz[:len(on)].copy_(torch.from_numpy(xyz))
+            self._upload_xyz(on)
             self.smax.zero_()
             d = self.ion_desc(len(on), width)
             tk = self._t0("ionize_apply")
@@ -489,10 +502,9 @@ class Sim:
                       nx, ny, nz, _f(sx), _f(sy), _f(sz),
                       _f(self.P["kappa_fuv"]), _f(self.dr), self.nsteps, _f(self.dx), _f(binr), _f(Lx), _f(Ly),
                       _f(Lz), self._p(self.atts[slot]), fnr, fnu, fnp, strm)
-        xyz = np.array([self.stars[i][:3] for i in on], dtype=np.float32).reshape(-1, 3)
         lum = np.array([self.q_eff(i) / self.P["q_main"] for i in on], dtype=np.float32)
+        self._upload_xyz(on)
         if len(on):
-            self.star_xyz[:len(on)].copy_(torch.from_numpy(xyz))
             self.lums[:len(on)].copy_(torch.from_numpy(lum))
         half = self.g0.dtype == torch.float16
         _lib.call("nv_fuv_apply", self._p(self.atts), fnr, fnu, fnp, self._p(self.star_xyz), self._p(self.lums),
@@ -520,17 +532,21 @@ class Sim:
         self._smax_ok = False
         self.version += 1
 
+    def _max_speed(self):
+        """the CFL signal speed of the current state into smax (nebula3d.max_speed)"""
+        nx, ny, nz = self.shape
+        self.smax.zero_()
+        if self.vel_state:
+            _lib.call("nv_max_speed_h", *[self._p(t) for t in self.U[1:]], self._p(self.xi), nx, ny, nz,
+                      _f(self.cn2), _f(self.ci2), self._p(self.smax), self._stream)
+        else:
+            _lib.call("nv_max_speed", *[self._p(t) for t in self.U], self._p(self.xi), nx, ny, nz, _f(self.cn2),
+                      _f(self.ci2), self._p(self.smax), self._stream)
+
     def _compute_dt(self):
         """dt = cfl dx / max(|u|+|v|+|w|+c) (Sim._compute_dt, sim.py:1009-1012), the max read in double."""
         if not self._smax_ok:
-            nx, ny, nz = self.shape
-            self.smax.zero_()
-            if self.vel_state:
-                _lib.call("nv_max_speed_h", *[self._p(t) for t in self.U[1:]], self._p(self.xi), nx, ny, nz,
-                          _f(self.cn2), _f(self.ci2), self._p(self.smax), self._stream)
-            else:
-                _lib.call("nv_max_speed", *[self._p(t) for t in self.U], self._p(self.xi), nx, ny, nz, _f(self.cn2),
-                          _f(self.ci2), self._p(self.smax), self._stream)
+            self._max_speed()
         smax = float(self.smax.item())
         self._smax_ok = False
         return self.P["cfl"] * self.dx / smax
@@ -545,7 +561,7 @@ class Sim:
         solver = S.SOLVERS[self.P.get("solver", "rusanov")]
         return solver, (S.LIMITERS[self.P.get("limiter", "minmod")] if solver == 1 else 0)
 
-    def _stage_desc(self, src, u0, dst, a, b, dt, nstars, wind, pulse, smax_ptr):
+    def _stage_desc(self, src, u0, dst, a, b, dt, nstars, wind, pulse, smax_ptr, dt_dev=None):
         """the arguments of one RK stage (nv_hydro_stage, sim.py:1017-1028) as an NvStageDesc; float fields are
         rounded to float32 as a Warp launch rounds its scalar arguments"""
         P = self.P
@@ -555,7 +571,7 @@ class Sim:
         d.xi = self._p(self.xi)
         d.stars, d.jets = self._p(self.star_arr), self._p(self.jet_arr)
         d.smax_out = smax_ptr
-        d.dt_dev = None
+        d.dt_dev = dt_dev
         d.nx, d.ny, d.nz = self.shape
         d.nstars = nstars
         d.wind, d.jet_cos, d.jet_r, d.pulse = wind, self.C["jet_cos"], P["jet_r"], pulse
@@ -563,10 +579,11 @@ class Sim:
         d.floor_, d.cn2, d.ci2, d.vmax = P["floor"], self.cn2, self.ci2, 6.0
         return d
 
-    def _hydro(self, dt, nstars, wind, pulse):
+    def _hydro(self, dt, nstars, wind, pulse, dt_dev=None):
         """Two RK stages (Sim._hydro, sim.py:1014-1028): U1 = U + dt L(U); U = U/2 + (U1 + dt L(U1))/2 in place.
         Stage 2 also reduces the next step's CFL signal speed into self.smax. The workgroup tile is the measured
-        fastest for this grid (tile="tune", nebulaviewer_amd.autotune) or the one given; no tile changes a bit."""
+        fastest for this grid (tile="tune", nebulaviewer_amd.autotune) or the one given; no tile changes a bit.
+        dt_dev: a device float the stages read their dt from (the device clock), dt then unused."""
         import ctypes
         solver, lim = self._solver_lim()
         if self.tile == "tune":
@@ -578,8 +595,8 @@ class Sim:
         strm = self._stream
         U = [self._p(t) for t in self.U]
         U1 = [self._p(self.stage[c]) for c in range(4)]
-        d1 = self._stage_desc(U, U, U1, 0.0, 1.0, dt, nstars, wind, pulse, None)
-        d2 = self._stage_desc(U1, U, U, 0.5, 0.5, dt, nstars, wind, pulse, self._p(self.smax))
+        d1 = self._stage_desc(U, U, U1, 0.0, 1.0, dt or 0.0, nstars, wind, pulse, None, dt_dev)
+        d2 = self._stage_desc(U1, U, U, 0.5, 0.5, dt or 0.0, nstars, wind, pulse, self._p(self.smax), dt_dev)
         tk = self._t0("hydro_stage")
         _lib.call("nv_hydro_stage_ex", ctypes.byref(d1), code, solver, lim, self.tile_shape[0], strm)
         self._t1(tk)
@@ -605,16 +622,69 @@ class Sim:
         self.it += 1
         self.dt = dt
 
+    def _clock_follows_host(self):
+        """True when host-side state of the main-phase step changes with the clock over the coming steps: a star
+        still to switch on, a luminosity ramp in progress, a pulsed active jet, main-phase stirring, or a dt cap
+        the device clock does not take (<= 0)"""
+        tr = float(self.P.get("q_ramp", 0.0))
+        for i, s in enumerate(self.stars):
+            if self.t < s[4] or (tr > 0.0 and self.t - s[4] < tr):
+                return True
+            if self.jet_of(i) > 0.0 and self.P["jet_pulse"] != 0.0:
+                return True
+        if self.P["drive_main"] > 0 and self.drv is not None:
+            return True
+        return self.dt_cap is not None and not self.dt_cap > 0.0
+
+    def _device_steps(self, k):
+        """k main-phase steps (Sim._main_step, sim.py:1099-1114) with the clock on the device: nv_cfl_dt takes dt
+        from the CFL maximum the previous stage 2 (or the ionize pass) reduced, in the host's double arithmetic
+        (cfl dx / smax, the cap, the float the stages read, t += dt), so no step waits for a readback; t and dt
+        return to the host after the last step. Only for steps whose host-side state does not follow the clock
+        (_clock_follows_host), so the star tables and the jet pulse are those of the first step."""
+        P = self.P
+        c = self._clk
+        if c is None:
+            f64 = dict(dtype=torch.float64, device=self.device)
+            c = self._clk = {"cfl_dx": torch.zeros(1, **f64), "dt_cap": torch.zeros(1, **f64),
+                             "dt_d": torch.zeros(1, **f64), "t": torch.zeros(1, **f64),
+                             "dt_f": torch.zeros(1, dtype=torch.float32, device=self.device)}
+        c["cfl_dx"].fill_(P["cfl"] * self.dx)
+        c["dt_cap"].fill_(self.dt_cap if self.dt_cap is not None else 0.0)
+        c["t"].fill_(self.t)
+        self._set_star_weights()
+        pulse = max(0.0, 1.0 + P["jet_pulse"] * math.sin(2 * math.pi * self.t / P["jet_period"]))
+        strm = self._stream
+        for _ in range(k):
+            if self.it % self.ion_every == 0:
+                self.ionize()
+            if not self._smax_ok:
+                self._max_speed()
+            _lib.call("nv_cfl_dt", self._p(self.smax), c["cfl_dx"].data_ptr(), c["dt_cap"].data_ptr(),
+                      c["dt_f"].data_ptr(), c["dt_d"].data_ptr(), c["t"].data_ptr(), 1, strm)
+            self._smax_ok = False
+            self._hydro(None, len(self.stars), P["wind"], pulse, dt_dev=c["dt_f"].data_ptr())
+            self.it += 1
+            self.version += 1
+        self.prof["device_clock_steps"] = self.prof.get("device_clock_steps", 0) + k
+        self.t = float(c["t"].item())
+        self.dt = float(c["dt_d"].item())
+
     # ------------------------------------------------------------------ public API
     def step(self, k=1):
         """advance k steps. Returns wall seconds."""
         t0 = time.perf_counter()
-        for _ in range(k):
+        done = 0
+        while done < k:
+            if self.phase == "main" and self.device_clock and not self._clock_follows_host():
+                self._device_steps(k - done)
+                break
             if self.phase == "pre":
                 self._pre_step()
             else:
                 self._main_step()
             self.version += 1
+            done += 1
         self._tick("step", t0)
         self.prof["n_steps"] = self.prof.get("n_steps", 0) + k
         return time.perf_counter() - t0

@@ -520,6 +520,40 @@ def test_sim_matches_oracle(env, scenario, n, steps, rays, fuv, bricks):
     np.testing.assert_array_equal(N(sim.g0).view(np.uint16), ref.g0.view(np.uint16), err_msg="g0")
 
 
+@pytest.mark.parametrize("scenario,n,steps,params,device_steps", [
+    ("wall", 48, 8, None, "all"),
+    ("sheet", 32, 7, dict(q_ramp=0.005), "some"),   # host clock while the luminosity ramps, then the device clock
+    ("torus_jet", 32, 5, None, "none"),             # a pulsed jet: the pulse follows the clock, host clock throughout
+])
+def test_sim_device_clock_matches_host_clock(env, scenario, n, steps, params, device_steps):
+    """Sim.step(k) with the clock on the device (nv_cfl_dt, no readback per step) against the host clock
+    (device_clock=False: Sim._compute_dt every step): the same fields, t, dt and it bit for bit, then again with a
+    dt cap, in calls of k steps against k calls of one step"""
+    from nebulaviewer_amd import scenarios as S
+    from nebulaviewer_amd.sim import Sim
+    say(f"[sim clock] {scenario} n={n} steps={steps} params={params}")
+    sims = [Sim(fast=False, setup=S.make_setup(scenario, n, params=params), rays=(32, 64), fuv_grid=(16, 16, 32),
+                verbose=False, device_clock=dc) for dc in (True, False)]
+    dev, host = sims
+    dev.step(steps)
+    host.step(steps)
+    used = dev.prof.get("device_clock_steps", 0)
+    assert {"all": used == steps, "some": 0 < used < steps, "none": used == 0}[device_steps], used
+    assert host.prof.get("device_clock_steps", 0) == 0
+
+    def same():
+        assert dev.t == host.t and dev.dt == host.dt and dev.it == host.it, (dev.t, host.t, dev.dt, host.dt)
+        for name in ("rho", "mx", "my", "mz", "xi", "dfront"):
+            np.testing.assert_array_equal(N(getattr(dev, name)), N(getattr(host, name)), err_msg=name)
+    same()
+    for s in sims:
+        s.dt_cap = 0.9 * host.dt
+    dev.step(3)
+    for _ in range(3):
+        host.step(1)
+    same()
+
+
 def test_sim_unfused_sequence_matches_oracle(env):
     sim, ref = run_pair(env, "sheet", 24, 5, (32, 64), (16, 16, 32), fused=False)
     assert_same_state(sim, ref)
