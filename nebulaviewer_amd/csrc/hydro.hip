@@ -342,16 +342,19 @@ struct Win {
   float x, c2, c;
 };
 
-// Which stage forms load the halo of the next plane one plane ahead (four more registers per thread): stage 1;
-// stage 2 already holds U0 and the CFL max and would spill at its 128-register limit.
-template <bool STAGE1>
+// Which stage forms load the halo of the next plane one plane ahead (four more registers per thread): stage 1; and
+// with QUEUE (the batch's persistent tile-queue kernel), stage 2 in the fast float32 form too (123 registers). The
+// exact and binary16-velocity forms' stage 2, which also hold U0 and the CFL max, would spill at their 128-register
+// limit; the single-grid fast stage 2 fits but measured slower with it (1.52 -> 1.60 ms at 512^3), the batch's
+// queue kernel faster (1.07 -> 1.02 ms per launch).
+template <bool STAGE1, bool QUEUE, class ST, class AR>
 struct HALO_PREFETCH {
-  static constexpr bool value = STAGE1;
+  static constexpr bool value = STAGE1 || (QUEUE && AR::FAST && !ST::VEL);
 };
 
 // One tile (64 z x 8 y columns, marching along x over ci planes) of one RK stage.
 // STAGE1: a = 0, b = 1 and U0 = U (sim.py:1025), so out = 0*U0 + 1*Un is Un (up to the sign of zero).
-template <class TL, class ST, class AR, int SOLVER, int LIM, bool SMAX, bool STAGE1>
+template <class TL, class ST, class AR, int SOLVER, int LIM, bool SMAX, bool STAGE1, bool QUEUE = false>
 __device__ __forceinline__ void stage_tile(const HydroArgs& A, int tile) {
   constexpr int TK = TL::TK, TJ = TL::TJ, PK = TL::PK, PJ = TL::PJ, CK = TL::CK, CJ = TL::CJ;
   constexpr int NHALO = TL::NHALO, NCHALO = TL::NCHALO, NT = TL::NT;
@@ -478,7 +481,7 @@ __device__ __forceinline__ void stage_tile(const HydroArgs& A, int tile) {
   // on the first NCHALO): with PF it is loaded into registers one plane ahead, so its latency is not exposed
   // between the start of a plane and its first barrier.
   static_assert(NHALO <= NT && NCHALO <= NT && NCHALO <= NT - NHALO, "one halo cell per thread");
-  constexpr bool PF = HALO_PREFETCH<STAGE1>::value;
+  constexpr bool PF = HALO_PREFETCH<STAGE1, QUEUE, ST, AR>::value;
   const bool hq = tid >= NT - NHALO, hc = tid < NCHALO;
   auto halo_load = [&](int p, float4& raw) {
     const long long pb = (long long)p * plane;
@@ -832,7 +835,7 @@ __global__ __launch_bounds__(TL::NT, 4) void hydro_stage_queue_kernel(const Hydr
     __syncthreads();                  // every thread has read the claim before the lead writes the next one
     if (t < 0) break;
     const int lo = member_of(start, n, t);
-    stage_tile<TL, ST, AR, SOLVER, LIM, SMAX, STAGE1>(args[lo], t - start[lo]);
+    stage_tile<TL, ST, AR, SOLVER, LIM, SMAX, STAGE1, true>(args[lo], t - start[lo]);
   }
   if (lead) {
     __threadfence();
