@@ -295,8 +295,9 @@ int nv_ionize_apply(const float* rfronts, int nu, int nphi, const float* star_xy
 /* Forms of the fused ionize pass (bits): NV_ION_VEL takes the CFL maximum over the binary16-velocity state (as
  * nv_ionize_apply_h); NV_ION_FAST evaluates each cell in fast FP32 arithmetic (the hardware square root and
  * reciprocal for the cell's distance, direction, atan2 quotient and velocities; float products for the divisions
- * by 2 pi, 6 and the ramp width): xi, dfront and the CFL maximum within a few ulp of the exact pass, for the
- * fast precision forms (NV_PREC_F32_FAST, NV_PREC_F16V), not bit-identical to the reference. */
+ * by 2 pi, 6 and the ramp width): xi, dfront and the CFL maximum as close to an FP64 evaluation as the exact
+ * pass's (tested on the same fronts), not bit-identical to the reference; for the fast precision forms
+ * (NV_PREC_F32_FAST, NV_PREC_F16V). */
 #define NV_ION_VEL 1
 #define NV_ION_FAST 2
 
