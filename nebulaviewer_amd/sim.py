@@ -55,7 +55,8 @@ class Sim:
 
     def __init__(self, scenario="torus", res=None, params=None, ic_mode="nebula3d", rays=None, fuv_grid=None,
                  ion_every=None, device="cuda:0", verbose=True, fast=True, setup=None, fused=True, seed=None,
-                 precision="fp32", long_rays="fp64", tile="tune", ray_bricks="auto", device_clock=True):
+                 precision="fp32", long_rays="fp64", tile="tune", ray_bricks="auto", device_clock=True,
+                 cull_stars=True):
         # the reference's constructor (sim.py:553-565): positional order up to fast, fast=True as the viewer runs it
         # (1-voxel rays, ionization every >= 3 steps, sim.py:743, 758); bake.py and the benchmark pass fast=False
         if ic_mode not in self.IC_MODES:
@@ -84,6 +85,12 @@ class Sim:
         # no readback per step, the same bits (False: the host computes every dt, sim.py:1009-1012)
         self.device_clock = bool(device_clock)
         self._clk = None
+        # two or more switched-on stars: the casts record each star's front range and the fused ionize pass skips,
+        # per cell, the stars that provably change neither xi nor dfront (the same bits either way)
+        self.cull_stars = bool(cull_stars)
+        self._mplans = None
+        self._mplan_key = None
+        self._dirs_tab = None
         self.device = torch.device(device)
         self.fast = fast
         self.verbose = verbose
@@ -383,6 +390,58 @@ class Sim:
         jw += [(0.0, 0.0, 1.0, 0.0)] * (self.cap - len(jw))
         self.jet_arr.copy_(torch.tensor(np.array(jw, dtype=np.float32)))
 
+    def _culled_plans(self, on, use_bricks, width):
+        """(ray plan, ionize plan) of the batch entry points for this grid's switched-on stars: one ray job per star
+        (its front table, the shared direction table, its front range) and the fused ionize pass over the grid with
+        the stars' front ranges, so that a cell skips the stars that provably change neither xi nor dfront. The
+        casts are cast_one on the same rays and the pass the same per-cell arithmetic, so xi, dfront and the CFL
+        maximum are those of the per-star casts and nv_ionize_apply_ex bit for bit. Rebuilt when a star, its
+        luminosity, the brick choice or the grid's fields change."""
+        from .batch import _Plan
+        nx, ny, nz = self.shape
+        nu, nphi = self.rfronts.shape[1:]
+        key = (tuple(on), tuple(self.q_eff(i) for i in on), tuple(tuple(self.stars[i][:3]) for i in on), use_bricks,
+               self.ion_form, self.generation, self.long_rays, width, self.rfronts.data_ptr(), self.star_xyz.data_ptr())
+        if key == self._mplan_key:
+            return self._mplans
+        L = _lib.load()
+        if self._dirs_tab is None or self._dirs_tab.numel() != 4 * nu * nphi:
+            self._dirs_tab = torch.empty(4 * nu * nphi, dtype=torch.float32, device=self.device)
+            _lib.call("nv_ray_dirs", nu, nphi, self._dirs_tab.data_ptr(), self._stream)
+        per = int(L.nv_ray_range_floats(nu, nphi))
+        ranges = torch.zeros(per * len(on), dtype=torch.float32, device=self.device)
+        Lx, Ly, Lz = self.bmax
+        jobs = (_lib.NvRayDesc * len(on))()
+        for slot, i in enumerate(on):
+            j = jobs[slot]
+            j.rho = self.U[0].data_ptr()
+            j.bricks = self._brick_buf.data_ptr() if use_bricks else None
+            j.nx, j.ny, j.nz = nx, ny, nz
+            j.sx, j.sy, j.sz = self.stars[i][:3]
+            j.Q = self.q_eff(i)
+            j.dr, j.nsteps, j.dx, j.rcap = self.dr, self.nsteps, self.dx, self.diag
+            j.bx, j.by, j.bz = Lx, Ly, Lz
+            j.rfront = self.rfronts[slot].data_ptr()
+            j.nu, j.nphi = nu, nphi
+            j.wide = 1 if self.long_rays == "fp64" else 0
+            j.dirs = self._dirs_tab.data_ptr()
+            j.rf_range = ranges.data_ptr() + 4 * per * slot
+        ion = (_lib.NvIonDesc * 1)()
+        ion[0] = self.ion_desc(len(on), width)
+        ion[0].rf_ranges = ranges.data_ptr()
+        P = _lib.P
+        form = self.ion_form
+
+        def build_rays(buf, nbytes):
+            _lib.call("nv_cast_rays_batch_plan", ctypes.cast(jobs, P), len(on), buf, nbytes)
+
+        def build_ion(buf, nbytes):
+            _lib.call("nv_ionize_apply_batch_plan", ctypes.cast(ion, P), 1, form, buf, nbytes)
+        self._mplans = (_Plan("nv_cast_rays_batch_plan_size", build_rays, len(on), self.device),
+                        _Plan("nv_ionize_apply_batch_plan_size", build_ion, 1, self.device), ranges)
+        self._mplan_key = key
+        return self._mplans
+
     def _upload_xyz(self, on):
         """star_xyz[:len(on)] = the positions of the switched-on stars; the copy (from pageable host memory, which
         waits for the stream) only when they changed"""
@@ -451,16 +510,28 @@ class Sim:
             from . import autotune
             use = autotune.tune_rays(self, len(on), casts)
         self.ray_bricks_used = bool(use)
-        casts(bool(use), timed=True)
         width = 1.2 * self.dx
         half = self.dfront.dtype == torch.float16
+        plans = self._culled_plans(on, bool(use), width) if self.fused and self.cull_stars and len(on) >= 2 else None
+        if plans is None:
+            casts(bool(use), timed=True)
         if self.fused:
             self._upload_xyz(on)
             self.smax.zero_()
-            d = self.ion_desc(len(on), width)
-            tk = self._t0("ionize_apply")
-            _lib.call("nv_ionize_apply_ex", ctypes.byref(d), self.ion_form, strm)
-            self._t1(tk)
+            if plans is not None:
+                # the batch entry points with this one grid: every star's casts in one launch (the brick build
+                # inside it), each recording its front range, then the culled pass
+                tk = self._t0("cast_rays")
+                _lib.call("nv_cast_rays_batch", *plans[0].args, strm)
+                self._t1(tk)
+                tk = self._t0("ionize_apply")
+                _lib.call("nv_ionize_apply_batch", *plans[1].args, strm)
+                self._t1(tk)
+            else:
+                d = self.ion_desc(len(on), width)
+                tk = self._t0("ionize_apply")
+                _lib.call("nv_ionize_apply_ex", ctypes.byref(d), self.ion_form, strm)
+                self._t1(tk)
             self._smax_ok = True
         else:
             # the reference launch sequence, kernel by kernel (sim.py:930-943)
