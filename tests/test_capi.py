@@ -4,6 +4,7 @@ import ctypes
 import os
 import re
 
+import numpy as np
 import pytest
 
 from nebulaviewer_amd import _lib
@@ -131,6 +132,40 @@ def test_batch_plans_validate(lib):
     assert lib.nv_hydro_stage_batch(buf, buf, None) == 1001     # a refused plan never got its header
     assert lib.nv_hydro_stage_batch_queue_ex(buf, buf, 0, None) == 1001
     assert lib.nv_hydro_stage_batch_queue_ex(buf, buf, -1, None) == 1001 and b"wgs_per_cu" in lib.nv_last_error()
+
+
+def _ion_plan_cuts(lib, specs):
+    """build an ionize batch plan on the host for members (n, nstars, culled) and return (start, xoff, xcd_run)"""
+    n = len(specs)
+    ions = (_lib.NvIonDesc * n)()
+    for d, (g, k, culled) in zip(ions, specs):
+        d.rfronts, d.star_xyz, d.xi, d.dfront_h = 1 << 20, 1 << 21, 1 << 22, 1 << 23   # never dereferenced here
+        d.nu, d.nphi, d.nstars, d.dx, d.width = 8, 16, k, 1.0 / g, 1.2 / g
+        d.nx = d.ny = d.nz = g
+        d.rf_ranges = (1 << 24) if culled else None
+    size = int(lib.nv_ionize_apply_batch_plan_size(n))
+    buf = ctypes.create_string_buffer(size)
+    assert lib.nv_ionize_apply_batch_plan(ions, n, 2, buf, size) == 0, lib.nv_last_error()
+    ints = np.frombuffer(buf.raw, np.int32)
+    xoff = ints[-9:]
+    start = ints[-9 - (n + 1):-9]
+    return start, xoff, int(ints[9])
+
+
+def test_ionize_batch_plan_xcd_cuts(lib):
+    """the batch ionize pass gives XCD x the contiguous tile run [xoff[x], xoff[x+1]): the cuts cover every tile once,
+    in order, and split the work (tiles weighted by the member's star count) evenly"""
+    rng = np.random.default_rng(5)
+    specs = [(int(rng.choice([48, 64, 96, 128, 160])), int(rng.integers(1, 6)), True) for _ in range(64)]
+    for case in (specs, specs[:1], [(512, 1, False)], [(64, 3, False), (48, 1, False), (160, 4, True)]):
+        start, xoff, run = _ion_plan_cuts(lib, case)
+        assert start[0] == 0 and xoff[0] == 0 and xoff[8] == start[-1]
+        assert np.all(np.diff(xoff) >= 0) and run == np.diff(xoff).max()
+        w = np.zeros(start[-1])
+        for s, (g, k, culled) in enumerate(case):
+            w[start[s]:start[s + 1]] = 1.0 if k <= 1 else 1.0 + (0.25 if culled else 0.8) * (k - 1)
+        per = np.array([w[xoff[x]:xoff[x + 1]].sum() for x in range(8)])
+        assert per.max() - per.mean() <= w.max() + 1e-9, (per, w.max())
 
 
 def test_rejects_null_fields(lib):
