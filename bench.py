@@ -18,8 +18,9 @@ read for the fused CFL max).
 cpu_baseline: the C++ oracle (oracle/, a restatement of the reference kernels; the reference's Warp CPU device
 cannot run here) on the same 512^3 grid: one ionization and one hydro step on all host cores.
 variants: the same workload in the fast-arithmetic and binary16-velocity forms, and the uneven batch (64
-independent members of 48^3..160^3, scenarios.batch_setups) stepped by SimBatch with HIP-graph replay, with its
-throughput relative to the single 512^3 grid (full step and hydro only).
+independent members of 48^3..160^3, scenarios.batch_setups) stepped by SimBatch with HIP-graph replay, in the
+headline's form ("batch") and in the binary16-velocity form ("batch_fp16-vel"), each with its throughput relative
+to the single 512^3 grid of the same form (full step and hydro only).
 """
 import argparse
 import json
@@ -205,12 +206,17 @@ def make_measure_batch(a, S, KernelTimer):
     import torch
     from nebulaviewer_amd import autotune
 
-    def measure_batch(count):
+    made = {}
+
+    def measure_batch(count, precision=None):
         from nebulaviewer_amd.batch import SimBatch
-        t0 = time.perf_counter()
-        setups = S.batch_setups(count)
-        log(f"[bench] batch of {count}: initial conditions in {time.perf_counter() - t0:.1f} s")
-        b = SimBatch(setups, precision=a.precision, graph=True)
+        precision = precision or a.precision
+        if count not in made:
+            t0 = time.perf_counter()
+            made[count] = S.batch_setups(count)
+            log(f"[bench] batch of {count}: initial conditions in {time.perf_counter() - t0:.1f} s")
+        setups = made[count]
+        b = SimBatch(setups, precision=precision, graph=True)
         cells = b.cells
         b.step(a.warmup)                       # captures the two kinds of step
         torch.cuda.synchronize()
@@ -229,13 +235,13 @@ def make_measure_batch(a, S, KernelTimer):
                  sizes=sorted({m.shape[0] for m in b.members}), value=cells * a.steps / wall,
                  ms_per_step=wall / a.steps * 1e3, hydro_launch_ms=hyd_s * 1e3,
                  hydro_only_cell_updates_per_s=cells / (2 * hyd_s),
-                 hydro_achieved_GBps=BYTES_PER_CELL_STEP[a.precision] / 2 * cells / hyd_s / 1e9,
+                 hydro_achieved_GBps=BYTES_PER_CELL_STEP[precision] / 2 * cells / hyd_s / 1e9, precision=precision,
                  kernels_ms={k: round(v["avg_ms"], 4) for k, v in ks.items()}, graph_replay=True,
                  tile={f"stage{k}": autotune.tile_name(t) for k, t in b.tile.items()},
                  ray_bricks=getattr(b, "ray_bricks_used", None),
                  dispatch={f"stage{k}": (f"queue, {b.queue_wgs[k] or 'all'} workgroups/CU" if v else "hardware")
                            for k, v in b.dispatch.items()})
-        log(f"[bench] batch: {r['value'] / 1e9:.2f} G cell-updates/s ({cells / 1e6:.1f} M cells, {r['stars']} stars), "
+        log(f"[bench] batch ({precision}): {r['value'] / 1e9:.2f} G cell-updates/s ({cells / 1e6:.1f} M cells, {r['stars']} stars), "
             f"{r['ms_per_step']:.3f} ms/step, hydro {hyd_s * 1e3:.3f} ms/launch; kernels {r['kernels_ms']}")
         del b
         torch.cuda.empty_cache()
@@ -351,12 +357,17 @@ def main():
             if p != a.precision:
                 variants[p] = measure(p)
         if a.batch > 0:
-            br = measure_batch(a.batch)
-            ref = main_r
-            br["vs_single_grid"] = {"step": br["value"] / ref["value"],
-                                    "hydro_only": br["hydro_only_cell_updates_per_s"] /
-                                    ref["hydro_only_cell_updates_per_s"]}
-            variants["batch"] = br
+            # the batch in the headline's precision and in the reduced-storage form, each against the single grid
+            # of the same form
+            for p, key in ((a.precision, "batch"), ("fp16-vel", "batch_fp16-vel")):
+                if key != "batch" and p == a.precision:
+                    continue
+                br = measure_batch(a.batch, p)
+                ref = main_r if p == a.precision else variants[p]
+                br["vs_single_grid"] = {"step": br["value"] / ref["value"],
+                                        "hydro_only": br["hydro_only_cell_updates_per_s"] /
+                                        ref["hydro_only_cell_updates_per_s"]}
+                variants[key] = br
     cells = math.prod(setup.shape)
     value = main_r["value"]
     achieved = main_r["hydro_achieved_GBps"]
