@@ -333,14 +333,17 @@ def main():
         achieved = BYTES_PER_CELL_STEP[precision] / 2 * cells / hyd_s / 1e9
         r = dict(value=cells * a.steps / wall, ms_per_step=wall / a.steps * 1e3, hydro_launch_ms=hyd_s * 1e3,
                  hydro_only_cell_updates_per_s=cells / (2 * hyd_s), hydro_achieved_GBps=achieved,
-                 kernels_ms={k: round(v["avg_ms"], 4) for k, v in ks.items()}, ion_every=sim.ion_every,
+                 kernels_ms={k: round(v["avg_ms"], 4) for k, v in ks.items()},
+                 kernels_min_max_ms={k: (round(v["min_ms"], 4), round(v["max_ms"], 4)) for k, v in ks.items()},
+                 ion_every=sim.ion_every,
                  rays=tuple(sim.rfronts.shape[1:]), nstars=len(sim.stars),
                  tile={f"stage{k + 1}": autotune.tile_name(t) for k, t in enumerate(sim.tile_shape)},
                  ray_bricks=getattr(sim, "ray_bricks_used", None),
                  ionization_roofline=ionization_roofline(ks, cells, sum(1 for st in sim.stars if sim.t >= st[4]),
                                                          precision))
         log(f"[bench] {precision}: {r['value'] / 1e9:.2f} G cell-updates/s, {r['ms_per_step']:.3f} ms/step, "
-            f"hydro {hyd_s * 1e3:.3f} ms/launch = {achieved:.0f} GB/s; kernels {r['kernels_ms']}")
+            f"hydro {hyd_s * 1e3:.3f} ms/launch = {achieved:.0f} GB/s; kernels {r['kernels_ms']}, "
+            f"(least, greatest) {r['kernels_min_max_ms']}")
         if precision == "fp32":
             r["mass_budget"] = mass_budget(sim)
             log(f"[bench] {precision}: mass budget of one more step {r['mass_budget']}")

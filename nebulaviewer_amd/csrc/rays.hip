@@ -1070,10 +1070,17 @@ extern "C" int nv_fill_f32(float* dst, float v, long long n, void* stream) {
   return check_launch("nv_fill_f32");
 }
 
+// Planes an ionize tile marches at most: a tile recomputes the unsmoothed xi of two planes beyond its chunk, and
+// enough tiles keep every CU busy to the end. The 512^3 wall's pass is fastest with chunks of 64 (1.00 ms against
+// 1.01 at 32 and 1.05 at 16). The 64-member batch is fastest with 16 (1.37-1.39 ms against 1.48-1.49 at 64, 1.40 at
+// 32 and 12, 1.42 at 8): more, shorter tiles even out the runs of its XCDs. Neither choice changes a bit.
+constexpr int kIonDepth = 64;
+constexpr int kIonBatchDepth = 16;
+
 static int fill_ionize(const char* name, const float* rfronts, int nu, int nphi, const float* star_xyz, int nstars,
                        float dx, float width, float* xi, uint16_t* dfront_h, float* dfront_f, int nx, int ny, int nz,
                        const void* f0, const void* f1, const void* f2, const void* f3, float cn2, float ci2,
-                       float* smax_out, long long jk_total, IonArgs& A) {
+                       float* smax_out, long long jk_total, int max_ci, IonArgs& A) {
   NV_REQUIRE(xi && (nstars == 0 || (rfronts && star_xyz)), "%s: null pointer", name);
   NV_REQUIRE((dfront_h != nullptr) != (dfront_f != nullptr), "%s: pass exactly one dfront buffer", name);
   NV_REQUIRE(nx > 0 && ny > 0 && nz > 0 && nu > 0 && nphi > 0 && nstars >= 0, "%s: bad sizes", name);
@@ -1086,7 +1093,7 @@ static int fill_ionize(const char* name, const float* rfronts, int nu, int nphi,
   A.rwidth = Recip{1.0 / (double)width, width};
   A.rw = 1.0f / width;
   A.rf_ranges = nullptr;
-  A.ci = (int)std::min<long long>(64, std::max<long long>(4, (long long)nx * jk_total / 2048));
+  A.ci = (int)std::min<long long>(max_ci, std::max<long long>(4, (long long)nx * jk_total / 2048));
   A.tiles_k = (nz + IK - 1) / IK;
   A.tiles_j = (ny + IJ - 1) / IJ;
   A.tiles = A.tiles_k * A.tiles_j * ((nx + A.ci - 1) / A.ci);
@@ -1110,7 +1117,7 @@ static int ionize_apply_launch(const char* name, int form, const float* rfronts,
   NV_REQUIRE(form >= 0 && form <= (NV_ION_VEL | NV_ION_FAST), "%s: bad form %d", name, form);
   IonArgs A;
   const int rc = fill_ionize(name, rfronts, nu, nphi, star_xyz, nstars, dx, width, xi, dfront_h, dfront_f, nx, ny, nz,
-                             f0, f1, f2, f3, cn2, ci2, smax_out, ion_jk(ny, nz), A);
+                             f0, f1, f2, f3, cn2, ci2, smax_out, ion_jk(ny, nz), kIonDepth, A);
   if (rc != NV_OK) return rc;
   const int per_xcd = (A.tiles + 7) / 8;
   const bool vel = (form & NV_ION_VEL) != 0;
@@ -1397,7 +1404,7 @@ extern "C" int nv_ionize_apply_batch_plan(const NvIonDesc* members, int n, int f
     IonArgs A;
     const int rc = fill_ionize(name, d.rfronts, d.nu, d.nphi, d.star_xyz, d.nstars, d.dx, d.width, d.xi, d.dfront_h,
                                d.dfront_f, d.nx, d.ny, d.nz, d.state[0], d.state[1], d.state[2], d.state[3], d.cn2,
-                               d.ci2, d.smax_out, jk_total, A);
+                               d.ci2, d.smax_out, jk_total, kIonBatchDepth, A);
     if (rc != NV_OK) return rc;
     A.rf_ranges = d.rf_ranges;
     ranges = ranges || d.rf_ranges != nullptr;

@@ -30,13 +30,16 @@ class KernelTimer:
         self.pairs.append((name, s, e))
 
     def summary(self):
-        """{name: dict(count, total_ms, avg_ms)}; synchronises the device."""
+        """{name: dict(count, total_ms, avg_ms, min_ms, max_ms)}; synchronises the device."""
         torch.cuda.synchronize(self.device)
         out = {}
         for name, s, e in self.pairs:
-            d = out.setdefault(name, dict(count=0, total_ms=0.0))
+            d = out.setdefault(name, dict(count=0, total_ms=0.0, min_ms=float("inf"), max_ms=0.0))
+            ms = s.elapsed_time(e)
             d["count"] += 1
-            d["total_ms"] += s.elapsed_time(e)
+            d["total_ms"] += ms
+            d["min_ms"] = min(d["min_ms"], ms)
+            d["max_ms"] = max(d["max_ms"], ms)
         for d in out.values():
             d["avg_ms"] = d["total_ms"] / max(d["count"], 1)
         return out

@@ -168,6 +168,22 @@ def test_ionize_batch_plan_xcd_cuts(lib):
         assert per.max() - per.mean() <= w.max() + 1e-9, (per, w.max())
 
 
+def test_ionize_batch_plan_chunk_depth(lib):
+    """a batch member's ionize tiles march at most 16 x-planes (32 z x 16 y cells each), the single-grid pass up to
+    64: the plan's tile prefix counts ceil(n/32) ceil(n/16) ceil(n/ci) tiles per member with
+    ci = min(16, max(4, n jk_total / 2048)), jk_total the batch's (z, y) tile columns"""
+    rng = np.random.default_rng(6)
+    for specs in ([(int(rng.choice([48, 64, 96, 128, 160])), 1, False) for _ in range(64)], [(512, 1, False)],
+                  [(256, 4, True)], [(24, 2, True), (40, 1, False)]):
+        start, _, _ = _ion_plan_cuts(lib, specs)
+        jk = [-(-g // 32) * -(-g // 16) for g, _, _ in specs]
+        want = []
+        for (g, _, _), c in zip(specs, jk):
+            ci = min(16, max(4, g * sum(jk) // 2048))
+            want.append(c * -(-g // ci))
+        np.testing.assert_array_equal(np.diff(start), want)
+
+
 def test_rejects_null_fields(lib):
     f = ctypes.c_float
     args = [None] * 13 + [8, 8, 8, None, 0, f(0), None] + [f(1)] * 11 + [1, 0, None, None]
