@@ -23,6 +23,8 @@ headline's form ("batch") and in the binary16-velocity form ("batch_fp16-vel"), 
 to the single 512^3 grid of the same form (full step and hydro only).
 """
 import argparse
+import contextlib
+import gc
 import json
 import math
 import os
@@ -42,6 +44,23 @@ HEADLINE = "fp32-fast"
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+@contextlib.contextmanager
+def no_gc():
+    """Python's cyclic garbage collector collected before the timed steps and paused during them, as timeit does:
+    a full collection in a process holding torch takes tens of milliseconds on the host, and one landing between
+    two launches leaves the device idle. Before this, three of five runs had one ray-cast window of the first form's
+    50 steps stretched from 1.2 ms to 7-23 ms, with every other launch at its usual time. Three runs with the
+    collector paused had none."""
+    gc.collect()
+    was = gc.isenabled()
+    gc.disable()
+    try:
+        yield
+    finally:
+        if was:
+            gc.enable()
 
 
 def measured_traffic(precision):
@@ -220,10 +239,11 @@ def make_measure_batch(a, S, KernelTimer):
         cells = b.cells
         b.step(a.warmup)                       # captures the two kinds of step
         torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        b.step(a.steps)
-        torch.cuda.synchronize()
-        wall = time.perf_counter() - t0
+        with no_gc():
+            t0 = time.perf_counter()
+            b.step(a.steps)
+            torch.cuda.synchronize()
+            wall = time.perf_counter() - t0
         timer = KernelTimer(reserve=32 * b.ion_every)
         b.timer = timer                        # per-kernel events (eager launches, no graph)
         b.step(2 * b.ion_every)
@@ -322,10 +342,11 @@ def main():
         timer = KernelTimer(reserve=16 * a.steps)   # every event of the timed steps created before them
         sim.timer = timer
         torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        sim.step(a.steps)
-        torch.cuda.synchronize()
-        wall = time.perf_counter() - t0
+        with no_gc():
+            t0 = time.perf_counter()
+            sim.step(a.steps)
+            torch.cuda.synchronize()
+            wall = time.perf_counter() - t0
         sim.timer = None
         ks = timer.summary()
         cells = math.prod(sim.shape)

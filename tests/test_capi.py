@@ -184,6 +184,35 @@ def test_ionize_batch_plan_chunk_depth(lib):
         np.testing.assert_array_equal(np.diff(start), want)
 
 
+def test_stage_batch_plan_march_depth(lib):
+    """with the library's march depth, a batch member's stage tiles march at most 32 x-planes (a single grid's up to
+    64): the plan's tile prefix counts ceil(n/32) ceil(n/16) ceil(n/ci) tiles of 32 z x 16 y per member with
+    ci = min(32, max(4, n jk_total / 2048)); a depth in the tile code (NV_TILE_DEPTH) is taken as given"""
+    rng = np.random.default_rng(7)
+    for sizes, depth in (([int(rng.choice([48, 64, 96, 128, 160])) for _ in range(64)], 0), ([512], 0),
+                         ([40, 24], 0), ([96, 160, 48], 16)):
+        n = len(sizes)
+        ds = (_lib.NvStageDesc * n)()
+        for d, g in zip(ds, sizes):
+            for c in range(4):
+                d.u[c], d.u0[c], d.out[c] = (1 << 20) + c, (1 << 21) + c, (1 << 22) + c   # never dereferenced here
+            d.xi = 1 << 23
+            d.nx = d.ny = d.nz = g
+            d.dt, d.dx, d.a, d.b, d.floor_, d.cn2, d.ci2, d.vmax = 1e-4, 1.0 / g, 0.0, 1.0, 1e-3, 1.0, 1.0, 6.0
+        size = int(lib.nv_hydro_stage_batch_plan_size(n))
+        buf = ctypes.create_string_buffer(size)
+        assert lib.nv_hydro_stage_batch_plan(ds, n, 1, 1, 0, 2 | (depth << 8), buf, size) == 0, lib.nv_last_error()
+        ints = np.frombuffer(buf.raw, np.int32)
+        start = ints[-(16 + n + 1):-16]
+        jk = [-(-g // 32) * -(-g // 16) for g in sizes]
+        want = []
+        for g, c in zip(sizes, jk):
+            ci = min(depth, g) if depth else min(32, max(4, g * sum(jk) // 2048))
+            want.append(c * -(-g // ci))
+        assert start[0] == 0 and ints[7] == start[-1]
+        np.testing.assert_array_equal(np.diff(start), want)
+
+
 def test_rejects_null_fields(lib):
     f = ctypes.c_float
     args = [None] * 13 + [8, 8, 8, None, 0, f(0), None] + [f(1)] * 11 + [1, 0, None, None]
